@@ -1,0 +1,194 @@
+/*
+ * msplat_amd.h -- C ABI of the MI355X (gfx950) Gaussian-splatting rasterizer.
+ *
+ * Every entry point takes plain device pointers, element counts and a
+ * hipStream_t (passed as void*), launches its kernels on that stream and
+ * returns 0 on success or a negative msplat_status code.  The library never
+ * allocates device memory: outputs and workspaces belong to the caller.
+ * Outputs are written in full (culled / skipped elements get zeros, matching
+ * the zero-filled tensors the reference returns), so callers need no memset.
+ *
+ * Layouts (row-major, fp32 unless noted):
+ *   xyz [n,3]  scale [n,3]  rotate [n,4] quaternion (w,x,y,z), used unnormalised
+ *   opacity [n]  feature [n,C]  intr [4] = fx,fy,cx,cy
+ *   extr: the first 12 floats are read as the row-major 3x4 [R|t] (a 4x4 works)
+ *   uv [n,2]  depth [n]  cov3d [n,6] = upper triangle 00,01,02,11,12,22
+ *   conic [n,3] = inverse 2D covariance a,b,c  radius/tiles [n] int32
+ *   idx_sorted [K] int32  tile_range [T,2] int32, T = ceil(W/16)*ceil(H/16)
+ *   rendered [C,H,W]  final_T [H,W]  ncontrib [H,W] int32
+ *   visible: one byte per Gaussian (torch.bool), may be NULL = all visible
+ *
+ * Reference files are cited relative to the msplat source tree.
+ */
+#ifndef MSPLAT_AMD_H
+#define MSPLAT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum msplat_status {
+  MSPLAT_OK = 0,
+  MSPLAT_ERR_ARGUMENT = -1,  /* bad size / null pointer / unsupported value */
+  MSPLAT_ERR_LAUNCH = -2,    /* a HIP launch or runtime call failed */
+  MSPLAT_ERR_WORKSPACE = -3, /* workspace smaller than the size query asked for */
+  MSPLAT_ERR_CAPACITY = -4   /* intersection count does not fit int32 */
+};
+
+/* Library version (major*10000 + minor*100 + patch). No reference counterpart. */
+int msplat_version(void);
+
+/* Message for the last non-zero status returned on the calling thread.
+ * Replaces the TORCH_CHECK text of msplat/include/utils.h:9-10. */
+const char *msplat_last_error(void);
+
+/* ---- projection: msplat/src/project_point.cu:147-179 (_C.project_point_forward,
+ *      bound at msplat/src/ext.cpp:14, called from msplat/project_point.py:59) */
+int msplat_project_point_forward(const float *xyz, int64_t n, const float *intr,
+                                 const float *extr, int W, int H, float nearest,
+                                 float extent, float *uv, float *depth,
+                                 void *stream);
+
+/* project_point.cu:181-228 (_C.project_point_backward, ext.cpp:15,
+ * project_point.py:77).  dL_ddepth may be NULL (= zeros).  dL_dintr [4] and
+ * dL_dextr [12] may be NULL (= not requested, project_point.cu:202-208);
+ * when given they are overwritten with the reduced camera gradient. */
+int msplat_project_point_backward(const float *xyz, int64_t n,
+                                  const float *intr, const float *extr,
+                                  const float *depth, const float *dL_duv,
+                                  const float *dL_ddepth, float *dL_dxyz,
+                                  float *dL_dintr, float *dL_dextr,
+                                  void *stream);
+
+/* ---- 3D covariance: msplat/src/compute_cov3d.cu:149-170 (ext.cpp:16,
+ *      compute_cov3d.py:39) */
+int msplat_compute_cov3d_forward(const float *scale, const float *rotate,
+                                 const uint8_t *visible, int64_t n,
+                                 float *cov3d, void *stream);
+
+/* compute_cov3d.cu:172-199 (ext.cpp:17, compute_cov3d.py:51) */
+int msplat_compute_cov3d_backward(const float *scale, const float *rotate,
+                                  const uint8_t *visible,
+                                  const float *dL_dcov3d, int64_t n,
+                                  float *dL_dscale, float *dL_drotate,
+                                  void *stream);
+
+/* ---- EWA splatting: msplat/src/ewa_project.cu:254-297 (ext.cpp:18,
+ *      ewa_project.py:59) */
+int msplat_ewa_project_forward(const float *xyz, const float *cov3d,
+                               const float *intr, const float *extr,
+                               const float *uv, const uint8_t *visible,
+                               int64_t n, int W, int H, float *conic,
+                               int32_t *radius, int32_t *tiles, void *stream);
+
+/* ewa_project.cu:299-345 (ext.cpp:19, ewa_project.py:71).  Camera gradients
+ * as in msplat_project_point_backward. */
+int msplat_ewa_project_backward(const float *xyz, const float *cov3d,
+                                const float *intr, const float *extr,
+                                const int32_t *radius, const float *dL_dconic,
+                                int64_t n, float *dL_dxyz, float *dL_dcov3d,
+                                float *dL_dintr, float *dL_dextr, void *stream);
+
+/* ---- fused projection + cov3d + EWA.  Same outputs as the chain
+ * project_point -> compute_cov3d(visible = depth != 0) -> ewa_project used by
+ * msplat/__init__.py:71-81, in one pass over the Gaussians. */
+int msplat_preprocess_forward(const float *xyz, const float *scale,
+                              const float *rotate, int64_t n,
+                              const float *intr, const float *extr, int W,
+                              int H, float nearest, float extent, float *uv,
+                              float *depth, float *conic, int32_t *radius,
+                              int32_t *tiles, void *stream);
+
+/* Adjoint of msplat_preprocess_forward: the sum of the three reference
+ * backward kernels (project_point.cu:59-145, compute_cov3d.cu:131-147,
+ * ewa_project.cu:85-252) as autograd composes them.  dL_ddepth may be NULL. */
+int msplat_preprocess_backward(const float *xyz, const float *scale,
+                               const float *rotate, int64_t n,
+                               const float *intr, const float *extr,
+                               const float *depth, const int32_t *radius,
+                               const float *dL_duv, const float *dL_ddepth,
+                               const float *dL_dconic, float *dL_dxyz,
+                               float *dL_dscale, float *dL_drotate,
+                               float *dL_dintr, float *dL_dextr, void *stream);
+
+/* ---- spherical harmonics, degree 0..10: msplat/src/compute_sh.cu:1696-1722
+ *      (ext.cpp:22, compute_sh.py:39).  shs [n,C,D], D = (deg+1)^2. */
+int msplat_compute_sh_forward(const float *shs, const float *dirs,
+                              const uint8_t *visible, int64_t n, int C, int D,
+                              float *value, void *stream);
+
+/* compute_sh.cu:1724-1753 (ext.cpp:23, compute_sh.py:51) */
+int msplat_compute_sh_backward(const float *shs, const float *dirs,
+                               const uint8_t *visible, const float *dL_dvalue,
+                               int64_t n, int C, int D, float *dL_dshs,
+                               float *dL_ddirs, void *stream);
+
+/* ---- tile sort.  Replaces msplat/sort_gaussian.py:42-52 (cumsum, keygen
+ * sort_gaussian.cu:74-113, torch.sort, gather, tile range
+ * sort_gaussian.cu:115-142) with two calls around one 8-byte device read.
+ *
+ * Step 1 computes, for every Gaussian, its tile rectangle (utils.h:17-37),
+ * orders Gaussians stably by depth bits and scans their tile counts.  It
+ * writes the intersection count K to *d_count (device int64).  If `tiles`
+ * is non-NULL it is checked against the rectangles and *d_count is set to -1
+ * on any disagreement (the reference would write out of bounds). */
+size_t msplat_sort_prepare_workspace_bytes(int64_t n);
+int msplat_sort_prepare(const float *uv, const float *depth,
+                        const int32_t *radius, const int32_t *tiles,
+                        int64_t n, int W, int H, void *workspace,
+                        size_t workspace_bytes, int64_t *d_count,
+                        void *stream);
+
+/* Step 2 emits the K (tile, Gaussian) pairs, sorts them stably by tile and
+ * writes idx_sorted [K] and tile_range [T,2], bit-exact with the reference's
+ * stable signed-int64 key order.  key_sorted [K] may be NULL; when given it
+ * receives the reference's 64-bit keys (tile<<32 | depth bits) in sorted
+ * order.  `prepared` is the step-1 workspace, unchanged. */
+size_t msplat_sort_emit_workspace_bytes(int64_t n, int64_t k, int W, int H);
+int msplat_sort_emit(const float *uv, const float *depth,
+                     const int32_t *radius, int64_t n, int W, int H,
+                     const void *prepared, int64_t k, void *workspace,
+                     size_t workspace_bytes, int32_t *idx_sorted,
+                     int32_t *tile_range, int64_t *key_sorted, void *stream);
+
+/* The reference keygen in its own order: sort_gaussian.cu:17-43,74-113
+ * (_C.compute_gaussian_key, ext.cpp:20).  `offsets` is the inclusive cumsum
+ * of tiles; keys [k] int64 and idx [k] int32 are fully written (slots the
+ * reference leaves untouched get 0, as its torch::zeros would). */
+int msplat_gaussian_keys(const float *uv, const float *depth,
+                         const int32_t *radius, const int32_t *offsets,
+                         int64_t n, int W, int H, int64_t k, int64_t *keys,
+                         int32_t *idx, void *stream);
+
+/* Tile ranges from sorted keys: sort_gaussian.cu:45-71,115-142
+ * (_C.compute_tile_gaussian_range, ext.cpp:21), without the out-of-bounds
+ * write for negative-depth keys (tile id -1). */
+int msplat_tile_range_from_keys(const int64_t *key_sorted, int64_t k, int W,
+                                int H, int32_t *tile_range, void *stream);
+
+/* ---- alpha blending: msplat/src/alpha_blending.cu:248-394 (ext.cpp:24,
+ *      alpha_blending.py:59).  All C channels in one walk per pixel. */
+int msplat_alpha_blending_forward(const float *uv, const float *conic,
+                                  const float *opacity, const float *feature,
+                                  int64_t n, int C, const int32_t *idx_sorted,
+                                  const int32_t *tile_range, float bg, int W,
+                                  int H, float *rendered, float *final_T,
+                                  int32_t *ncontrib, void *stream);
+
+/* alpha_blending.cu:396-573 (ext.cpp:25, alpha_blending.py:92).  Gradient
+ * outputs are overwritten (zeroed, then accumulated). */
+int msplat_alpha_blending_backward(
+    const float *uv, const float *conic, const float *opacity,
+    const float *feature, int64_t n, int C, const int32_t *idx_sorted,
+    const int32_t *tile_range, float bg, int W, int H, const float *final_T,
+    const int32_t *ncontrib, const float *dL_drendered, float *dL_duv,
+    float *dL_dconic, float *dL_dopacity, float *dL_dfeature, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSPLAT_AMD_H */

@@ -1,0 +1,263 @@
+// Front-to-back alpha blending over the sorted per-tile Gaussian lists,
+// forward and adjoint (msplat/src/alpha_blending.cu:16-246).
+//
+// Work decomposition (gfx950): a 16x16 tile is split into four 8x8
+// quadrants, each handled by a one-wavefront workgroup (64 lanes = 64
+// pixels).  A wave stages 64 Gaussians of the tile list at a time in LDS
+// (position, conic, opacity and all C feature channels) and every lane walks
+// them; the wave stops as soon as all 64 of its pixels are saturated (a
+// ballot), independent of the other quadrants, and needs no workgroup-wide
+// barriers.  The four quadrants of one tile read the same list, so their
+// workgroup ids are chosen congruent modulo 8: workgroups are dispatched
+// round-robin over the 8 XCDs, which puts the four on one XCD and its L2.
+//
+// All C channels are blended in one walk (the reference re-walks the list
+// for every 32-channel chunk).
+#pragma once
+
+#include "common.hpp"
+
+namespace msplat {
+
+constexpr int kBlendLanes = 64;
+
+struct QuadPix {
+  int tile, px, py;
+  bool active;  // workgroup maps to a real tile
+};
+
+// blockIdx.x -> (tile, quadrant): xcd = b % 8 selects the tile within a group
+// of 8 tiles, (b / 8) % 4 the quadrant, b / 32 the group.
+MSPLAT_DEV QuadPix quad_pixel(int gx, int ntiles) {
+  const int b = blockIdx.x;
+  const int tile = (b >> 5) * 8 + (b & 7);
+  const int quad = (b >> 3) & 3;
+  QuadPix q;
+  q.tile = tile;
+  q.active = tile < ntiles;
+  const int tx = q.active ? tile % gx : 0, ty = q.active ? tile / gx : 0;
+  const int lane = threadIdx.x;
+  q.px = tx * kTile + (quad & 1) * 8 + (lane & 7);
+  q.py = ty * kTile + (quad >> 1) * 8 + (lane >> 3);
+  return q;
+}
+
+inline unsigned blend_grid(int ntiles) { return unsigned(((ntiles + 7) / 8) * 32); }
+
+// exp(x) as 2^(x log2 e) on the hardware exp2 (v_exp_f32).
+MSPLAT_DEV float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+
+template <int CMAX>
+struct BlendStage {
+  float u[kBlendLanes], v[kBlendLanes], a[kBlendLanes], b[kBlendLanes], c[kBlendLanes], o[kBlendLanes];
+  float f[CMAX * kBlendLanes];  // channel-major: f[ch * 64 + slot]
+};
+
+// Stage Gaussian g into LDS slot `slot`: channels [c0, c0 + cc) of its row.
+template <int CMAX>
+MSPLAT_DEV void stage_gaussian(BlendStage<CMAX> &s, int slot, int g, const float *__restrict__ uv,
+                               const float *__restrict__ conic, const float *__restrict__ opacity,
+                               const float *__restrict__ feature, int C, int c0, int cc) {
+  s.u[slot] = uv[2 * int64_t(g)];
+  s.v[slot] = uv[2 * int64_t(g) + 1];
+  s.a[slot] = conic[3 * int64_t(g)];
+  s.b[slot] = conic[3 * int64_t(g) + 1];
+  s.c[slot] = conic[3 * int64_t(g) + 2];
+  s.o[slot] = opacity[g];
+  const float *fr = feature + int64_t(g) * C + c0;
+#pragma unroll
+  for (int ch = 0; ch < CMAX; ++ch)
+    if (ch < cc) s.f[ch * kBlendLanes + slot] = fr[ch];
+}
+
+// power = -0.5 (a dx^2 + c dy^2) - b dx dy with d = mean - pixel
+// (alpha_blending.cu:74-82).
+MSPLAT_DEV float gauss_power(float a, float b, float c, float dx, float dy) {
+  return -0.5f * ((a * dx) * dx + (c * dy) * dy) - (b * dx) * dy;
+}
+
+template <int CMAX>
+__global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(
+    const float *__restrict__ uv, const float *__restrict__ conic, const float *__restrict__ opacity,
+    const float *__restrict__ feature, int C, int c0, const int32_t *__restrict__ idx_sorted,
+    const int32_t *__restrict__ tile_range, float bg, int W, int H, int gx, int ntiles,
+    float *__restrict__ out, float *__restrict__ final_T, int32_t *__restrict__ ncontrib) {
+  __shared__ BlendStage<CMAX> s;
+  const QuadPix q = quad_pixel(gx, ntiles);
+  if (!q.active) return;
+  const int lane = threadIdx.x;
+  const bool inside = q.px < W && q.py < H;
+  const float pxf = float(q.px), pyf = float(q.py);
+  const int start = tile_range[2 * q.tile], end = tile_range[2 * q.tile + 1];
+  const int cc = C - c0 < CMAX ? C - c0 : CMAX;  // channels of this pass
+
+  bool done = !inside;
+  float T = 1.f;
+  int contributor = 0, last = 0;
+  float F[CMAX];
+#pragma unroll
+  for (int ch = 0; ch < CMAX; ++ch) F[ch] = 0.f;
+
+  for (int base = start; base < end; base += kBlendLanes) {
+    if (__ballot(!done) == 0ull) break;
+    const int cnt = end - base < kBlendLanes ? end - base : kBlendLanes;
+    __syncthreads();
+    if (lane < cnt) stage_gaussian<CMAX>(s, lane, idx_sorted[base + lane], uv, conic, opacity, feature, C, c0, cc);
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      if (!done) {
+        ++contributor;
+        const float dx = s.u[j] - pxf, dy = s.v[j] - pyf;
+        const float power = gauss_power(s.a[j], s.b[j], s.c[j], dx, dy);
+        if (!(power > 0.f)) {  // a NaN power is not skipped, as in the reference
+          const float alpha = fminf(0.99f, s.o[j] * fast_exp(power));
+          if (!(alpha < 1.f / 255.f)) {
+            const float nT = T * (1.f - alpha);
+            if (nT < 0.0001f) {
+              done = true;
+            } else {
+#pragma unroll
+              for (int ch = 0; ch < CMAX; ++ch)
+                if (ch < cc) F[ch] += s.f[ch * kBlendLanes + j] * alpha * T;
+              T = nT;
+              last = contributor;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (inside) {
+    const int64_t pix = int64_t(q.py) * W + q.px;
+    const int64_t hw = int64_t(H) * W;
+    if (c0 == 0) {
+      final_T[pix] = T;
+      ncontrib[pix] = last;
+    }
+#pragma unroll
+    for (int ch = 0; ch < CMAX; ++ch)
+      if (ch < cc) out[(c0 + ch) * hw + pix] = F[ch] + T * bg;
+  }
+}
+
+// Adjoint: walk each pixel's contributors back to front, recovering T by
+// division (alpha_blending.cu:155-245); per Gaussian, the wave sums its 64
+// lanes' contributions and issues one atomic per gradient value.
+template <int CMAX>
+__global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
+    const float *__restrict__ uv, const float *__restrict__ conic, const float *__restrict__ opacity,
+    const float *__restrict__ feature, int C, int c0, const int32_t *__restrict__ idx_sorted,
+    const int32_t *__restrict__ tile_range, float bg, int W, int H, int gx, int ntiles,
+    const float *__restrict__ final_T, const int32_t *__restrict__ ncontrib, const float *__restrict__ dout,
+    float *__restrict__ duv, float *__restrict__ dconic, float *__restrict__ dopac, float *__restrict__ dfeat) {
+  __shared__ BlendStage<CMAX> s;
+  const QuadPix q = quad_pixel(gx, ntiles);
+  if (!q.active) return;
+  const int lane = threadIdx.x;
+  const bool inside = q.px < W && q.py < H;
+  const float pxf = float(q.px), pyf = float(q.py);
+  const int start = tile_range[2 * q.tile];
+  const int64_t pix = int64_t(q.py) * W + q.px;
+  const int64_t hw = int64_t(H) * W;
+
+  const int cc = C - c0 < CMAX ? C - c0 : CMAX;  // channels of this pass
+  const float T_final = inside ? final_T[pix] : 0.f;
+  const int last = inside ? ncontrib[pix] : 0;
+  float g[CMAX];
+  float bg_dot = 0.f;
+#pragma unroll
+  for (int ch = 0; ch < CMAX; ++ch) {
+    g[ch] = (inside && ch < cc) ? dout[(c0 + ch) * hw + pix] : 0.f;
+    if (ch < cc) bg_dot += bg * g[ch];
+  }
+  const int wave_last = wave_max(last);
+
+  float T = T_final;
+  float accum[CMAX], last_f[CMAX];
+#pragma unroll
+  for (int ch = 0; ch < CMAX; ++ch) accum[ch] = last_f[ch] = 0.f;
+  float last_alpha = 0.f;
+
+  for (int hi = start + wave_last; hi > start; hi -= kBlendLanes) {
+    const int lo = hi - kBlendLanes > start ? hi - kBlendLanes : start;
+    const int cnt = hi - lo;
+    __syncthreads();
+    if (lane < cnt) stage_gaussian<CMAX>(s, lane, idx_sorted[lo + lane], uv, conic, opacity, feature, C, c0, cc);
+    __syncthreads();
+    for (int j = cnt - 1; j >= 0; --j) {
+      const int rel = lo + j - start;  // 0-based position in the tile list
+      bool contrib = rel < last;
+      const float dx = s.u[j] - pxf, dy = s.v[j] - pyf;
+      const float a = s.a[j], b = s.b[j], c = s.c[j], op = s.o[j];
+      float G = 0.f, alpha = 0.f;
+      if (contrib) {
+        const float power = gauss_power(a, b, c, dx, dy);
+        if (power > 0.f) {
+          contrib = false;
+        } else {
+          G = fast_exp(power);
+          alpha = fminf(0.99f, op * G);
+          if (alpha < 1.f / 255.f) contrib = false;
+        }
+      }
+      const uint64_t any = __ballot(contrib);
+      if (any == 0ull) continue;
+      float vuv0 = 0.f, vuv1 = 0.f, vc0 = 0.f, vc1 = 0.f, vc2 = 0.f, vo = 0.f;
+      float vf[CMAX];
+#pragma unroll
+      for (int ch = 0; ch < CMAX; ++ch) vf[ch] = 0.f;
+      if (contrib) {
+        T = T / (1.f - alpha);
+        const float dcol = alpha * T;
+        float dL_dalpha = 0.f;
+#pragma unroll
+        for (int ch = 0; ch < CMAX; ++ch) {
+          if (ch < cc) {
+            const float f = s.f[ch * kBlendLanes + j];
+            accum[ch] = last_alpha * last_f[ch] + (1.f - last_alpha) * accum[ch];
+            last_f[ch] = f;
+            dL_dalpha += (f - accum[ch]) * g[ch];
+            vf[ch] = dcol * g[ch];
+          }
+        }
+        dL_dalpha *= T;
+        last_alpha = alpha;
+        dL_dalpha += (-T_final / (1.f - alpha)) * bg_dot;
+        const float dL_dG = op * dL_dalpha;
+        const float gdx = ((-G) * dx) * a - (G * dy) * b;
+        const float gdy = ((-G) * dy) * c - (G * dx) * b;
+        vuv0 = dL_dG * gdx;
+        vuv1 = dL_dG * gdy;
+        vc0 = (((-0.5f * G) * dx) * dx) * dL_dG;
+        vc1 = (((-G) * dx) * dy) * dL_dG;
+        vc2 = (((-0.5f * G) * dy) * dy) * dL_dG;
+        vo = G * dL_dalpha;
+      }
+      // Wave sums; lane k then owns value k and issues its atomic.
+      const int gidx = idx_sorted[lo + j];
+      float mine = 0.f;
+      float *dst = nullptr;
+      {
+        const float r0 = wave_sum(vuv0), r1 = wave_sum(vuv1);
+        const float r2 = wave_sum(vc0), r3 = wave_sum(vc1), r4 = wave_sum(vc2);
+        const float r5 = wave_sum(vo);
+        if (lane == 0) { mine = r0; dst = duv + 2 * int64_t(gidx); }
+        if (lane == 1) { mine = r1; dst = duv + 2 * int64_t(gidx) + 1; }
+        if (lane == 2) { mine = r2; dst = dconic + 3 * int64_t(gidx); }
+        if (lane == 3) { mine = r3; dst = dconic + 3 * int64_t(gidx) + 1; }
+        if (lane == 4) { mine = r4; dst = dconic + 3 * int64_t(gidx) + 2; }
+        if (lane == 5) { mine = r5; dst = dopac + gidx; }
+      }
+#pragma unroll
+      for (int ch = 0; ch < CMAX; ++ch) {
+        if (ch < cc) {
+          const float r = wave_sum(vf[ch]);
+          if (lane == 6 + ch) { mine = r; dst = dfeat + int64_t(gidx) * C + c0 + ch; }
+        }
+      }
+      if (dst) unsafeAtomicAdd(dst, mine);
+    }
+  }
+}
+
+}  // namespace msplat

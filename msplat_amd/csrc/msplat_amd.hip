@@ -1,0 +1,526 @@
+// C ABI of the gfx950 rasterizer (declared in include/msplat_amd.h).
+//
+// This is the library's only translation unit: it includes every kernel
+// header once.  Each entry point validates its arguments, launches on the
+// caller's stream and reports launch errors; nothing here allocates.
+#include "../../include/msplat_amd.h"
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "blend.hpp"
+#include "common.hpp"
+#include "preprocess.hpp"
+#include "sh.hpp"
+#include "sort.hpp"
+
+using namespace msplat;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+int launched(const char *what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MSPLAT_ERR_LAUNCH, "%s: %s", what, hipGetErrorString(e));
+  return MSPLAT_OK;
+}
+
+hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+#define REQUIRE(cond, ...)                                         \
+  do {                                                             \
+    if (!(cond)) return fail(MSPLAT_ERR_ARGUMENT, __VA_ARGS__);    \
+  } while (0)
+
+bool valid_image(int W, int H) { return W > 0 && H > 0 && W <= (1 << 16) && H <= (1 << 16); }
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Workspace carving: sequential 256-byte aligned slices.
+struct Carver {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    const size_t o = off;
+    off += align256(bytes);
+    return o;
+  }
+};
+
+struct PrepLayout {
+  size_t area, ka, va, kb, vb, offs, hist, sums, flag, total;
+};
+
+PrepLayout prep_layout(int64_t n) {
+  Carver c;
+  PrepLayout L;
+  const size_t un = size_t(n);
+  L.area = c.take(un * 4);
+  L.ka = c.take(un * 4);
+  L.va = c.take(un * 4);
+  L.kb = c.take(un * 4);
+  L.vb = c.take(un * 4);
+  L.offs = c.take(un * 8);
+  const int64_t nt = sortk::radix_tiles(n);
+  L.hist = c.take(size_t(256 * nt) * 4);
+  int64_t ns = sortk::scan_tiles(256 * nt);
+  if (sortk::scan_tiles(n) > ns) ns = sortk::scan_tiles(n);
+  L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
+  L.flag = c.take(4);
+  L.total = c.off;
+  return L;
+}
+
+struct EmitLayout {
+  size_t ca, ga, cb, gb, hist, sums, total;
+};
+
+EmitLayout emit_layout(int64_t k) {
+  Carver c;
+  EmitLayout L;
+  const size_t uk = size_t(k);
+  L.ca = c.take(uk * 4);
+  L.ga = c.take(uk * 4);
+  L.cb = c.take(uk * 4);
+  L.gb = c.take(uk * 4);
+  const int64_t nt = sortk::radix_tiles(k);
+  L.hist = c.take(size_t(256 * nt) * 4);
+  const int64_t ns = sortk::scan_tiles(256 * nt);
+  L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
+  L.total = c.off;
+  return L;
+}
+
+template <typename T>
+T *at(void *base, size_t off) {
+  return reinterpret_cast<T *>(static_cast<char *>(base) + off);
+}
+template <typename T>
+const T *at(const void *base, size_t off) {
+  return reinterpret_cast<const T *>(static_cast<const char *>(base) + off);
+}
+
+int bit_length(uint32_t v) { return v ? 32 - __builtin_clz(v) : 0; }
+
+// Channel bucket for one blending pass (rem <= 64).
+int channel_bucket(int rem) {
+  if (rem <= 1) return 1;
+  if (rem <= 2) return 2;
+  if (rem <= 3) return 3;
+  if (rem <= 4) return 4;
+  if (rem <= 8) return 8;
+  if (rem <= 16) return 16;
+  if (rem <= 32) return 32;
+  return 64;
+}
+
+int sh_degree(int D) {
+  for (int l = 0; l <= shc::kMaxDeg; ++l)
+    if ((l + 1) * (l + 1) == D) return l;
+  return -1;
+}
+
+template <int L>
+void sh_fwd_launch(int64_t n, int C, const float *shs, const float *dirs, const uint8_t *vis, float *val,
+                   hipStream_t s) {
+  sh_fwd_kernel<L><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dirs, vis, val);
+}
+template <int L>
+void sh_bwd_launch(int64_t n, int C, const float *shs, const float *dirs, const uint8_t *vis, const float *dval,
+                   float *dshs, float *ddirs, hipStream_t s) {
+  sh_bwd_kernel<L><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dirs, vis, dval, dshs, ddirs);
+}
+
+#define MSPLAT_SH_DISPATCH(L, CALL) \
+  switch (L) {                      \
+    case 0: CALL(0); break;         \
+    case 1: CALL(1); break;         \
+    case 2: CALL(2); break;         \
+    case 3: CALL(3); break;         \
+    case 4: CALL(4); break;         \
+    case 5: CALL(5); break;         \
+    case 6: CALL(6); break;         \
+    case 7: CALL(7); break;         \
+    case 8: CALL(8); break;         \
+    case 9: CALL(9); break;         \
+    default: CALL(10); break;       \
+  }
+}  // namespace
+
+extern "C" {
+
+int msplat_version(void) { return 100; }
+
+const char *msplat_last_error(void) { return g_error.c_str(); }
+
+// ---------------------------------------------------------------- projection
+
+int msplat_project_point_forward(const float *xyz, int64_t n, const float *intr, const float *extr, int W,
+                                 int H, float nearest, float extent, float *uv, float *depth, void *stream) {
+  REQUIRE(n >= 0, "project_point_forward: n = %lld", (long long)n);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(xyz && intr && extr && uv && depth, "project_point_forward: null pointer");
+  project_fwd_kernel<<<grid_1d(n, kGeomThreads), kGeomThreads, 0, as_stream(stream)>>>(xyz, n, intr, extr, W, H,
+                                                                                       nearest, extent, uv, depth);
+  return launched("project_point_forward");
+}
+
+int msplat_project_point_backward(const float *xyz, int64_t n, const float *intr, const float *extr,
+                                  const float *depth, const float *dL_duv, const float *dL_ddepth, float *dL_dxyz,
+                                  float *dL_dintr, float *dL_dextr, void *stream) {
+  REQUIRE(n >= 0, "project_point_backward: n = %lld", (long long)n);
+  hipStream_t s = as_stream(stream);
+  const bool cam = dL_dintr || dL_dextr;
+  if (dL_dintr) launch_fill_zero(dL_dintr, 4, s);
+  if (dL_dextr) launch_fill_zero(dL_dextr, 12, s);
+  if (n == 0) return launched("project_point_backward");
+  REQUIRE(xyz && intr && extr && depth && dL_duv && dL_dxyz, "project_point_backward: null pointer");
+  if (cam)
+    project_bwd_kernel<true><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+        xyz, n, intr, extr, depth, dL_duv, dL_ddepth, dL_dxyz, dL_dintr, dL_dextr);
+  else
+    project_bwd_kernel<false><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+        xyz, n, intr, extr, depth, dL_duv, dL_ddepth, dL_dxyz, nullptr, nullptr);
+  return launched("project_point_backward");
+}
+
+// ---------------------------------------------------------------- covariance
+
+int msplat_compute_cov3d_forward(const float *scale, const float *rotate, const uint8_t *visible, int64_t n,
+                                 float *cov3d, void *stream) {
+  REQUIRE(n >= 0, "compute_cov3d_forward: n = %lld", (long long)n);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(scale && rotate && cov3d, "compute_cov3d_forward: null pointer");
+  cov3d_fwd_kernel<<<grid_1d(n, kGeomThreads), kGeomThreads, 0, as_stream(stream)>>>(scale, rotate, visible, n,
+                                                                                     cov3d);
+  return launched("compute_cov3d_forward");
+}
+
+int msplat_compute_cov3d_backward(const float *scale, const float *rotate, const uint8_t *visible,
+                                  const float *dL_dcov3d, int64_t n, float *dL_dscale, float *dL_drotate,
+                                  void *stream) {
+  REQUIRE(n >= 0, "compute_cov3d_backward: n = %lld", (long long)n);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(scale && rotate && dL_dcov3d && dL_dscale && dL_drotate, "compute_cov3d_backward: null pointer");
+  cov3d_bwd_kernel<<<grid_1d(n, kGeomThreads), kGeomThreads, 0, as_stream(stream)>>>(
+      scale, rotate, visible, dL_dcov3d, n, dL_dscale, dL_drotate);
+  return launched("compute_cov3d_backward");
+}
+
+// ---------------------------------------------------------------- EWA
+
+int msplat_ewa_project_forward(const float *xyz, const float *cov3d, const float *intr, const float *extr,
+                               const float *uv, const uint8_t *visible, int64_t n, int W, int H, float *conic,
+                               int32_t *radius, int32_t *tiles, void *stream) {
+  REQUIRE(n >= 0, "ewa_project_forward: n = %lld", (long long)n);
+  REQUIRE(valid_image(W, H), "ewa_project_forward: bad image size %dx%d", W, H);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(xyz && cov3d && intr && extr && uv && conic && radius && tiles, "ewa_project_forward: null pointer");
+  ewa_fwd_kernel<<<grid_1d(n, kGeomThreads), kGeomThreads, 0, as_stream(stream)>>>(
+      xyz, cov3d, intr, extr, uv, visible, n, tiles_x(W), tiles_y(H), conic, radius, tiles);
+  return launched("ewa_project_forward");
+}
+
+int msplat_ewa_project_backward(const float *xyz, const float *cov3d, const float *intr, const float *extr,
+                                const int32_t *radius, const float *dL_dconic, int64_t n, float *dL_dxyz,
+                                float *dL_dcov3d, float *dL_dintr, float *dL_dextr, void *stream) {
+  REQUIRE(n >= 0, "ewa_project_backward: n = %lld", (long long)n);
+  hipStream_t s = as_stream(stream);
+  const bool cam = dL_dintr || dL_dextr;
+  if (dL_dintr) launch_fill_zero(dL_dintr, 4, s);
+  if (dL_dextr) launch_fill_zero(dL_dextr, 12, s);
+  if (n == 0) return launched("ewa_project_backward");
+  REQUIRE(xyz && cov3d && intr && extr && radius && dL_dconic && dL_dxyz && dL_dcov3d,
+          "ewa_project_backward: null pointer");
+  if (cam)
+    ewa_bwd_kernel<true><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+        xyz, cov3d, intr, extr, radius, dL_dconic, n, dL_dxyz, dL_dcov3d, dL_dintr, dL_dextr);
+  else
+    ewa_bwd_kernel<false><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+        xyz, cov3d, intr, extr, radius, dL_dconic, n, dL_dxyz, dL_dcov3d, nullptr, nullptr);
+  return launched("ewa_project_backward");
+}
+
+// ---------------------------------------------------------------- fused
+
+int msplat_preprocess_forward(const float *xyz, const float *scale, const float *rotate, int64_t n,
+                              const float *intr, const float *extr, int W, int H, float nearest, float extent,
+                              float *uv, float *depth, float *conic, int32_t *radius, int32_t *tiles,
+                              void *stream) {
+  REQUIRE(n >= 0, "preprocess_forward: n = %lld", (long long)n);
+  REQUIRE(valid_image(W, H), "preprocess_forward: bad image size %dx%d", W, H);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(xyz && scale && rotate && intr && extr && uv && depth && conic && radius && tiles,
+          "preprocess_forward: null pointer");
+  preprocess_fwd_kernel<<<grid_1d(n, kGeomThreads), kGeomThreads, 0, as_stream(stream)>>>(
+      xyz, scale, rotate, n, intr, extr, W, H, tiles_x(W), tiles_y(H), nearest, extent, uv, depth, conic, radius,
+      tiles);
+  return launched("preprocess_forward");
+}
+
+int msplat_preprocess_backward(const float *xyz, const float *scale, const float *rotate, int64_t n,
+                               const float *intr, const float *extr, const float *depth, const int32_t *radius,
+                               const float *dL_duv, const float *dL_ddepth, const float *dL_dconic, float *dL_dxyz,
+                               float *dL_dscale, float *dL_drotate, float *dL_dintr, float *dL_dextr,
+                               void *stream) {
+  REQUIRE(n >= 0, "preprocess_backward: n = %lld", (long long)n);
+  hipStream_t s = as_stream(stream);
+  const bool cam = dL_dintr || dL_dextr;
+  if (dL_dintr) launch_fill_zero(dL_dintr, 4, s);
+  if (dL_dextr) launch_fill_zero(dL_dextr, 12, s);
+  if (n == 0) return launched("preprocess_backward");
+  REQUIRE(xyz && scale && rotate && intr && extr && depth && radius && dL_duv && dL_dconic && dL_dxyz &&
+              dL_dscale && dL_drotate,
+          "preprocess_backward: null pointer");
+  if (cam)
+    preprocess_bwd_kernel<true><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+        xyz, scale, rotate, n, intr, extr, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
+        dL_drotate, dL_dintr, dL_dextr);
+  else
+    preprocess_bwd_kernel<false><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+        xyz, scale, rotate, n, intr, extr, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
+        dL_drotate, nullptr, nullptr);
+  return launched("preprocess_backward");
+}
+
+// ---------------------------------------------------------------- SH
+
+int msplat_compute_sh_forward(const float *shs, const float *dirs, const uint8_t *visible, int64_t n, int C, int D,
+                              float *value, void *stream) {
+  REQUIRE(n >= 0 && C >= 0, "compute_sh_forward: n = %lld, C = %d", (long long)n, C);
+  const int L = sh_degree(D);
+  REQUIRE(L >= 0, "compute_sh_forward: D = %d is not (deg+1)^2 with deg <= 10", D);
+  if (n == 0 || C == 0) return MSPLAT_OK;
+  REQUIRE(shs && dirs && value, "compute_sh_forward: null pointer");
+  hipStream_t s = as_stream(stream);
+#define CALL(LL) sh_fwd_launch<LL>(n, C, shs, dirs, visible, value, s)
+  MSPLAT_SH_DISPATCH(L, CALL)
+#undef CALL
+  return launched("compute_sh_forward");
+}
+
+int msplat_compute_sh_backward(const float *shs, const float *dirs, const uint8_t *visible,
+                               const float *dL_dvalue, int64_t n, int C, int D, float *dL_dshs, float *dL_ddirs,
+                               void *stream) {
+  REQUIRE(n >= 0 && C >= 0, "compute_sh_backward: n = %lld, C = %d", (long long)n, C);
+  const int L = sh_degree(D);
+  REQUIRE(L >= 0, "compute_sh_backward: D = %d is not (deg+1)^2 with deg <= 10", D);
+  if (n == 0) return MSPLAT_OK;
+  hipStream_t s = as_stream(stream);
+  if (C == 0) {
+    launch_fill_zero(dL_ddirs, 3 * n, s);
+    return launched("compute_sh_backward");
+  }
+  REQUIRE(shs && dirs && dL_dvalue && dL_dshs && dL_ddirs, "compute_sh_backward: null pointer");
+#define CALL(LL) sh_bwd_launch<LL>(n, C, shs, dirs, visible, dL_dvalue, dL_dshs, dL_ddirs, s)
+  MSPLAT_SH_DISPATCH(L, CALL)
+#undef CALL
+  return launched("compute_sh_backward");
+}
+
+// ---------------------------------------------------------------- sort
+
+size_t msplat_sort_prepare_workspace_bytes(int64_t n) { return prep_layout(n < 0 ? 0 : n).total; }
+
+int msplat_sort_prepare(const float *uv, const float *depth, const int32_t *radius, const int32_t *tiles, int64_t n,
+                        int W, int H, void *workspace, size_t workspace_bytes, int64_t *d_count, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX), "sort_prepare: n = %lld", (long long)n);
+  REQUIRE(valid_image(W, H), "sort_prepare: bad image size %dx%d", W, H);
+  REQUIRE(d_count, "sort_prepare: null count pointer");
+  const PrepLayout L = prep_layout(n);
+  if (workspace_bytes < L.total)
+    return fail(MSPLAT_ERR_WORKSPACE, "sort_prepare: workspace %zu < %zu bytes", workspace_bytes, L.total);
+  REQUIRE(workspace, "sort_prepare: null workspace");
+  hipStream_t s = as_stream(stream);
+  int32_t *flag = at<int32_t>(workspace, L.flag);
+  launch_fill_zero(flag, 1, s);
+  if (n > 0) {
+    REQUIRE(uv && depth && radius, "sort_prepare: null pointer");
+    int32_t *area = at<int32_t>(workspace, L.area);
+    uint32_t *ka = at<uint32_t>(workspace, L.ka), *va = at<uint32_t>(workspace, L.va);
+    uint32_t *kb = at<uint32_t>(workspace, L.kb), *vb = at<uint32_t>(workspace, L.vb);
+    sortk::prepare_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
+        uv, depth, radius, tiles, n, tiles_x(W), tiles_y(H), area, ka, va, flag);
+    const sortk::RadixBuffers rb{at<uint32_t>(workspace, L.hist), at<int64_t>(workspace, L.sums)};
+    // four 8-bit passes over the depth bits: a -> b -> a -> b -> a
+    sortk::radix_pass(ka, va, kb, vb, n, 0, 8, rb, s);
+    sortk::radix_pass(kb, vb, ka, va, n, 8, 8, rb, s);
+    sortk::radix_pass(ka, va, kb, vb, n, 16, 8, rb, s);
+    sortk::radix_pass(kb, vb, ka, va, n, 24, 8, rb, s);
+    sortk::exclusive_scan<int32_t, int64_t>(area, va, n, at<int64_t>(workspace, L.sums),
+                                            at<int64_t>(workspace, L.offs), d_count, s);
+  }
+  sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, d_count, n);
+  return launched("sort_prepare");
+}
+
+size_t msplat_sort_emit_workspace_bytes(int64_t n, int64_t k, int W, int H) {
+  (void)n;
+  (void)W;
+  (void)H;
+  return emit_layout(k < 0 ? 0 : k).total;
+}
+
+int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius, int64_t n, int W, int H,
+                     const void *prepared, int64_t k, void *workspace, size_t workspace_bytes, int32_t *idx_sorted,
+                     int32_t *tile_range, int64_t *key_sorted, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX), "sort_emit: n = %lld", (long long)n);
+  REQUIRE(valid_image(W, H), "sort_emit: bad image size %dx%d", W, H);
+  if (k < 0) return fail(MSPLAT_ERR_ARGUMENT, "sort_emit: tile counts disagree with the tile rectangles");
+  if (k > int64_t(INT32_MAX))
+    return fail(MSPLAT_ERR_CAPACITY, "sort_emit: %lld intersections exceed int32 indexing", (long long)k);
+  REQUIRE(tile_range, "sort_emit: null tile_range");
+  const EmitLayout L = emit_layout(k);
+  if (workspace_bytes < L.total)
+    return fail(MSPLAT_ERR_WORKSPACE, "sort_emit: workspace %zu < %zu bytes", workspace_bytes, L.total);
+  hipStream_t s = as_stream(stream);
+  const int gx = tiles_x(W), gy = tiles_y(H), T = gx * gy;
+  launch_fill_zero(tile_range, 2 * int64_t(T), s);
+  if (k == 0) return launched("sort_emit");
+  REQUIRE(uv && depth && radius && prepared && workspace && idx_sorted, "sort_emit: null pointer");
+  const PrepLayout P = prep_layout(n);
+  const uint32_t *order = at<uint32_t>(prepared, P.va);
+  const int64_t *offs = at<int64_t>(prepared, P.offs);
+  uint32_t *ca = at<uint32_t>(workspace, L.ca), *ga = at<uint32_t>(workspace, L.ga);
+  uint32_t *cb = at<uint32_t>(workspace, L.cb), *gb = at<uint32_t>(workspace, L.gb);
+  sortk::emit_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(uv, depth, radius, order, offs, n, gx,
+                                                                              gy, ca, ga);
+  const sortk::RadixBuffers rb{at<uint32_t>(workspace, L.hist), at<int64_t>(workspace, L.sums)};
+  const int bits = bit_length(uint32_t(T));  // classes 0..T
+  const int npass = (bits + 7) / 8;
+  const int db = (bits + npass - 1) / npass;
+  uint32_t *sk = ca, *sv = ga;
+  for (int p = 0; p < npass; ++p) {
+    const int shift = p * db;
+    const int pb = bits - shift < db ? bits - shift : db;
+    uint32_t *dk = (sk == ca) ? cb : ca;
+    uint32_t *dv = (p == npass - 1) ? reinterpret_cast<uint32_t *>(idx_sorted) : ((sv == ga) ? gb : ga);
+    sortk::radix_pass(sk, sv, dk, dv, k, shift, pb, rb, s);
+    sk = dk;
+    sv = dv;
+  }
+  sortk::range_kernel<<<grid_1d(k, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, k, tile_range);
+  if (key_sorted)
+    sortk::sorted_keys_kernel<<<grid_1d(k, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, idx_sorted, depth, k,
+                                                                                      key_sorted);
+  return launched("sort_emit");
+}
+
+int msplat_gaussian_keys(const float *uv, const float *depth, const int32_t *radius, const int32_t *offsets,
+                         int64_t n, int W, int H, int64_t k, int64_t *keys, int32_t *idx, void *stream) {
+  REQUIRE(n >= 0 && k >= 0, "gaussian_keys: n = %lld, k = %lld", (long long)n, (long long)k);
+  REQUIRE(valid_image(W, H), "gaussian_keys: bad image size %dx%d", W, H);
+  hipStream_t s = as_stream(stream);
+  if (k > 0) {
+    REQUIRE(keys && idx, "gaussian_keys: null output");
+    launch_fill_zero(keys, k, s);
+    launch_fill_zero(idx, k, s);
+  }
+  if (n == 0 || k == 0) return launched("gaussian_keys");
+  REQUIRE(uv && depth && radius && offsets, "gaussian_keys: null pointer");
+  sortk::ref_keys_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
+      uv, depth, radius, offsets, n, tiles_x(W), tiles_y(H), k, keys, idx);
+  return launched("gaussian_keys");
+}
+
+int msplat_tile_range_from_keys(const int64_t *key_sorted, int64_t k, int W, int H, int32_t *tile_range,
+                                void *stream) {
+  REQUIRE(k >= 0, "tile_range_from_keys: k = %lld", (long long)k);
+  REQUIRE(valid_image(W, H), "tile_range_from_keys: bad image size %dx%d", W, H);
+  REQUIRE(tile_range, "tile_range_from_keys: null tile_range");
+  hipStream_t s = as_stream(stream);
+  const int T = tiles_x(W) * tiles_y(H);
+  launch_fill_zero(tile_range, 2 * int64_t(T), s);
+  if (k > 0) {
+    REQUIRE(key_sorted, "tile_range_from_keys: null keys");
+    sortk::ref_range_kernel<<<grid_1d(k, sortk::kThreads), sortk::kThreads, 0, s>>>(key_sorted, k, T, tile_range);
+  }
+  return launched("tile_range_from_keys");
+}
+
+// ---------------------------------------------------------------- blending
+
+namespace {
+#define MSPLAT_BLEND_DISPATCH(CM, CALL) \
+  switch (CM) {                         \
+    case 1: CALL(1); break;             \
+    case 2: CALL(2); break;             \
+    case 3: CALL(3); break;             \
+    case 4: CALL(4); break;             \
+    case 8: CALL(8); break;             \
+    case 16: CALL(16); break;           \
+    case 32: CALL(32); break;           \
+    default: CALL(64); break;           \
+  }
+}  // namespace
+
+int msplat_alpha_blending_forward(const float *uv, const float *conic, const float *opacity, const float *feature,
+                                  int64_t n, int C, const int32_t *idx_sorted, const int32_t *tile_range, float bg,
+                                  int W, int H, float *rendered, float *final_T, int32_t *ncontrib, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX) && C >= 0, "alpha_blending_forward: n = %lld, C = %d", (long long)n,
+          C);
+  REQUIRE(valid_image(W, H), "alpha_blending_forward: bad image size %dx%d", W, H);
+  REQUIRE(tile_range && final_T && ncontrib && (C == 0 || rendered), "alpha_blending_forward: null output");
+  REQUIRE(n == 0 || (uv && conic && opacity && (C == 0 || feature)), "alpha_blending_forward: null input");
+  hipStream_t s = as_stream(stream);
+  const int gx = tiles_x(W), T = gx * tiles_y(H);
+  const dim3 grid(blend_grid(T));
+  int c0 = 0;
+  do {
+    const int rem = C - c0 < 64 ? C - c0 : 64;
+    const int cm = channel_bucket(rem);
+#define CALL(CMV)                                                                                              \
+  blend_fwd_kernel<CMV><<<grid, kBlendLanes, 0, s>>>(uv, conic, opacity, feature, C, c0, idx_sorted, tile_range, \
+                                                     bg, W, H, gx, T, rendered, final_T, ncontrib)
+    MSPLAT_BLEND_DISPATCH(cm, CALL)
+#undef CALL
+    c0 += 64;
+  } while (c0 < C);
+  return launched("alpha_blending_forward");
+}
+
+int msplat_alpha_blending_backward(const float *uv, const float *conic, const float *opacity, const float *feature,
+                                   int64_t n, int C, const int32_t *idx_sorted, const int32_t *tile_range, float bg,
+                                   int W, int H, const float *final_T, const int32_t *ncontrib,
+                                   const float *dL_drendered, float *dL_duv, float *dL_dconic, float *dL_dopacity,
+                                   float *dL_dfeature, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX) && C >= 0, "alpha_blending_backward: n = %lld, C = %d", (long long)n,
+          C);
+  REQUIRE(valid_image(W, H), "alpha_blending_backward: bad image size %dx%d", W, H);
+  hipStream_t s = as_stream(stream);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(uv && conic && opacity && idx_sorted && tile_range && final_T && ncontrib && dL_duv && dL_dconic &&
+              dL_dopacity && (C == 0 || (feature && dL_drendered && dL_dfeature)),
+          "alpha_blending_backward: null pointer");
+  launch_fill_zero(dL_duv, 2 * n, s);
+  launch_fill_zero(dL_dconic, 3 * n, s);
+  launch_fill_zero(dL_dopacity, n, s);
+  if (C > 0) launch_fill_zero(dL_dfeature, n * C, s);
+  if (C == 0) return launched("alpha_blending_backward");  // no channel, no gradient
+  const int gx = tiles_x(W), T = gx * tiles_y(H);
+  const dim3 grid(blend_grid(T));
+  for (int c0 = 0; c0 < C; c0 += 64) {
+    const int rem = C - c0 < 64 ? C - c0 : 64;
+    const int cm = channel_bucket(rem);
+#define CALL(CMV)                                                                                                  \
+  blend_bwd_kernel<CMV><<<grid, kBlendLanes, 0, s>>>(uv, conic, opacity, feature, C, c0, idx_sorted, tile_range,   \
+                                                     bg, W, H, gx, T, final_T, ncontrib, dL_drendered, dL_duv,     \
+                                                     dL_dconic, dL_dopacity, dL_dfeature)
+    MSPLAT_BLEND_DISPATCH(cm, CALL)
+#undef CALL
+  }
+  return launched("alpha_blending_backward");
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+// Real spherical harmonics of degree 0..10, forward and adjoint.
+//
+// The reference writes each of the 121 basis functions out as a polynomial
+// in the direction (x, y, z) (msplat/src/compute_sh.cu:116-503) and
+// differentiates those polynomials literally (:505-1598).  Every one of them
+// has the structure
+//
+//     Y_lm = K_lm * A_m(x, y) * Q_l|m|(z, rho2)
+//     A_m  = Re (x + i y)^m      (m >= 0)      Im (x + i y)^|m|   (m < 0)
+//     K_lm = sqrt((2l+1)/(4 pi) * (l-|m|)!/(l+|m|)!)  (times sqrt 2 if m != 0)
+//     Q    = (-1)^m d^m P_l / dz^m  with every r^2 replaced by rho2,
+//
+// where rho2 = x^2 + y^2 + z^2 in bands 0..3 (homogeneous forms) and rho2 = 1
+// in bands 4..10.  We evaluate that structure with the associated-Legendre
+// recurrence in z and the complex-power recurrence in (x, y) instead of 121
+// expanded polynomials, and carry the partial derivatives through the same
+// recurrences, so values and dL/ddir equal the reference's polynomials (and
+// their partials) without requiring |dir| = 1.
+#pragma once
+
+#include "common.hpp"
+
+namespace msplat {
+namespace shc {
+
+constexpr int kMaxDeg = 10;
+
+constexpr double csqrt(double v) {
+  if (v <= 0.0) return 0.0;
+  double x = v > 1.0 ? v : 1.0;
+  for (int i = 0; i < 200; ++i) x = 0.5 * (x + v / x);
+  return x;
+}
+constexpr double cfact(int n) {
+  double f = 1.0;
+  for (int i = 2; i <= n; ++i) f *= i;
+  return f;
+}
+constexpr double kPi = 3.14159265358979323846;
+// K_lm including the sqrt(2) of the real combinations.
+constexpr float knorm(int l, int m) {
+  const double k = csqrt((2.0 * l + 1.0) / (4.0 * kPi) * cfact(l - m) / cfact(l + m));
+  return static_cast<float>(m == 0 ? k : k * csqrt(2.0));
+}
+// Q_mm = (-1)^m (2m-1)!!
+constexpr float qmm(int m) {
+  double v = 1.0;
+  for (int i = 1; i <= m; ++i) v *= -(2.0 * i - 1.0);
+  return static_cast<float>(v);
+}
+
+}  // namespace shc
+
+// Calls emit(k, Y, dY/dx, dY/dy, dY/dz) for every basis index k = l^2 + l + m
+// of degree <= L.  With all loops unrolled, k is a compile-time constant at
+// every call, so callers can index register arrays with it.
+template <int L, typename Emit>
+MSPLAT_DEV void sh_basis(float x, float y, float z, Emit &&emit) {
+  float cr[L + 1], ci[L + 1];  // Re / Im of (x + i y)^m
+  cr[0] = 1.f;
+  ci[0] = 0.f;
+#pragma unroll
+  for (int m = 1; m <= L; ++m) {
+    cr[m] = x * cr[m - 1] - y * ci[m - 1];
+    ci[m] = x * ci[m - 1] + y * cr[m - 1];
+  }
+  const float rho2 = (x * x + y * y) + z * z;
+#pragma unroll
+  for (int m = 0; m <= L; ++m) {
+    float q1 = 0.f, q2 = 0.f, d1 = 0.f, d2 = 0.f;  // Q_{l-1}, Q_{l-2} and d/dz
+#pragma unroll
+    for (int l = m; l <= L; ++l) {
+      float q, dq;
+      if (l == m) {
+        q = shc::qmm(m);
+        dq = 0.f;
+      } else if (l == m + 1) {
+        q = float(2 * m + 1) * z * q1;
+        dq = float(2 * m + 1) * q1;
+      } else {
+        const float a = float(2 * l - 1) / float(l - m);
+        const float b = float(l + m - 1) / float(l - m);
+        q = a * z * q1 - b * q2;
+        dq = a * (q1 + z * d1) - b * d2;
+      }
+      // Bands 2 and 3 use r^2 where the recurrence has 1 (only the three
+      // functions with l - m >= 2 differ): (2,0), (3,1), (3,0).
+      float qv = q, qx = 0.f, qy = 0.f, qz = dq;
+      if (l == 2 && m == 0) {
+        qv = 1.5f * z * z - 0.5f * rho2;
+        qx = -x;
+        qy = -y;
+        qz = 2.f * z;
+      } else if (l == 3 && m == 1) {
+        qv = -7.5f * z * z + 1.5f * rho2;
+        qx = 3.f * x;
+        qy = 3.f * y;
+        qz = -12.f * z;
+      } else if (l == 3 && m == 0) {
+        qv = 2.5f * z * z * z - 1.5f * z * rho2;
+        qx = -3.f * x * z;
+        qy = -3.f * y * z;
+        qz = 4.5f * z * z - 1.5f * rho2;
+      }
+      const float K = shc::knorm(l, m);
+      if (m == 0) {
+        emit(l * l + l, K * qv, K * qx, K * qy, K * qz);
+      } else {
+        const float fm = float(m);
+        // cos part: A = Re z^m, dA/dx = m Re z^{m-1}, dA/dy = -m Im z^{m-1}
+        emit(l * l + l + m, K * (cr[m] * qv), K * (fm * cr[m - 1] * qv + cr[m] * qx),
+             K * (-fm * ci[m - 1] * qv + cr[m] * qy), K * (cr[m] * qz));
+        // sin part: A = Im z^m, dA/dx = m Im z^{m-1}, dA/dy = m Re z^{m-1}
+        emit(l * l + l - m, K * (ci[m] * qv), K * (fm * ci[m - 1] * qv + ci[m] * qx),
+             K * (fm * cr[m - 1] * qv + ci[m] * qy), K * (ci[m] * qz));
+      }
+      q2 = q1;
+      q1 = q;
+      d2 = d1;
+      d1 = dq;
+    }
+  }
+}
+
+constexpr int kShThreads = 256;
+
+// value[i, c] = sum_k Y_k(dir_i) * shs[i, c, k]  (compute_sh.cu:1600-1637)
+template <int L>
+__global__ __launch_bounds__(kShThreads) void sh_fwd_kernel(int64_t n, int C, const float *__restrict__ shs,
+                                                            const float *__restrict__ dirs,
+                                                            const uint8_t *__restrict__ vis,
+                                                            float *__restrict__ value) {
+  constexpr int D = (L + 1) * (L + 1);
+  const int64_t i = int64_t(blockIdx.x) * kShThreads + threadIdx.x;
+  if (i >= n) return;
+  float *out = value + i * C;
+  if (vis && !vis[i]) {
+    for (int c = 0; c < C; ++c) out[c] = 0.f;
+    return;
+  }
+  float Y[D];
+  sh_basis<L>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2],
+              [&](int k, float v, float, float, float) { Y[k] = v; });
+  const float *row = shs + i * int64_t(C) * D;
+  for (int c = 0; c < C; ++c) {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc += Y[k] * row[c * D + k];
+    out[c] = acc;
+  }
+}
+
+// dL/dshs[i,c,k] = Y_k g_c;  dL/ddir = sum_k (sum_c g_c shs[i,c,k]) grad Y_k
+// (compute_sh.cu:1639-1694).
+template <int L>
+__global__ __launch_bounds__(kShThreads) void sh_bwd_kernel(int64_t n, int C, const float *__restrict__ shs,
+                                                            const float *__restrict__ dirs,
+                                                            const uint8_t *__restrict__ vis,
+                                                            const float *__restrict__ dval,
+                                                            float *__restrict__ dshs,
+                                                            float *__restrict__ ddirs) {
+  constexpr int D = (L + 1) * (L + 1);
+  const int64_t i = int64_t(blockIdx.x) * kShThreads + threadIdx.x;
+  if (i >= n) return;
+  float *drow = dshs + i * int64_t(C) * D;
+  if (vis && !vis[i]) {
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int k = 0; k < D; ++k) drow[c * D + k] = 0.f;
+    ddirs[3 * i] = ddirs[3 * i + 1] = ddirs[3 * i + 2] = 0.f;
+    return;
+  }
+  const float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
+  const float *row = shs + i * int64_t(C) * D;
+  const float *g = dval + i * C;
+  {
+    float Y[D];
+    sh_basis<L>(x, y, z, [&](int k, float v, float, float, float) { Y[k] = v; });
+    for (int c = 0; c < C; ++c) {
+      const float gc = g[c];
+#pragma unroll
+      for (int k = 0; k < D; ++k) drow[c * D + k] = Y[k] * gc;
+    }
+  }
+  float w[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) w[k] = 0.f;
+  for (int c = 0; c < C; ++c) {
+    const float gc = g[c];
+#pragma unroll
+    for (int k = 0; k < D; ++k) w[k] += gc * row[c * D + k];
+  }
+  float gx = 0.f, gy = 0.f, gz = 0.f;
+  sh_basis<L>(x, y, z, [&](int k, float, float dx, float dy, float dz) {
+    gx += w[k] * dx;
+    gy += w[k] * dy;
+    gz += w[k] * dz;
+  });
+  ddirs[3 * i] = gx;
+  ddirs[3 * i + 1] = gy;
+  ddirs[3 * i + 2] = gz;
+}
+
+}  // namespace msplat

@@ -1,0 +1,361 @@
+// Tile binning and sorting of (tile, Gaussian) intersections.
+//
+// The reference (msplat/sort_gaussian.py:42-52, msplat/src/sort_gaussian.cu)
+// writes one 64-bit key (tile << 32 | int32 depth bits, sign-extended) per
+// intersection in Gaussian order and stably sorts all K keys as signed int64
+// (torch.sort, 8 radix passes over 64-bit keys and indices).  Sorting by that
+// key is the same as sorting by (class, depth bits as unsigned, Gaussian id)
+// where class = 0 for keys whose depth sign bit is set (the sign extension
+// turns their tile into -1, so they sort first) and tile + 1 otherwise.
+//
+// We therefore
+//   1. stably sort the n Gaussians by their 32-bit depth pattern (4 passes
+//      over n, not K),
+//   2. scan their tile counts in that order and emit the (class, id) pairs,
+//      so the emitted sequence is already ordered by (depth bits, id),
+//   3. stably radix-sort the K pairs by class alone: ceil(log2(T+1)) bits,
+//      two 7-bit passes at 1080p,
+// which yields the reference's order exactly, with 8-byte pairs instead of
+// 16-byte (key, index) pairs and 2 passes over K instead of 8.
+#pragma once
+
+#include "common.hpp"
+
+namespace msplat {
+namespace sortk {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kRadixTile = kThreads * kItems;  // keys per radix workgroup
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kThreads * kScanItems;  // elements per scan workgroup
+
+MSPLAT_HD int64_t radix_tiles(int64_t len) { return (len + kRadixTile - 1) / kRadixTile; }
+MSPLAT_HD int64_t scan_tiles(int64_t len) { return (len + kScanTile - 1) / kScanTile; }
+
+MSPLAT_DEV uint32_t fbits(float v) { return __float_as_uint(v); }
+
+// ---- device-wide exclusive scan (reduce, scan of partials, apply) ------------
+
+// Value at position i of the scanned sequence: in[perm[i]] or in[i].
+template <typename In>
+MSPLAT_DEV int64_t scan_load(const In *in, const uint32_t *perm, int64_t i) {
+  return static_cast<int64_t>(perm ? in[perm[i]] : in[i]);
+}
+
+MSPLAT_DEV int64_t wave_incl_scan(int64_t v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int64_t o = __shfl_up(v, off, kWave);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over the workgroup; returns the
+// workgroup total through *total.
+MSPLAT_DEV int64_t block_excl_scan(int64_t v, int64_t *total) {
+  __shared__ int64_t wsum[kThreads / kWave];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int64_t incl = wave_incl_scan(v);
+  if (lane == kWave - 1) wsum[wave] = incl;
+  __syncthreads();
+  int64_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / kWave; ++w) {
+    before += (w < wave) ? wsum[w] : 0;
+    all += wsum[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+template <typename In>
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const In *__restrict__ in,
+                                                               const uint32_t *__restrict__ perm, int64_t len,
+                                                               int64_t *__restrict__ sums) {
+  const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
+  int64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j)
+    if (base + j < len) s += scan_load(in, perm, base + j);
+  int64_t total;
+  block_excl_scan(s, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One workgroup scans the partial sums in place and stores the grand total.
+__global__ __launch_bounds__(kThreads) void scan_partials_kernel(int64_t *__restrict__ sums, int64_t nb,
+                                                                 int64_t *__restrict__ total_out) {
+  int64_t carry = 0;
+  for (int64_t base = 0; base < nb; base += kThreads) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t v = i < nb ? sums[i] : 0;
+    int64_t total;
+    const int64_t ex = block_excl_scan(v, &total);
+    if (i < nb) sums[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+template <typename In, typename Out>
+__global__ __launch_bounds__(kThreads) void scan_apply_kernel(const In *__restrict__ in,
+                                                              const uint32_t *__restrict__ perm, int64_t len,
+                                                              const int64_t *__restrict__ sums,
+                                                              Out *__restrict__ out) {
+  const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
+  int64_t vals[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    vals[j] = (base + j < len) ? scan_load(in, perm, base + j) : 0;
+    s += vals[j];
+  }
+  int64_t total;
+  int64_t run = block_excl_scan(s, &total) + sums[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    if (base + j < len) out[base + j] = static_cast<Out>(run);
+    run += vals[j];
+  }
+}
+
+// sums must hold scan_tiles(len) entries.
+template <typename In, typename Out>
+inline void exclusive_scan(const In *in, const uint32_t *perm, int64_t len, int64_t *sums, Out *out,
+                           int64_t *total, hipStream_t s) {
+  const int64_t nb = scan_tiles(len);
+  if (nb == 0) return;
+  scan_reduce_kernel<In><<<dim3(unsigned(nb)), dim3(kThreads), 0, s>>>(in, perm, len, sums);
+  scan_partials_kernel<<<dim3(1), dim3(kThreads), 0, s>>>(sums, nb, total);
+  scan_apply_kernel<In, Out><<<dim3(unsigned(nb)), dim3(kThreads), 0, s>>>(in, perm, len, sums, out);
+}
+
+// ---- stable LSD radix sort of (uint32 key, uint32 value) pairs --------------
+
+// Per-workgroup digit histogram, stored digit-major (hist[d * tiles + t]) so
+// that an exclusive scan of the flat array gives every (digit, workgroup) its
+// first output slot in stable order.
+__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__restrict__ keys, int64_t len,
+                                                              int shift, int bits, uint32_t *__restrict__ hist,
+                                                              int64_t ntiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t mask = (1u << bits) - 1u;
+  const int64_t base = int64_t(blockIdx.x) * kRadixTile;
+#pragma unroll 4
+  for (int it = 0; it < kItems; ++it) {
+    const int64_t i = base + it * kThreads + threadIdx.x;
+    if (i < len) atomicAdd(&h[(keys[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < (1 << bits)) hist[int64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Rank every key among equal digits of its round (wave-level match through
+// ballots, then a prefix over the 4 waves) and scatter it.  Rounds are taken
+// in input order, so equal digits keep their input order (stable).
+__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t len, int shift, int bits,
+    const uint32_t *__restrict__ offsets, int64_t ntiles, uint32_t *__restrict__ kout,
+    uint32_t *__restrict__ vout) {
+  constexpr int kWaves = kThreads / kWave;
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t cnt[kWaves][256];
+  const int tid = threadIdx.x, wave = tid / kWave;
+  const uint32_t mask = (1u << bits) - 1u;
+  run[tid] = tid < (1 << bits) ? offsets[int64_t(tid) * ntiles + blockIdx.x] : 0u;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) cnt[w][tid] = 0u;
+  __syncthreads();
+  const int64_t base = int64_t(blockIdx.x) * kRadixTile;
+  for (int it = 0; it < kItems; ++it) {
+    const int64_t i = base + it * kThreads + tid;
+    const bool valid = i < len;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const uint32_t d = (key >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(valid && bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t rank =
+        __builtin_amdgcn_mbcnt_hi(uint32_t(peers >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(peers), 0u));
+    if (valid && rank == 0) cnt[wave][d] = uint32_t(__popcll(peers));
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) pos += (w < wave) ? cnt[w][d] : 0u;
+      kout[pos] = key;
+      vout[pos] = vin[i];
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      add += cnt[w][tid];
+      cnt[w][tid] = 0u;
+    }
+    run[tid] += add;
+    __syncthreads();
+  }
+}
+
+struct RadixBuffers {
+  uint32_t *hist;  // 256 * radix_tiles(len)
+  int64_t *sums;   // scan_tiles(256 * radix_tiles(len))
+};
+
+// One stable pass over bits [shift, shift + bits).
+inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t len,
+                       int shift, int bits, const RadixBuffers &rb, hipStream_t s) {
+  const int64_t nt = radix_tiles(len);
+  if (nt == 0) return;
+  radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, len, shift, bits, rb.hist, nt);
+  const int64_t hlen = int64_t(1 << bits) * nt;
+  exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
+  radix_scatter_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, vin, len, shift, bits, rb.hist, nt,
+                                                                      kout, vout);
+}
+
+// ---- sort-specific kernels ---------------------------------------------------
+
+// Tile count per Gaussian from its rectangle, depth bits as the first sort
+// key, identity as the value; flags any disagreement with caller-given tiles.
+__global__ __launch_bounds__(kThreads) void prepare_kernel(const float *__restrict__ uv,
+                                                           const float *__restrict__ depth,
+                                                           const int32_t *__restrict__ radius,
+                                                           const int32_t *__restrict__ tiles, int64_t n, int gx,
+                                                           int gy, int32_t *__restrict__ area,
+                                                           uint32_t *__restrict__ keys,
+                                                           uint32_t *__restrict__ vals, int32_t *__restrict__ flag) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int r = radius[i];
+  int a = 0;
+  if (r > 0) a = tile_rect(uv[2 * i], uv[2 * i + 1], r, gx, gy).area();
+  area[i] = a;
+  keys[i] = fbits(depth[i]);
+  vals[i] = uint32_t(i);
+  if (tiles && tiles[i] != a) atomicOr(flag, 1);
+}
+
+__global__ void finish_count_kernel(const int32_t *__restrict__ flag, int64_t *__restrict__ count, int64_t n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (n == 0) *count = 0;
+    if (*flag) *count = -1;
+  }
+}
+
+// Emit (class, id) for every tile of every Gaussian, Gaussians in depth order.
+// One wave walks its 64 Gaussians one at a time and writes each Gaussian's
+// pairs with all 64 lanes, so large splats do not serialise on one thread.
+__global__ __launch_bounds__(kThreads) void emit_kernel(const float *__restrict__ uv,
+                                                        const float *__restrict__ depth,
+                                                        const int32_t *__restrict__ radius,
+                                                        const uint32_t *__restrict__ order,
+                                                        const int64_t *__restrict__ offs, int64_t n, int gx, int gy,
+                                                        uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int lane = threadIdx.x % kWave;
+  int x0 = 0, y0 = 0, w = 1, a = 0;
+  uint32_t g = 0, cbase = 0;
+  int64_t o = 0;
+  if (i < n) {
+    g = order[i];
+    const int r = radius[g];
+    if (r > 0) {
+      const TileRect t = tile_rect(uv[2 * g], uv[2 * g + 1], r, gx, gy);
+      a = t.area();
+      x0 = t.x0;
+      y0 = t.y0;
+      w = imax(1, t.x1 - t.x0);
+      // depth sign bit set: the reference key's upper word is all ones
+      cbase = (fbits(depth[g]) >> 31) ? 0u : 1u;
+      o = offs[i];
+    }
+  }
+  for (int j = 0; j < kWave; ++j) {
+    const int aj = __shfl(a, j, kWave);
+    if (aj == 0) continue;
+    const int x0j = __shfl(x0, j, kWave), y0j = __shfl(y0, j, kWave), wj = __shfl(w, j, kWave);
+    const uint32_t gj = __shfl(g, j, kWave), cj = __shfl(cbase, j, kWave);
+    const int64_t oj = __shfl(o, j, kWave);
+    for (int t = lane; t < aj; t += kWave) {
+      const int ty = y0j + t / wj, tx = x0j + t % wj;
+      cls[oj + t] = cj ? uint32_t(ty * gx + tx) + 1u : 0u;
+      gid[oj + t] = gj;
+    }
+  }
+}
+
+// tile_range[c-1] = [first, last+1) of class c in the sorted classes;
+// class 0 (negative depth) has no tile.  Unset tiles stay (0, 0).
+__global__ __launch_bounds__(kThreads) void range_kernel(const uint32_t *__restrict__ cls, int64_t k,
+                                                         int32_t *__restrict__ range) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= k) return;
+  const uint32_t c = cls[i];
+  if (c == 0) return;
+  if (i == 0 || cls[i - 1] != c) range[2 * (int64_t(c) - 1)] = int32_t(i);
+  if (i == k - 1 || cls[i + 1] != c) range[2 * (int64_t(c) - 1) + 1] = int32_t(i + 1);
+}
+
+// The reference's 64-bit keys, rebuilt from (class, id) after sorting.
+__global__ __launch_bounds__(kThreads) void sorted_keys_kernel(const uint32_t *__restrict__ cls,
+                                                               const int32_t *__restrict__ idx,
+                                                               const float *__restrict__ depth, int64_t k,
+                                                               int64_t *__restrict__ keys) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= k) return;
+  const uint32_t c = cls[i];
+  const uint32_t b = fbits(depth[idx[i]]);
+  keys[i] = c == 0 ? int64_t(int32_t(b)) : ((int64_t(c) - 1) << 32) | int64_t(b);
+}
+
+// Reference-order keygen (sort_gaussian.cu:17-43): Gaussian i writes its keys
+// from offsets[i-1] on; writes stay inside [0, k).
+__global__ __launch_bounds__(kThreads) void ref_keys_kernel(const float *__restrict__ uv,
+                                                            const float *__restrict__ depth,
+                                                            const int32_t *__restrict__ radius,
+                                                            const int32_t *__restrict__ offsets, int64_t n, int gx,
+                                                            int gy, int64_t k, int64_t *__restrict__ keys,
+                                                            int32_t *__restrict__ idx) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n || radius[i] <= 0) return;
+  const TileRect t = tile_rect(uv[2 * i], uv[2 * i + 1], radius[i], gx, gy);
+  int64_t cur = i == 0 ? 0 : offsets[i - 1];
+  const int64_t dk = int64_t(int32_t(fbits(depth[i])));
+  for (int y = t.y0; y < t.y1; ++y)
+    for (int x = t.x0; x < t.x1; ++x, ++cur)
+      if (cur >= 0 && cur < k) {
+        keys[cur] = (int64_t(y * gx + x) << 32) | dk;
+        idx[cur] = int32_t(i);
+      }
+}
+
+// sort_gaussian.cu:45-71 with bounds on the tile id.
+__global__ __launch_bounds__(kThreads) void ref_range_kernel(const int64_t *__restrict__ keys, int64_t k,
+                                                             int ntiles, int32_t *__restrict__ range) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= k) return;
+  const int cur = int(keys[i] >> 32);
+  const bool cur_ok = cur >= 0 && cur < ntiles;
+  if (i == 0 && cur_ok) range[2 * cur] = 0;
+  if (i == k - 1 && cur_ok) range[2 * cur + 1] = int32_t(k);
+  if (i == 0) return;
+  const int prev = int(keys[i - 1] >> 32);
+  if (prev != cur) {
+    if (prev >= 0 && prev < ntiles) range[2 * prev + 1] = int32_t(i);
+    if (cur_ok) range[2 * cur] = int32_t(i);
+  }
+}
+
+}  // namespace sortk
+}  // namespace msplat

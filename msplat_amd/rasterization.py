@@ -1,0 +1,101 @@
+"""The full pipeline: the drop-in for msplat.rasterization (msplat/__init__.py:22-93).
+
+The reference chains five autograd ops (project_point, compute_cov3d with
+visible = depth != 0, ewa_project, sort_gaussian, alpha_blending).  Here the
+three per-Gaussian stages run as one fused kernel in each direction and the
+whole chain is a single autograd node with a hand-written adjoint; results
+equal the chain's (the fused kernels share the per-stage device code).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .alpha_blending import blend_backward, blend_forward
+from .project_point import camera_grad_shape
+from .sort_gaussian import sort_gaussian
+
+
+def preprocess_forward(xyz, scale, rotate, intr, extr, W: int, H: int, nearest: float = 0.0,
+                       extent: float = 1.3):
+    """Fused projection + covariance + EWA: ``(uv, depth, conic, radius, tiles)``."""
+    x = _lib.f32("xyz", xyz)
+    n = x.shape[0]
+    dev = x.device
+    uv = torch.empty((n, 2), dtype=torch.float32, device=dev)
+    depth = torch.empty((n, 1), dtype=torch.float32, device=dev)
+    conic = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    radius = torch.empty(n, dtype=torch.int32, device=dev)
+    tiles = torch.empty(n, dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        _lib.call("msplat_preprocess_forward", _lib.ptr(x), _lib.ptr(_lib.f32("scale", scale)),
+                  _lib.ptr(_lib.f32("rotate", rotate)), n, _lib.ptr(_lib.f32("intr", intr)),
+                  _lib.ptr(_lib.f32("extr", extr)), int(W), int(H), float(nearest), float(extent), _lib.ptr(uv),
+                  _lib.ptr(depth), _lib.ptr(conic), _lib.ptr(radius), _lib.ptr(tiles), _lib.stream_ptr())
+    return uv, depth, conic, radius, tiles
+
+
+def preprocess_backward(xyz, scale, rotate, intr, extr, depth, radius, dL_duv, dL_dconic, dL_ddepth=None,
+                        want_intr: bool = False, want_extr: bool = False):
+    """Adjoint of preprocess_forward: ``(dxyz, dscale, drotate, dintr|None, dextr12|None)``."""
+    x = _lib.f32("xyz", xyz)
+    n = x.shape[0]
+    dev = x.device
+    dxyz = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    dscale = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    drot = torch.empty((n, 4), dtype=torch.float32, device=dev)
+    dintr = torch.empty(4, dtype=torch.float32, device=dev) if want_intr else None
+    dextr = torch.empty(12, dtype=torch.float32, device=dev) if want_extr else None
+    g_d = _lib.f32("dL_ddepth", dL_ddepth) if dL_ddepth is not None else None
+    with torch.cuda.device(dev):
+        _lib.call("msplat_preprocess_backward", _lib.ptr(x), _lib.ptr(_lib.f32("scale", scale)),
+                  _lib.ptr(_lib.f32("rotate", rotate)), n, _lib.ptr(_lib.f32("intr", intr)),
+                  _lib.ptr(_lib.f32("extr", extr)), _lib.ptr(_lib.f32("depth", depth)),
+                  _lib.ptr(_lib.i32("radius", radius)), _lib.ptr(_lib.f32("dL_duv", dL_duv)), _lib.ptr(g_d),
+                  _lib.ptr(_lib.f32("dL_dconic", dL_dconic)), _lib.ptr(dxyz), _lib.ptr(dscale), _lib.ptr(drot),
+                  _lib.ptr(dintr), _lib.ptr(dextr), _lib.stream_ptr())
+    return dxyz, dscale, drot, dintr, dextr
+
+
+def rasterization(xyz, scale, rotate, opacity, feature, intr, extr, W: int, H: int, bg: float, ndc=None):
+    """Render ``feature`` [P,C] of 3D Gaussians to a ``[C, H, W]`` image.
+
+    xyz/scale [P,3], rotate [P,4] quaternions (w,x,y,z), opacity [P,1],
+    intr = [fx, fy, cx, cy], extr = world-to-camera [R|t] ([3,4] or [4,4]).
+    ``ndc`` [P,2] only collects ``dL/duv * (W/2, H/2)``.
+    """
+    return _Rasterize.apply(xyz, scale, rotate, opacity, feature, intr, extr, W, H, bg, ndc)
+
+
+class _Rasterize(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, scale, rotate, opacity, feature, intr, extr, W, H, bg, ndc):
+        uv, depth, conic, radius, tiles = preprocess_forward(xyz, scale, rotate, intr, extr, W, H)
+        idx_sorted, tile_range = sort_gaussian(uv, depth, W, H, radius, None)
+        out, final_T, ncontrib = blend_forward(uv, conic, opacity, feature, idx_sorted, tile_range, bg, W, H)
+        ctx.W, ctx.H, ctx.bg = W, H, bg
+        ctx.has_ndc = ndc is not None
+        ctx.save_for_backward(xyz, scale, rotate, opacity, feature, intr, extr, uv, depth, conic, radius,
+                              idx_sorted, tile_range, final_T, ncontrib)
+        return out
+
+    @staticmethod
+    def backward(ctx, dL_dout):
+        (xyz, scale, rotate, opacity, feature, intr, extr, uv, depth, conic, radius, idx_sorted, tile_range,
+         final_T, ncontrib) = ctx.saved_tensors
+        duv, dconic, dopac, dfeat = blend_backward(uv, conic, opacity, feature, idx_sorted, tile_range, ctx.bg,
+                                                   ctx.W, ctx.H, final_T, ncontrib, dL_dout)
+        need_geom = ctx.needs_input_grad[0] or ctx.needs_input_grad[1] or ctx.needs_input_grad[2] \
+            or intr.requires_grad or extr.requires_grad
+        dxyz = dscale = drot = dintr = dextr = None
+        if need_geom:
+            dxyz, dscale, drot, dintr, dextr = preprocess_backward(
+                xyz, scale, rotate, intr, extr, depth, radius, duv, dconic,
+                want_intr=intr.requires_grad, want_extr=extr.requires_grad)
+        dndc = None
+        if ctx.has_ndc:
+            dndc = duv * torch.tensor([0.5 * ctx.W, 0.5 * ctx.H], dtype=duv.dtype, device=duv.device)[None]
+        return (dxyz, dscale, drot, dopac.view_as(opacity), dfeat,
+                dintr, camera_grad_shape(extr, dextr) if dextr is not None else None,
+                None, None, None, dndc)

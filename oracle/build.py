@@ -1,0 +1,28 @@
+"""Build the C part of the oracle (test infrastructure only): oracle/liboracle_blend.so.
+
+The reference's own sources cannot be built here (CUDA only, glm submodule
+empty), so there is no oracle/_ref build; see DESIGN.md.
+"""
+
+from __future__ import annotations
+
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "blend.c")
+OUT = os.path.join(HERE, "liboracle_blend.so")
+
+
+def build(verbose: bool = False) -> str:
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-std=c11",
+           SRC, "-o", OUT + ".tmp", "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=HERE)
+    os.replace(OUT + ".tmp", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))
