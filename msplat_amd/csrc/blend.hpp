@@ -233,29 +233,36 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         vc2 = (((-0.5f * G) * dy) * dy) * dL_dG;
         vo = G * dL_dalpha;
       }
-      // Wave sums; lane k then owns value k and issues its atomic.
+      // Wave sums; value v (6 geometry values, then cc channels) is issued by
+      // lane v % 64 in atomic round v / 64 (two rounds when cc > 58).
       const int gidx = idx_sorted[lo + j];
-      float mine = 0.f;
-      float *dst = nullptr;
+      float mine[2] = {0.f, 0.f};
+      float *dst[2] = {nullptr, nullptr};
       {
         const float r0 = wave_sum(vuv0), r1 = wave_sum(vuv1);
         const float r2 = wave_sum(vc0), r3 = wave_sum(vc1), r4 = wave_sum(vc2);
         const float r5 = wave_sum(vo);
-        if (lane == 0) { mine = r0; dst = duv + 2 * int64_t(gidx); }
-        if (lane == 1) { mine = r1; dst = duv + 2 * int64_t(gidx) + 1; }
-        if (lane == 2) { mine = r2; dst = dconic + 3 * int64_t(gidx); }
-        if (lane == 3) { mine = r3; dst = dconic + 3 * int64_t(gidx) + 1; }
-        if (lane == 4) { mine = r4; dst = dconic + 3 * int64_t(gidx) + 2; }
-        if (lane == 5) { mine = r5; dst = dopac + gidx; }
+        if (lane == 0) { mine[0] = r0; dst[0] = duv + 2 * int64_t(gidx); }
+        if (lane == 1) { mine[0] = r1; dst[0] = duv + 2 * int64_t(gidx) + 1; }
+        if (lane == 2) { mine[0] = r2; dst[0] = dconic + 3 * int64_t(gidx); }
+        if (lane == 3) { mine[0] = r3; dst[0] = dconic + 3 * int64_t(gidx) + 1; }
+        if (lane == 4) { mine[0] = r4; dst[0] = dconic + 3 * int64_t(gidx) + 2; }
+        if (lane == 5) { mine[0] = r5; dst[0] = dopac + gidx; }
       }
 #pragma unroll
       for (int ch = 0; ch < CMAX; ++ch) {
         if (ch < cc) {
           const float r = wave_sum(vf[ch]);
-          if (lane == 6 + ch) { mine = r; dst = dfeat + int64_t(gidx) * C + c0 + ch; }
+          constexpr int kRound0 = kBlendLanes - 6;  // channels issued in round 0
+          if (ch < kRound0) {
+            if (lane == 6 + ch) { mine[0] = r; dst[0] = dfeat + int64_t(gidx) * C + c0 + ch; }
+          } else {
+            if (lane == ch - kRound0) { mine[1] = r; dst[1] = dfeat + int64_t(gidx) * C + c0 + ch; }
+          }
         }
       }
-      if (dst) unsafeAtomicAdd(dst, mine);
+      if (dst[0]) unsafeAtomicAdd(dst[0], mine[0]);
+      if (dst[1]) unsafeAtomicAdd(dst[1], mine[1]);
     }
   }
 }

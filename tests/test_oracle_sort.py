@@ -1,0 +1,98 @@
+"""Pin the sort oracle (CPU only): the reference's known-answer test, brute
+force with Python's stable sorted() over the signed 64-bit keys, invariants."""
+
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import sort as osort
+from tests.conftest import GOLDEN
+
+
+def test_known_answer():
+    kat = json.load(open(os.path.join(GOLDEN, "sort_kat.json")))
+    uv = np.array(kat["uv"], np.float32)
+    idx, rng, keys = osort.sort_gaussian(uv, np.array(kat["depth"], np.float32), np.array(kat["radius"]),
+                                         kat["W"], kat["H"])
+    assert idx.tolist() == kat["idx_sorted"]
+    assert rng.tolist() == kat["tile_range"]
+    _, _, area = osort.gaussian_keys(uv, np.array(kat["depth"], np.float32), np.array(kat["radius"]), kat["W"],
+                                     kat["H"])
+    assert area.tolist() == kat["tiles"]
+
+
+def brute(uv, depth, radius, W, H):
+    """Keys by explicit loops, signed 64-bit, then Python's stable sort."""
+    gx, gy = (W + 15) // 16, (H + 15) // 16
+    items = []
+    for g in range(len(uv)):
+        r = int(radius[g])
+        if r <= 0:
+            continue
+        x0 = min(gx, max(0, int((np.float32(uv[g, 0]) - np.float32(r)) / np.float32(16))))
+        y0 = min(gy, max(0, int((np.float32(uv[g, 1]) - np.float32(r)) / np.float32(16))))
+        x1 = min(gx, max(0, int((np.float32(uv[g, 0]) + np.float32(r) + np.float32(15)) / np.float32(16))))
+        y1 = min(gy, max(0, int((np.float32(uv[g, 1]) + np.float32(r) + np.float32(15)) / np.float32(16))))
+        dbits = struct.unpack("<i", struct.pack("<f", float(depth[g])))[0]  # int32 bit pattern
+        for y in range(y0, y1):
+            for x in range(x0, x1):
+                key = ((y * gx + x) << 32) | (dbits & 0xFFFFFFFF)
+                if dbits < 0:
+                    key |= 0xFFFFFFFF << 32  # sign extension
+                if key >= 1 << 63:
+                    key -= 1 << 64
+                items.append((key, g))
+    items = sorted(items, key=lambda kv: kv[0])  # stable
+    keys = [k for k, _ in items]
+    ids = [g for _, g in items]
+    rng = np.zeros((gx * gy, 2), np.int32)
+    for i, k in enumerate(keys):
+        t = k >> 32
+        if t < 0:
+            continue
+        if i == 0 or (keys[i - 1] >> 32) != t:
+            rng[t, 0] = i
+        if i == len(keys) - 1 or (keys[i + 1] >> 32) != t:
+            rng[t, 1] = i + 1
+    return np.array(keys, np.int64), np.array(ids, np.int32), rng
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    W, H = int(rng.integers(20, 90)), int(rng.integers(20, 70))
+    n = 60
+    uv = rng.uniform(-20, max(W, H) + 20, (n, 2)).astype(np.float32)
+    depth = rng.choice(np.float32([-1.5, -0.25, 0.5, 1.0, 2.0]), n)  # ties and negative depths
+    radius = rng.integers(-1, 30, n)
+    bk, bi, br = brute(uv, depth, radius, W, H)
+    idx, trange, keys = osort.sort_gaussian(uv, depth, radius, W, H)
+    assert keys.tolist() == bk.tolist()
+    assert idx.tolist() == bi.tolist()
+    assert np.array_equal(trange, br)
+
+
+def test_invariants():
+    rng = np.random.default_rng(9)
+    W, H = 300, 200
+    n = 2000
+    uv = rng.uniform(-50, 350, (n, 2)).astype(np.float32)
+    depth = rng.uniform(-1, 5, n).astype(np.float32)
+    radius = rng.integers(0, 80, n)
+    idx, trange, keys = osort.sort_gaussian(uv, depth, radius, W, H)
+    _, _, area = osort.gaussian_keys(uv, depth, radius, W, H)
+    assert np.all(np.diff(keys) >= 0)
+    assert np.array_equal(np.bincount(idx, minlength=n), area)
+    neg = int((keys < 0).sum())
+    assert neg == int(area[depth < 0].sum())
+    live = trange[trange[:, 1] > trange[:, 0]]
+    # ranges tile [neg, K) exactly, in tile order
+    assert live[0, 0] == neg and live[-1, 1] == len(keys)
+    assert np.all(live[1:, 0] == live[:-1, 1])
+    for t in range(len(trange)):
+        s, e = trange[t]
+        assert np.all((keys[s:e] >> 32) == t)
+        assert np.all(np.diff(depth[idx[s:e]]) >= 0)  # front to back within a tile
