@@ -233,36 +233,40 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         vc2 = (((-0.5f * G) * dy) * dy) * dL_dG;
         vo = G * dL_dalpha;
       }
-      // Wave sums; value v (6 geometry values, then cc channels) is issued by
-      // lane v % 64 in atomic round v / 64 (two rounds when cc > 58).
+      // Sum the 6 + cc per-lane contributions over the wave in one transposed
+      // reduction; lane 16 r + k then owns value 4 k' + kRowValue[r] of
+      // vector k' = k + 16 q in atomic round q and issues its atomic.
       const int gidx = idx_sorted[lo + j];
-      float mine[2] = {0.f, 0.f};
-      float *dst[2] = {nullptr, nullptr};
-      {
-        const float r0 = wave_sum(vuv0), r1 = wave_sum(vuv1);
-        const float r2 = wave_sum(vc0), r3 = wave_sum(vc1), r4 = wave_sum(vc2);
-        const float r5 = wave_sum(vo);
-        if (lane == 0) { mine[0] = r0; dst[0] = duv + 2 * int64_t(gidx); }
-        if (lane == 1) { mine[0] = r1; dst[0] = duv + 2 * int64_t(gidx) + 1; }
-        if (lane == 2) { mine[0] = r2; dst[0] = dconic + 3 * int64_t(gidx); }
-        if (lane == 3) { mine[0] = r3; dst[0] = dconic + 3 * int64_t(gidx) + 1; }
-        if (lane == 4) { mine[0] = r4; dst[0] = dconic + 3 * int64_t(gidx) + 2; }
-        if (lane == 5) { mine[0] = r5; dst[0] = dopac + gidx; }
-      }
+      constexpr int NV = ((6 + CMAX) + 3) / 4 * 4;
+      float vals[NV];
+      vals[0] = vuv0;
+      vals[1] = vuv1;
+      vals[2] = vc0;
+      vals[3] = vc1;
+      vals[4] = vc2;
+      vals[5] = vo;
 #pragma unroll
-      for (int ch = 0; ch < CMAX; ++ch) {
-        if (ch < cc) {
-          const float r = wave_sum(vf[ch]);
-          constexpr int kRound0 = kBlendLanes - 6;  // channels issued in round 0
-          if (ch < kRound0) {
-            if (lane == 6 + ch) { mine[0] = r; dst[0] = dfeat + int64_t(gidx) * C + c0 + ch; }
-          } else {
-            if (lane == ch - kRound0) { mine[1] = r; dst[1] = dfeat + int64_t(gidx) * C + c0 + ch; }
-          }
+      for (int ch = 0; ch < NV - 6; ++ch) vals[6 + ch] = ch < CMAX ? vf[ch < CMAX ? ch : 0] : 0.f;
+      float tot[NV / 4];
+      wave_sum_transposed<NV>(vals, tot);
+      const int row = lane >> 4, k = lane & 15;
+#pragma unroll
+      for (int q = 0; q * 16 < NV / 4; ++q) {
+        const int vec = k + 16 * q;
+        float mine = 0.f;
+#pragma unroll
+        for (int jj = 0; jj < NV / 4; ++jj)
+          if (jj == vec) mine = tot[jj];
+        const int v = 4 * vec + kRowValue[row];
+        float *dst = nullptr;
+        if (vec < NV / 4 && v < 6 + cc) {
+          if (v < 2) dst = duv + 2 * int64_t(gidx) + v;
+          else if (v < 5) dst = dconic + 3 * int64_t(gidx) + (v - 2);
+          else if (v == 5) dst = dopac + gidx;
+          else dst = dfeat + int64_t(gidx) * C + c0 + (v - 6);
         }
+        if (dst) unsafeAtomicAdd(dst, mine);
       }
-      if (dst[0]) unsafeAtomicAdd(dst[0], mine[0]);
-      if (dst[1]) unsafeAtomicAdd(dst[1], mine[1]);
     }
   }
 }

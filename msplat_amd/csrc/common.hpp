@@ -76,6 +76,50 @@ MSPLAT_DEV int wave_max(int v) {
 
 MSPLAT_DEV bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// ---- transposed multi-value wave reduction (gfx950) ---------------------------
+// Sums NV values (NV a multiple of 4) over the 64 lanes at once.  Lanes form
+// 4 rows of 16.  v_permlane32_swap pairs value 2i with 2i+1 across the two
+// wave halves, v_permlane16_swap pairs the results across rows, and four DPP
+// row rotations finish each row.  On return out[j] holds, in every lane of
+// row r, the total of value 4 j + kRowValue[r].  About 1.75 + 1 instructions
+// per value instead of 12 for a separate butterfly per value.
+constexpr int kRowValue[4] = {0, 2, 1, 3};
+
+MSPLAT_DEV float dpp_row_ror_add(float v, int ctrl) {
+  // ctrl: 0x121..0x12F = row_ror:1..15 (rotate within each row of 16 lanes)
+  int r;
+  switch (ctrl) {
+    case 8: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false); break;
+    case 4: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false); break;
+    case 2: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x122, 0xf, 0xf, false); break;
+    default: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xf, 0xf, false); break;
+  }
+  return v + __int_as_float(r);
+}
+
+template <int NV>
+MSPLAT_DEV void wave_sum_transposed(const float (&in)[NV], float (&out)[NV / 4]) {
+  static_assert(NV % 4 == 0, "pad to a multiple of 4");
+  float a[NV / 2];
+#pragma unroll
+  for (int i = 0; i < NV / 2; ++i) {
+    const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(in[2 * i]), __float_as_uint(in[2 * i + 1]),
+                                                    false, false);
+    a[i] = __uint_as_float(s[0]) + __uint_as_float(s[1]);
+  }
+#pragma unroll
+  for (int j = 0; j < NV / 4; ++j) {
+    const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(a[2 * j]), __float_as_uint(a[2 * j + 1]),
+                                                    false, false);
+    float b = __uint_as_float(s[0]) + __uint_as_float(s[1]);
+    b = dpp_row_ror_add(b, 8);
+    b = dpp_row_ror_add(b, 4);
+    b = dpp_row_ror_add(b, 2);
+    b = dpp_row_ror_add(b, 1);
+    out[j] = b;
+  }
+}
+
 // Sum a block of `kVals` per-thread values over a 256-thread workgroup and add
 // the result to global memory with one atomic per value per workgroup.
 template <int kVals, int kThreads>

@@ -83,7 +83,7 @@ PrepLayout prep_layout(int64_t n) {
 }
 
 struct EmitLayout {
-  size_t ca, ga, cb, gb, hist, sums, total;
+  size_t ca, ga, cb, gb, owner, hist, sums, total;
 };
 
 EmitLayout emit_layout(int64_t k) {
@@ -94,6 +94,7 @@ EmitLayout emit_layout(int64_t k) {
   L.ga = c.take(uk * 4);
   L.cb = c.take(uk * 4);
   L.gb = c.take(uk * 4);
+  L.owner = c.take(size_t(sortk::emit_chunks(k)) * 8);
   const int64_t nt = sortk::radix_tiles(k);
   L.hist = c.take(size_t(256 * nt) * 4);
   const int64_t ns = sortk::scan_tiles(256 * nt);
@@ -131,15 +132,41 @@ int sh_degree(int D) {
   return -1;
 }
 
+// Workgroup shape of the LDS-staged SH kernels: G rows of odd stride S in at
+// most 64 KiB of LDS; G = 0 when a row is too long (direct kernels instead).
+ShGeom sh_geom(int C, int D) {
+  ShGeom g;
+  g.R = C * D;
+  g.S = g.R | 1;
+  g.G = 0;
+  for (int G = 256; G >= 32; G >>= 1)
+    if (size_t(G) * g.S * 4 <= 65536) {
+      g.G = G;
+      break;
+    }
+  g.gdivR = g.G ? g.G / g.R : 0;
+  g.gmodR = g.G ? g.G % g.R : 0;
+  return g;
+}
+
 template <int L>
 void sh_fwd_launch(int64_t n, int C, const float *shs, const float *dirs, const uint8_t *vis, float *val,
                    hipStream_t s) {
-  sh_fwd_kernel<L><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dirs, vis, val);
+  const ShGeom g = sh_geom(C, (L + 1) * (L + 1));
+  if (g.G)
+    sh_fwd_lds_kernel<L><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dirs, vis, val, g);
+  else
+    sh_fwd_kernel<L><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dirs, vis, val);
 }
 template <int L>
 void sh_bwd_launch(int64_t n, int C, const float *shs, const float *dirs, const uint8_t *vis, const float *dval,
                    float *dshs, float *ddirs, hipStream_t s) {
-  sh_bwd_kernel<L><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dirs, vis, dval, dshs, ddirs);
+  const ShGeom g = sh_geom(C, (L + 1) * (L + 1));
+  if (g.G)
+    sh_bwd_lds_kernel<L><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dirs, vis, dval, dshs,
+                                                                              ddirs, g);
+  else
+    sh_bwd_kernel<L><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dirs, vis, dval, dshs, ddirs);
 }
 
 #define MSPLAT_SH_DISPATCH(L, CALL) \
@@ -394,8 +421,11 @@ int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius,
   const int64_t *offs = at<int64_t>(prepared, P.offs);
   uint32_t *ca = at<uint32_t>(workspace, L.ca), *ga = at<uint32_t>(workspace, L.ga);
   uint32_t *cb = at<uint32_t>(workspace, L.cb), *gb = at<uint32_t>(workspace, L.gb);
-  sortk::emit_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(uv, depth, radius, order, offs, n, gx,
-                                                                              gy, ca, ga);
+  int64_t *owner = at<int64_t>(workspace, L.owner);
+  sortk::emit_owner_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(at<int32_t>(prepared, P.area),
+                                                                                    order, offs, n, owner);
+  sortk::emit_chunk_kernel<<<dim3(unsigned(sortk::emit_chunks(k))), sortk::kThreads, 0, s>>>(
+      uv, depth, radius, order, offs, owner, n, k, gx, gy, ca, ga);
   const sortk::RadixBuffers rb{at<uint32_t>(workspace, L.hist), at<int64_t>(workspace, L.sums)};
   const int bits = bit_length(uint32_t(T));  // classes 0..T
   const int npass = (bits + 7) / 8;

@@ -49,6 +49,21 @@ constexpr float qmm(int m) {
   return static_cast<float>(v);
 }
 
+// The constants, evaluated once at compile time (called with loop variables,
+// knorm/qmm would otherwise run as double-precision code in every thread).
+struct Table {
+  float k[kMaxDeg + 1][kMaxDeg + 1];
+  float q[kMaxDeg + 1];
+};
+constexpr Table make_table() {
+  Table t{};
+  for (int l = 0; l <= kMaxDeg; ++l)
+    for (int m = 0; m <= l; ++m) t.k[l][m] = knorm(l, m);
+  for (int m = 0; m <= kMaxDeg; ++m) t.q[m] = qmm(m);
+  return t;
+}
+constexpr Table kTable = make_table();
+
 }  // namespace shc
 
 // Calls emit(k, Y, dY/dx, dY/dy, dY/dz) for every basis index k = l^2 + l + m
@@ -72,7 +87,7 @@ MSPLAT_DEV void sh_basis(float x, float y, float z, Emit &&emit) {
     for (int l = m; l <= L; ++l) {
       float q, dq;
       if (l == m) {
-        q = shc::qmm(m);
+        q = shc::kTable.q[m];
         dq = 0.f;
       } else if (l == m + 1) {
         q = float(2 * m + 1) * z * q1;
@@ -102,7 +117,7 @@ MSPLAT_DEV void sh_basis(float x, float y, float z, Emit &&emit) {
         qy = -3.f * y * z;
         qz = 4.5f * z * z - 1.5f * rho2;
       }
-      const float K = shc::knorm(l, m);
+      const float K = shc::kTable.k[l][m];
       if (m == 0) {
         emit(l * l + l, K * qv, K * qx, K * qy, K * qz);
       } else {
@@ -123,6 +138,131 @@ MSPLAT_DEV void sh_basis(float x, float y, float z, Emit &&emit) {
 }
 
 constexpr int kShThreads = 256;
+
+// ---- LDS-staged kernels ------------------------------------------------------
+// A workgroup of G threads owns G consecutive Gaussians, whose coefficient
+// rows (R = C*D floats each) form one contiguous block of HBM.  The block is
+// moved between HBM and LDS with fully coalesced loads/stores (consecutive
+// lanes, consecutive addresses) and each thread then works on its own row
+// in LDS.  Rows are stored with an odd stride S so that the 64 lanes reading
+// element k of 64 different rows hit 64 different banks.
+
+struct ShGeom {
+  int R;      // floats per Gaussian row (C*D)
+  int S;      // LDS row stride (R rounded up to odd)
+  int G;      // Gaussians (threads) per workgroup
+  int gdivR;  // G / R and G % R: step of (row, col) when the flat index advances by G
+  int gmodR;
+};
+
+// Copy rows [g0, g0 + rows) of a row-major [*, R] array between HBM and LDS.
+template <bool kToLds>
+MSPLAT_DEV void sh_block_copy(float *__restrict__ lds, float *__restrict__ glob, int64_t g0, int rows,
+                              const ShGeom &sg) {
+  const int total = rows * sg.R;
+  int e = threadIdx.x;
+  int row = e / sg.R, col = e % sg.R;
+  float *base = glob + g0 * sg.R;
+  for (; e < total; e += sg.G) {
+    if constexpr (kToLds)
+      lds[row * sg.S + col] = base[e];
+    else
+      base[e] = lds[row * sg.S + col];
+    row += sg.gdivR;
+    col += sg.gmodR;
+    if (col >= sg.R) {
+      col -= sg.R;
+      ++row;
+    }
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void sh_fwd_lds_kernel(int64_t n, int C, const float *__restrict__ shs,
+                                                         const float *__restrict__ dirs,
+                                                         const uint8_t *__restrict__ vis,
+                                                         float *__restrict__ value, ShGeom sg) {
+  constexpr int D = (L + 1) * (L + 1);
+  extern __shared__ float lds[];
+  const int64_t g0 = int64_t(blockIdx.x) * sg.G;
+  const int rows = n - g0 < sg.G ? int(n - g0) : sg.G;
+  sh_block_copy<true>(lds, const_cast<float *>(shs), g0, rows, sg);
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t >= rows) return;
+  const int64_t i = g0 + t;
+  float *out = value + i * C;
+  if (vis && !vis[i]) {
+    for (int c = 0; c < C; ++c) out[c] = 0.f;
+    return;
+  }
+  float Y[D];
+  sh_basis<L>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2],
+              [&](int k, float v, float, float, float) { Y[k] = v; });
+  const float *row = lds + t * sg.S;
+  for (int c = 0; c < C; ++c) {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc += Y[k] * row[c * D + k];
+    out[c] = acc;
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void sh_bwd_lds_kernel(int64_t n, int C, const float *__restrict__ shs,
+                                                         const float *__restrict__ dirs,
+                                                         const uint8_t *__restrict__ vis,
+                                                         const float *__restrict__ dval,
+                                                         float *__restrict__ dshs, float *__restrict__ ddirs,
+                                                         ShGeom sg) {
+  constexpr int D = (L + 1) * (L + 1);
+  extern __shared__ float lds[];
+  const int64_t g0 = int64_t(blockIdx.x) * sg.G;
+  const int rows = n - g0 < sg.G ? int(n - g0) : sg.G;
+  sh_block_copy<true>(lds, const_cast<float *>(shs), g0, rows, sg);
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < rows) {
+    const int64_t i = g0 + t;
+    float *row = lds + t * sg.S;
+    const bool live = !vis || vis[i];
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (live) {
+      const float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
+      const float *g = dval + i * C;
+      float w[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) w[k] = 0.f;
+      for (int c = 0; c < C; ++c) {
+        const float gc = g[c];
+#pragma unroll
+        for (int k = 0; k < D; ++k) w[k] += gc * row[c * D + k];
+      }
+      // the row is consumed: overwrite it with dL/dshs = Y_k g_c
+      {
+        float Y[D];
+        sh_basis<L>(x, y, z, [&](int k, float v, float, float, float) { Y[k] = v; });
+        for (int c = 0; c < C; ++c) {
+          const float gc = g[c];
+#pragma unroll
+          for (int k = 0; k < D; ++k) row[c * D + k] = Y[k] * gc;
+        }
+      }
+      sh_basis<L>(x, y, z, [&](int k, float, float dx, float dy, float dz) {
+        gx += w[k] * dx;
+        gy += w[k] * dy;
+        gz += w[k] * dz;
+      });
+    } else {
+      for (int r = 0; r < sg.R; ++r) row[r] = 0.f;
+    }
+    ddirs[3 * i] = gx;
+    ddirs[3 * i + 1] = gy;
+    ddirs[3 * i + 2] = gz;
+  }
+  __syncthreads();
+  sh_block_copy<false>(lds, dshs, g0, rows, sg);
+}
 
 // value[i, c] = sum_k Y_k(dir_i) * shs[i, c, k]  (compute_sh.cu:1600-1637)
 template <int L>

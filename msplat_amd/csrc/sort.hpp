@@ -156,6 +156,99 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
   if (threadIdx.x < (1 << bits)) hist[int64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one 4096-key tile in three steps:
+//  1. each wave ranks its own contiguous 1024 keys (16 rounds of 64) among
+//     equal digits with ballot matching and per-wave running counters in LDS
+//     (wave-synchronous: no workgroup barrier inside the loop);
+//  2. one exchange turns per-wave counts into the tile's digit-sorted order
+//     and the keys are reordered through LDS;
+//  3. the tile is written out digit run by digit run, so consecutive lanes
+//     store consecutive addresses (coalesced), at offsets from the global scan.
+__global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t len, int shift, int bits,
+    const uint32_t *__restrict__ offsets, int64_t ntiles, uint32_t *__restrict__ kout,
+    uint32_t *__restrict__ vout) {
+  constexpr int kWaves = kThreads / kWave;
+  constexpr int kPerWave = kRadixTile / kWaves;  // 1024 keys, contiguous
+  __shared__ uint32_t s_key[kRadixTile];
+  __shared__ uint32_t s_val[kRadixTile];
+  __shared__ uint32_t s_run[kWaves][256];  // per-wave running digit counts, then wave offsets
+  __shared__ uint32_t s_lstart[256];       // tile-local first slot of each digit
+  __shared__ uint32_t s_gstart[256];       // global first slot of each digit for this tile
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const uint32_t mask = (1u << bits) - 1u;
+  const int ndig = 1 << bits;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) s_run[w][tid] = 0u;
+  if (tid < ndig) s_gstart[tid] = offsets[int64_t(tid) * ntiles + blockIdx.x];
+  __syncthreads();
+
+  const int64_t base = int64_t(blockIdx.x) * kRadixTile + int64_t(wave) * kPerWave;
+  uint32_t key[kItems], val[kItems], rnk[kItems];
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const int64_t i = base + it * kWave + lane;
+    const bool valid = i < len;
+    key[it] = valid ? kin[i] : 0u;
+    val[it] = valid ? vin[i] : 0u;
+    const uint32_t d = (key[it] >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(valid && bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t below =
+        __builtin_amdgcn_mbcnt_hi(uint32_t(peers >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(peers), 0u));
+    // wave-synchronous LDS: all lanes read the running count, then the one
+    // leader per digit advances it; the wavefront-scope fences keep the
+    // compiler from reordering or reusing these accesses across rounds.
+    rnk[it] = valid ? s_run[wave][d] + below : 0xffffffffu;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (valid && below == 0) s_run[wave][d] += uint32_t(__popcll(peers));
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+  __syncthreads();
+  // digit totals -> tile-local starts (exclusive scan over digits) and wave offsets
+  uint32_t tot = 0;
+  if (tid < ndig) {
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) tot += s_run[w][tid];
+  }
+  int64_t dummy;
+  const uint32_t lstart = uint32_t(block_excl_scan(tid < ndig ? int64_t(tot) : 0, &dummy));
+  if (tid < ndig) {
+    s_lstart[tid] = lstart;
+    uint32_t off = lstart;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t c = s_run[w][tid];
+      s_run[w][tid] = off;
+      off += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    if (rnk[it] != 0xffffffffu) {
+      const uint32_t d = (key[it] >> shift) & mask;
+      const uint32_t slot = s_run[wave][d] + rnk[it];
+      s_key[slot] = key[it];
+      s_val[slot] = val[it];
+    }
+  }
+  __syncthreads();
+  const int64_t tbase = int64_t(blockIdx.x) * kRadixTile;
+  const int cnt = len - tbase < kRadixTile ? int(len - tbase) : kRadixTile;
+  for (int e = tid; e < cnt; e += kThreads) {
+    const uint32_t k = s_key[e];
+    const uint32_t d = (k >> shift) & mask;
+    const uint32_t pos = s_gstart[d] + (uint32_t(e) - s_lstart[d]);
+    kout[pos] = k;
+    vout[pos] = s_val[e];
+  }
+}
+
 // Rank every key among equal digits of its round (wave-level match through
 // ballots, then a prefix over the 4 waves) and scatter it.  Rounds are taken
 // in input order, so equal digits keep their input order (stable).
@@ -220,8 +313,8 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
   radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, len, shift, bits, rb.hist, nt);
   const int64_t hlen = int64_t(1 << bits) * nt;
   exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
-  radix_scatter_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, vin, len, shift, bits, rb.hist, nt,
-                                                                      kout, vout);
+  radix_scatter_tile_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, vin, len, shift, bits, rb.hist,
+                                                                           nt, kout, vout);
 }
 
 // ---- sort-specific kernels ---------------------------------------------------
@@ -250,6 +343,157 @@ __global__ void finish_count_kernel(const int32_t *__restrict__ flag, int64_t *_
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     if (n == 0) *count = 0;
     if (*flag) *count = -1;
+  }
+}
+
+// ---- emission with equal output chunks per workgroup --------------------------
+// The K output slots are cut into chunks of kEmitChunk; workgroup c writes
+// chunk c.  Depth order clusters the large near splats, so splitting by
+// output slots (not by Gaussians) is what balances the work.
+
+constexpr int kEmitChunk = 4096;
+
+MSPLAT_HD int64_t emit_chunks(int64_t k) { return (k + kEmitChunk - 1) / kEmitChunk; }
+
+// owner[c] = depth-ordered position of the Gaussian whose run holds slot c * kEmitChunk.
+__global__ __launch_bounds__(kThreads) void emit_owner_kernel(const int32_t *__restrict__ area,
+                                                              const uint32_t *__restrict__ order,
+                                                              const int64_t *__restrict__ offs, int64_t n,
+                                                              int64_t *__restrict__ owner) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int a = area[order[i]];
+  if (a == 0) return;
+  const int64_t o = offs[i];
+  for (int64_t c = (o + kEmitChunk - 1) / kEmitChunk; c * kEmitChunk < o + a; ++c) owner[c] = i;
+}
+
+// Slots are written p = tid, tid + 256, ... (coalesced).  The workgroup stages
+// 256 consecutive Gaussians (starting at the chunk's owner) at a time: start
+// offsets relative to the chunk, tile box and class; a thread finds the
+// Gaussian holding its first slot by binary search and later ones by walking
+// forward.
+__global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
+    const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius,
+    const uint32_t *__restrict__ order, const int64_t *__restrict__ offs, const int64_t *__restrict__ owner,
+    int64_t n, int64_t k, int gx, int gy, uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
+  __shared__ int32_t s_rel[kThreads + 1];
+  __shared__ int32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
+  __shared__ uint32_t s_g[kThreads], s_c[kThreads];
+  const int tid = threadIdx.x;
+  const int64_t c0 = int64_t(blockIdx.x) * kEmitChunk;
+  const int span = int(k - c0 < kEmitChunk ? k - c0 : kEmitChunk);
+  auto rel_of = [&](int64_t off) {  // offset relative to the chunk, clamped into int32
+    const int64_t r = off - c0;
+    return int32_t(r < -(int64_t(1) << 30) ? -(1 << 30) : (r > (int64_t(1) << 30) ? (1 << 30) : r));
+  };
+  int64_t gbase = owner[blockIdx.x];
+  if (gbase < 0 || gbase >= n) gbase = 0;  // every chunk has an owner; never read outside [0, n)
+  for (;;) {
+    const int64_t i = gbase + tid;
+    int32_t rel = 1 << 30;
+    if (i < n) {
+      const uint32_t g = order[i];
+      const int r = radius[g];
+      rel = rel_of(offs[i]);
+      s_g[tid] = g;
+      if (r > 0) {
+        const TileRect t = tile_rect(uv[2 * g], uv[2 * g + 1], r, gx, gy);
+        s_x0[tid] = t.x0;
+        s_y0[tid] = t.y0;
+        s_w[tid] = imax(1, t.x1 - t.x0);
+        // depth sign bit set: the reference key's upper word is all ones
+        s_c[tid] = (fbits(depth[g]) >> 31) ? 0u : 1u;
+      }
+    }
+    s_rel[tid] = rel;
+    if (tid == 0) s_rel[kThreads] = gbase + kThreads < n ? rel_of(offs[gbase + kThreads]) : rel_of(k);
+    __syncthreads();
+    const int lo_slot = imax(0, s_rel[0]);
+    const int hi_slot = imin(span, s_rel[kThreads]);
+    int p = lo_slot + tid;
+    if (p < hi_slot) {
+      int lo = 0, hi = kThreads - 1;  // largest j with s_rel[j] <= p (skips empty runs)
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_rel[mid] <= p) lo = mid; else hi = mid - 1;
+      }
+      for (; p < hi_slot; p += kThreads) {
+        while (lo + 1 < kThreads && s_rel[lo + 1] <= p) ++lo;
+        const int t = p - s_rel[lo];
+        const int w = s_w[lo];
+        // t / w through the fp32 reciprocal, corrected to the exact quotient
+        int q = int(float(t) * __builtin_amdgcn_rcpf(float(w)));
+        int rr = t - q * w;
+        if (rr < 0) { --q; rr += w; } else if (rr >= w) { ++q; rr -= w; }
+        cls[c0 + p] = s_c[lo] ? uint32_t((s_y0[lo] + q) * gx + s_x0[lo] + rr) + 1u : 0u;
+        gid[c0 + p] = s_g[lo];
+      }
+    }
+    if (s_rel[kThreads] >= span) break;  // uniform: every thread read the same value
+    gbase += kThreads;
+    __syncthreads();
+  }
+}
+
+// Emit (class, id) for every tile of every Gaussian, Gaussians in depth order,
+// load-balanced: workgroup b owns depth-ordered Gaussians [256 b, 256 b + 256),
+// whose pairs form one contiguous run of the output.  Every thread writes
+// output slots p = tid, tid + 256, ... of that run (coalesced stores) and finds
+// the Gaussian owning slot p by binary search over the run's start offsets.
+__global__ __launch_bounds__(kThreads) void emit_lb_kernel(const float *__restrict__ uv,
+                                                           const float *__restrict__ depth,
+                                                           const int32_t *__restrict__ radius,
+                                                           const uint32_t *__restrict__ order,
+                                                           const int64_t *__restrict__ offs, int64_t n, int gx,
+                                                           int gy, uint32_t *__restrict__ cls,
+                                                           uint32_t *__restrict__ gid) {
+  __shared__ int32_t s_rel[kThreads];
+  __shared__ int32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
+  __shared__ uint32_t s_g[kThreads], s_c[kThreads];
+  const int tid = threadIdx.x;
+  const int64_t first = int64_t(blockIdx.x) * kThreads;
+  const int64_t i = first + tid;
+  int a = 0;
+  if (i < n) {
+    const uint32_t g = order[i];
+    const int r = radius[g];
+    s_g[tid] = g;
+    if (r > 0) {
+      const TileRect t = tile_rect(uv[2 * g], uv[2 * g + 1], r, gx, gy);
+      a = t.area();
+      s_x0[tid] = t.x0;
+      s_y0[tid] = t.y0;
+      s_w[tid] = imax(1, t.x1 - t.x0);
+      // depth sign bit set: the reference key's upper word is all ones
+      s_c[tid] = (fbits(depth[g]) >> 31) ? 0u : 1u;
+    }
+  }
+  int64_t total64;
+  s_rel[tid] = int32_t(block_excl_scan(a, &total64));
+  __syncthreads();
+  const int total = int(total64);
+  if (total == 0) return;
+  const int64_t base = offs[first];
+  int p = tid;
+  if (p >= total) return;
+  int lo = 0, hi = kThreads - 1;  // largest j with s_rel[j] <= p (skips empty runs)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_rel[mid] <= p) lo = mid; else hi = mid - 1;
+  }
+  for (; p < total; p += kThreads) {
+    // slots advance by 256; walk forward to the run containing p
+    while (lo + 1 < kThreads && s_rel[lo + 1] <= p) ++lo;
+    const int t = p - s_rel[lo];
+    const int w = s_w[lo];
+    // t / w through the fp32 reciprocal, then corrected to the exact quotient
+    int q = int(float(t) * __builtin_amdgcn_rcpf(float(w)));
+    int r = t - q * w;
+    if (r < 0) { --q; r += w; } else if (r >= w) { ++q; r -= w; }
+    const int ty = s_y0[lo] + q, tx = s_x0[lo] + r;
+    cls[base + p] = s_c[lo] ? uint32_t(ty * gx + tx) + 1u : 0u;
+    gid[base + p] = s_g[lo];
   }
 }
 
