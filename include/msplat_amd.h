@@ -131,11 +131,12 @@ int msplat_compute_sh_backward(const float *shs, const float *dirs,
  * sort_gaussian.cu:115-142) with two calls around one 8-byte device read.
  *
  * Step 1 computes, for every Gaussian, its tile rectangle (utils.h:17-37),
- * orders Gaussians stably by depth bits and scans their tile counts.  It
- * writes the intersection count K to *d_count (device int64).  If `tiles`
- * is non-NULL it is checked against the rectangles and *d_count is set to -1
- * on any disagreement (the reference would write out of bounds). */
-size_t msplat_sort_prepare_workspace_bytes(int64_t n);
+ * orders Gaussians stably by depth bits, scans their tile counts and counts
+ * keys per tile.  It writes the intersection count K to *d_count (device
+ * int64).  If `tiles` is non-NULL it is checked against the rectangles and
+ * *d_count is set to -1 on any disagreement (the reference would write out
+ * of bounds).  Size queries return 0 for an invalid image size. */
+size_t msplat_sort_prepare_workspace_bytes(int64_t n, int W, int H);
 int msplat_sort_prepare(const float *uv, const float *depth,
                         const int32_t *radius, const int32_t *tiles,
                         int64_t n, int W, int H, void *workspace,

@@ -47,10 +47,12 @@ inline unsigned blend_grid(int ntiles) { return unsigned(((ntiles + 7) / 8) * 32
 // exp(x) as 2^(x log2 e) on the hardware exp2 (v_exp_f32).
 MSPLAT_DEV float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 
+// One staged Gaussian is a 16-byte aligned record (u, v, a, b | c, o, f0, f1 |
+// f2 ...), so the inner loops fetch it with 128-bit LDS broadcast reads.
 template <int CMAX>
 struct BlendStage {
-  float u[kBlendLanes], v[kBlendLanes], a[kBlendLanes], b[kBlendLanes], c[kBlendLanes], o[kBlendLanes];
-  float f[CMAX * kBlendLanes];  // channel-major: f[ch * 64 + slot]
+  static constexpr int kQuads = (6 + CMAX + 3) / 4;  // float4s per record
+  float4 rec[kBlendLanes * kQuads];
 };
 
 // Stage Gaussian g into LDS slot `slot`: channels [c0, c0 + cc) of its row.
@@ -58,16 +60,44 @@ template <int CMAX>
 MSPLAT_DEV void stage_gaussian(BlendStage<CMAX> &s, int slot, int g, const float *__restrict__ uv,
                                const float *__restrict__ conic, const float *__restrict__ opacity,
                                const float *__restrict__ feature, int C, int c0, int cc) {
-  s.u[slot] = uv[2 * int64_t(g)];
-  s.v[slot] = uv[2 * int64_t(g) + 1];
-  s.a[slot] = conic[3 * int64_t(g)];
-  s.b[slot] = conic[3 * int64_t(g) + 1];
-  s.c[slot] = conic[3 * int64_t(g) + 2];
-  s.o[slot] = opacity[g];
+  constexpr int NQ = BlendStage<CMAX>::kQuads;
+  float r[NQ * 4];
+  r[0] = uv[2 * int64_t(g)];
+  r[1] = uv[2 * int64_t(g) + 1];
+  r[2] = conic[3 * int64_t(g)];
+  r[3] = conic[3 * int64_t(g) + 1];
+  r[4] = conic[3 * int64_t(g) + 2];
+  r[5] = opacity[g];
   const float *fr = feature + int64_t(g) * C + c0;
 #pragma unroll
-  for (int ch = 0; ch < CMAX; ++ch)
-    if (ch < cc) s.f[ch * kBlendLanes + slot] = fr[ch];
+  for (int i = 6; i < NQ * 4; ++i) r[i] = (i - 6 < cc) ? fr[i - 6] : 0.f;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) s.rec[slot * NQ + q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+}
+
+// Geometry of record j: (u, v, a, b) and (c, o, f0, f1).
+template <int CMAX>
+MSPLAT_DEV void stage_geom(const BlendStage<CMAX> &s, int j, float4 &g0, float4 &g1) {
+  constexpr int NQ = BlendStage<CMAX>::kQuads;
+  g0 = s.rec[j * NQ];
+  g1 = s.rec[j * NQ + 1];
+}
+
+// All CMAX channels of record j (channels 0, 1 come from g1).
+template <int CMAX>
+MSPLAT_DEV void stage_features(const BlendStage<CMAX> &s, int j, const float4 &g1, float (&f)[CMAX]) {
+  constexpr int NQ = BlendStage<CMAX>::kQuads;
+  f[0] = g1.z;
+  if constexpr (CMAX > 1) f[1] = g1.w;
+#pragma unroll
+  for (int q = 2; q < NQ; ++q) {
+    const float4 v = s.rec[j * NQ + q];
+    const int b = 4 * q - 6;
+    if (b < CMAX) f[b] = v.x;
+    if (b + 1 < CMAX) f[b + 1] = v.y;
+    if (b + 2 < CMAX) f[b + 2] = v.z;
+    if (b + 3 < CMAX) f[b + 3] = v.w;
+  }
 }
 
 // power = -0.5 (a dx^2 + c dy^2) - b dx dy with d = mean - pixel
@@ -107,18 +137,22 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(
     for (int j = 0; j < cnt; ++j) {
       if (!done) {
         ++contributor;
-        const float dx = s.u[j] - pxf, dy = s.v[j] - pyf;
-        const float power = gauss_power(s.a[j], s.b[j], s.c[j], dx, dy);
+        float4 g0, g1;
+        stage_geom(s, j, g0, g1);
+        const float dx = g0.x - pxf, dy = g0.y - pyf;
+        const float power = gauss_power(g0.z, g0.w, g1.x, dx, dy);
         if (!(power > 0.f)) {  // a NaN power is not skipped, as in the reference
-          const float alpha = fminf(0.99f, s.o[j] * fast_exp(power));
+          const float alpha = fminf(0.99f, g1.y * fast_exp(power));
           if (!(alpha < 1.f / 255.f)) {
             const float nT = T * (1.f - alpha);
             if (nT < 0.0001f) {
               done = true;
             } else {
+              float f[CMAX];
+              stage_features(s, j, g1, f);
 #pragma unroll
               for (int ch = 0; ch < CMAX; ++ch)
-                if (ch < cc) F[ch] += s.f[ch * kBlendLanes + j] * alpha * T;
+                if (ch < cc) F[ch] += f[ch] * alpha * T;
               T = nT;
               last = contributor;
             }
@@ -187,8 +221,10 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
     for (int j = cnt - 1; j >= 0; --j) {
       const int rel = lo + j - start;  // 0-based position in the tile list
       bool contrib = rel < last;
-      const float dx = s.u[j] - pxf, dy = s.v[j] - pyf;
-      const float a = s.a[j], b = s.b[j], c = s.c[j], op = s.o[j];
+      float4 g0, g1;
+      stage_geom(s, j, g0, g1);
+      const float dx = g0.x - pxf, dy = g0.y - pyf;
+      const float a = g0.z, b = g0.w, c = g1.x, op = g1.y;
       float G = 0.f, alpha = 0.f;
       if (contrib) {
         const float power = gauss_power(a, b, c, dx, dy);
@@ -207,13 +243,18 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
 #pragma unroll
       for (int ch = 0; ch < CMAX; ++ch) vf[ch] = 0.f;
       if (contrib) {
-        T = T / (1.f - alpha);
+        // 1 / (1 - alpha) on the hardware reciprocal (the reference divides
+        // with fast-math); 1 - alpha >= 0.01, so it is well conditioned.
+        const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+        T = T * inv;
         const float dcol = alpha * T;
         float dL_dalpha = 0.f;
+        float fe[CMAX];
+        stage_features(s, j, g1, fe);
 #pragma unroll
         for (int ch = 0; ch < CMAX; ++ch) {
           if (ch < cc) {
-            const float f = s.f[ch * kBlendLanes + j];
+            const float f = fe[ch];
             accum[ch] = last_alpha * last_f[ch] + (1.f - last_alpha) * accum[ch];
             last_f[ch] = f;
             dL_dalpha += (f - accum[ch]) * g[ch];
@@ -222,7 +263,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         }
         dL_dalpha *= T;
         last_alpha = alpha;
-        dL_dalpha += (-T_final / (1.f - alpha)) * bg_dot;
+        dL_dalpha += (-T_final * inv) * bg_dot;
         const float dL_dG = op * dL_dalpha;
         const float gdx = ((-G) * dx) * a - (G * dy) * b;
         const float gdy = ((-G) * dy) * c - (G * dx) * b;

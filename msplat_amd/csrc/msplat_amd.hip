@@ -62,7 +62,9 @@ struct PrepLayout {
   size_t area, ka, va, kb, vb, offs, hist, sums, flag, total;
 };
 
-PrepLayout prep_layout(int64_t n) {
+// (W, H are part of the size query so that the layout may depend on the
+// tile grid; it currently does not.)
+PrepLayout prep_layout(int64_t n, int W, int H) {
   Carver c;
   PrepLayout L;
   const size_t un = size_t(n);
@@ -78,6 +80,8 @@ PrepLayout prep_layout(int64_t n) {
   if (sortk::scan_tiles(n) > ns) ns = sortk::scan_tiles(n);
   L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
   L.flag = c.take(4);
+  (void)W;
+  (void)H;
   L.total = c.off;
   return L;
 }
@@ -86,10 +90,12 @@ struct EmitLayout {
   size_t ca, ga, cb, gb, owner, hist, sums, total;
 };
 
-EmitLayout emit_layout(int64_t k) {
+EmitLayout emit_layout(int64_t k, int W, int H) {
   Carver c;
   EmitLayout L;
   const size_t uk = size_t(k);
+  (void)W;
+  (void)H;
   L.ca = c.take(uk * 4);
   L.ga = c.take(uk * 4);
   L.cb = c.take(uk * 4);
@@ -358,14 +364,17 @@ int msplat_compute_sh_backward(const float *shs, const float *dirs, const uint8_
 
 // ---------------------------------------------------------------- sort
 
-size_t msplat_sort_prepare_workspace_bytes(int64_t n) { return prep_layout(n < 0 ? 0 : n).total; }
+size_t msplat_sort_prepare_workspace_bytes(int64_t n, int W, int H) {
+  if (!valid_image(W, H)) return 0;
+  return prep_layout(n < 0 ? 0 : n, W, H).total;
+}
 
 int msplat_sort_prepare(const float *uv, const float *depth, const int32_t *radius, const int32_t *tiles, int64_t n,
                         int W, int H, void *workspace, size_t workspace_bytes, int64_t *d_count, void *stream) {
   REQUIRE(n >= 0 && n <= int64_t(INT32_MAX), "sort_prepare: n = %lld", (long long)n);
   REQUIRE(valid_image(W, H), "sort_prepare: bad image size %dx%d", W, H);
   REQUIRE(d_count, "sort_prepare: null count pointer");
-  const PrepLayout L = prep_layout(n);
+  const PrepLayout L = prep_layout(n, W, H);
   if (workspace_bytes < L.total)
     return fail(MSPLAT_ERR_WORKSPACE, "sort_prepare: workspace %zu < %zu bytes", workspace_bytes, L.total);
   REQUIRE(workspace, "sort_prepare: null workspace");
@@ -394,9 +403,8 @@ int msplat_sort_prepare(const float *uv, const float *depth, const int32_t *radi
 
 size_t msplat_sort_emit_workspace_bytes(int64_t n, int64_t k, int W, int H) {
   (void)n;
-  (void)W;
-  (void)H;
-  return emit_layout(k < 0 ? 0 : k).total;
+  if (!valid_image(W, H)) return 0;
+  return emit_layout(k < 0 ? 0 : k, W, H).total;
 }
 
 int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius, int64_t n, int W, int H,
@@ -408,7 +416,7 @@ int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius,
   if (k > int64_t(INT32_MAX))
     return fail(MSPLAT_ERR_CAPACITY, "sort_emit: %lld intersections exceed int32 indexing", (long long)k);
   REQUIRE(tile_range, "sort_emit: null tile_range");
-  const EmitLayout L = emit_layout(k);
+  const EmitLayout L = emit_layout(k, W, H);
   if (workspace_bytes < L.total)
     return fail(MSPLAT_ERR_WORKSPACE, "sort_emit: workspace %zu < %zu bytes", workspace_bytes, L.total);
   hipStream_t s = as_stream(stream);
@@ -416,7 +424,7 @@ int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius,
   launch_fill_zero(tile_range, 2 * int64_t(T), s);
   if (k == 0) return launched("sort_emit");
   REQUIRE(uv && depth && radius && prepared && workspace && idx_sorted, "sort_emit: null pointer");
-  const PrepLayout P = prep_layout(n);
+  const PrepLayout P = prep_layout(n, W, H);
   const uint32_t *order = at<uint32_t>(prepared, P.va);
   const int64_t *offs = at<int64_t>(prepared, P.offs);
   uint32_t *ca = at<uint32_t>(workspace, L.ca), *ga = at<uint32_t>(workspace, L.ga);
@@ -440,7 +448,7 @@ int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius,
     sk = dk;
     sv = dv;
   }
-  sortk::range_kernel<<<grid_1d(k, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, k, tile_range);
+  sortk::range_vec_kernel<<<grid_1d((k + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, k, tile_range);
   if (key_sorted)
     sortk::sorted_keys_kernel<<<grid_1d(k, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, idx_sorted, depth, k,
                                                                                       key_sorted);

@@ -249,57 +249,6 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
   }
 }
 
-// Rank every key among equal digits of its round (wave-level match through
-// ballots, then a prefix over the 4 waves) and scatter it.  Rounds are taken
-// in input order, so equal digits keep their input order (stable).
-__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(
-    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t len, int shift, int bits,
-    const uint32_t *__restrict__ offsets, int64_t ntiles, uint32_t *__restrict__ kout,
-    uint32_t *__restrict__ vout) {
-  constexpr int kWaves = kThreads / kWave;
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t cnt[kWaves][256];
-  const int tid = threadIdx.x, wave = tid / kWave;
-  const uint32_t mask = (1u << bits) - 1u;
-  run[tid] = tid < (1 << bits) ? offsets[int64_t(tid) * ntiles + blockIdx.x] : 0u;
-#pragma unroll
-  for (int w = 0; w < kWaves; ++w) cnt[w][tid] = 0u;
-  __syncthreads();
-  const int64_t base = int64_t(blockIdx.x) * kRadixTile;
-  for (int it = 0; it < kItems; ++it) {
-    const int64_t i = base + it * kThreads + tid;
-    const bool valid = i < len;
-    const uint32_t key = valid ? kin[i] : 0u;
-    const uint32_t d = (key >> shift) & mask;
-    uint64_t peers = __ballot(valid);
-    for (int b = 0; b < bits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(valid && bit);
-      peers &= bit ? bb : ~bb;
-    }
-    const uint32_t rank =
-        __builtin_amdgcn_mbcnt_hi(uint32_t(peers >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(peers), 0u));
-    if (valid && rank == 0) cnt[wave][d] = uint32_t(__popcll(peers));
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = run[d] + rank;
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) pos += (w < wave) ? cnt[w][d] : 0u;
-      kout[pos] = key;
-      vout[pos] = vin[i];
-    }
-    __syncthreads();
-    uint32_t add = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      add += cnt[w][tid];
-      cnt[w][tid] = 0u;
-    }
-    run[tid] += add;
-    __syncthreads();
-  }
-}
-
 struct RadixBuffers {
   uint32_t *hist;  // 256 * radix_tiles(len)
   int64_t *sums;   // scan_tiles(256 * radix_tiles(len))
@@ -337,6 +286,37 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(const float *__restri
   keys[i] = fbits(depth[i]);
   vals[i] = uint32_t(i);
   if (tiles && tiles[i] != a) atomicOr(flag, 1);
+}
+
+// tile_range[c-1] = [first, last+1) of class c in the sorted classes; class 0
+// (negative depth) has no tile.  Each thread reads 4 consecutive classes with
+// one 16-byte load and compares them with their neighbours.  The caller
+// zero-fills tile_range (unset tiles stay (0, 0)).
+__global__ __launch_bounds__(kThreads) void range_vec_kernel(const uint32_t *__restrict__ cls, int64_t k,
+                                                             int32_t *__restrict__ range) {
+  const int64_t i0 = (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4;
+  if (i0 >= k) return;
+  uint32_t c[6];  // c[0] = class at i0 - 1, c[1..4] = i0..i0+3, c[5] = i0 + 4 (0xffffffff outside)
+  c[0] = i0 > 0 ? cls[i0 - 1] : 0xffffffffu;
+  if (i0 + 4 <= k) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(cls + i0);
+    c[1] = v.x;
+    c[2] = v.y;
+    c[3] = v.z;
+    c[4] = v.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[1 + j] = i0 + j < k ? cls[i0 + j] : 0xffffffffu;
+  }
+  c[5] = i0 + 4 < k ? cls[i0 + 4] : 0xffffffffu;
+#pragma unroll
+  for (int j = 1; j <= 4; ++j) {
+    const uint32_t cj = c[j];
+    if (cj == 0u || cj == 0xffffffffu) continue;
+    const int64_t i = i0 + j - 1;
+    if (c[j - 1] != cj) range[2 * (int64_t(cj) - 1)] = int32_t(i);
+    if (c[j + 1] != cj) range[2 * (int64_t(cj) - 1) + 1] = int32_t(i + 1);
+  }
 }
 
 __global__ void finish_count_kernel(const int32_t *__restrict__ flag, int64_t *__restrict__ count, int64_t n) {
@@ -434,121 +414,6 @@ __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
     gbase += kThreads;
     __syncthreads();
   }
-}
-
-// Emit (class, id) for every tile of every Gaussian, Gaussians in depth order,
-// load-balanced: workgroup b owns depth-ordered Gaussians [256 b, 256 b + 256),
-// whose pairs form one contiguous run of the output.  Every thread writes
-// output slots p = tid, tid + 256, ... of that run (coalesced stores) and finds
-// the Gaussian owning slot p by binary search over the run's start offsets.
-__global__ __launch_bounds__(kThreads) void emit_lb_kernel(const float *__restrict__ uv,
-                                                           const float *__restrict__ depth,
-                                                           const int32_t *__restrict__ radius,
-                                                           const uint32_t *__restrict__ order,
-                                                           const int64_t *__restrict__ offs, int64_t n, int gx,
-                                                           int gy, uint32_t *__restrict__ cls,
-                                                           uint32_t *__restrict__ gid) {
-  __shared__ int32_t s_rel[kThreads];
-  __shared__ int32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
-  __shared__ uint32_t s_g[kThreads], s_c[kThreads];
-  const int tid = threadIdx.x;
-  const int64_t first = int64_t(blockIdx.x) * kThreads;
-  const int64_t i = first + tid;
-  int a = 0;
-  if (i < n) {
-    const uint32_t g = order[i];
-    const int r = radius[g];
-    s_g[tid] = g;
-    if (r > 0) {
-      const TileRect t = tile_rect(uv[2 * g], uv[2 * g + 1], r, gx, gy);
-      a = t.area();
-      s_x0[tid] = t.x0;
-      s_y0[tid] = t.y0;
-      s_w[tid] = imax(1, t.x1 - t.x0);
-      // depth sign bit set: the reference key's upper word is all ones
-      s_c[tid] = (fbits(depth[g]) >> 31) ? 0u : 1u;
-    }
-  }
-  int64_t total64;
-  s_rel[tid] = int32_t(block_excl_scan(a, &total64));
-  __syncthreads();
-  const int total = int(total64);
-  if (total == 0) return;
-  const int64_t base = offs[first];
-  int p = tid;
-  if (p >= total) return;
-  int lo = 0, hi = kThreads - 1;  // largest j with s_rel[j] <= p (skips empty runs)
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (s_rel[mid] <= p) lo = mid; else hi = mid - 1;
-  }
-  for (; p < total; p += kThreads) {
-    // slots advance by 256; walk forward to the run containing p
-    while (lo + 1 < kThreads && s_rel[lo + 1] <= p) ++lo;
-    const int t = p - s_rel[lo];
-    const int w = s_w[lo];
-    // t / w through the fp32 reciprocal, then corrected to the exact quotient
-    int q = int(float(t) * __builtin_amdgcn_rcpf(float(w)));
-    int r = t - q * w;
-    if (r < 0) { --q; r += w; } else if (r >= w) { ++q; r -= w; }
-    const int ty = s_y0[lo] + q, tx = s_x0[lo] + r;
-    cls[base + p] = s_c[lo] ? uint32_t(ty * gx + tx) + 1u : 0u;
-    gid[base + p] = s_g[lo];
-  }
-}
-
-// Emit (class, id) for every tile of every Gaussian, Gaussians in depth order.
-// One wave walks its 64 Gaussians one at a time and writes each Gaussian's
-// pairs with all 64 lanes, so large splats do not serialise on one thread.
-__global__ __launch_bounds__(kThreads) void emit_kernel(const float *__restrict__ uv,
-                                                        const float *__restrict__ depth,
-                                                        const int32_t *__restrict__ radius,
-                                                        const uint32_t *__restrict__ order,
-                                                        const int64_t *__restrict__ offs, int64_t n, int gx, int gy,
-                                                        uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
-  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  const int lane = threadIdx.x % kWave;
-  int x0 = 0, y0 = 0, w = 1, a = 0;
-  uint32_t g = 0, cbase = 0;
-  int64_t o = 0;
-  if (i < n) {
-    g = order[i];
-    const int r = radius[g];
-    if (r > 0) {
-      const TileRect t = tile_rect(uv[2 * g], uv[2 * g + 1], r, gx, gy);
-      a = t.area();
-      x0 = t.x0;
-      y0 = t.y0;
-      w = imax(1, t.x1 - t.x0);
-      // depth sign bit set: the reference key's upper word is all ones
-      cbase = (fbits(depth[g]) >> 31) ? 0u : 1u;
-      o = offs[i];
-    }
-  }
-  for (int j = 0; j < kWave; ++j) {
-    const int aj = __shfl(a, j, kWave);
-    if (aj == 0) continue;
-    const int x0j = __shfl(x0, j, kWave), y0j = __shfl(y0, j, kWave), wj = __shfl(w, j, kWave);
-    const uint32_t gj = __shfl(g, j, kWave), cj = __shfl(cbase, j, kWave);
-    const int64_t oj = __shfl(o, j, kWave);
-    for (int t = lane; t < aj; t += kWave) {
-      const int ty = y0j + t / wj, tx = x0j + t % wj;
-      cls[oj + t] = cj ? uint32_t(ty * gx + tx) + 1u : 0u;
-      gid[oj + t] = gj;
-    }
-  }
-}
-
-// tile_range[c-1] = [first, last+1) of class c in the sorted classes;
-// class 0 (negative depth) has no tile.  Unset tiles stay (0, 0).
-__global__ __launch_bounds__(kThreads) void range_kernel(const uint32_t *__restrict__ cls, int64_t k,
-                                                         int32_t *__restrict__ range) {
-  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= k) return;
-  const uint32_t c = cls[i];
-  if (c == 0) return;
-  if (i == 0 || cls[i - 1] != c) range[2 * (int64_t(c) - 1)] = int32_t(i);
-  if (i == k - 1 || cls[i + 1] != c) range[2 * (int64_t(c) - 1) + 1] = int32_t(i + 1);
 }
 
 // The reference's 64-bit keys, rebuilt from (class, id) after sorting.

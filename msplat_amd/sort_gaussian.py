@@ -43,7 +43,7 @@ def sort_gaussian_keys(uv, depth, W: int, H: int, radius, tiles=None, want_keys:
     gx, gy = tile_grid(int(W), int(H))
     with torch.cuda.device(dev):
         stream = _lib.stream_ptr()
-        prep_bytes = int(_lib.query("msplat_sort_prepare_workspace_bytes", n))
+        prep_bytes = int(_lib.query("msplat_sort_prepare_workspace_bytes", n, int(W), int(H)))
         prep = torch.empty(prep_bytes, dtype=torch.uint8, device=dev)
         count = torch.empty(1, dtype=torch.int64, device=dev)
         _lib.call("msplat_sort_prepare", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), _lib.ptr(t), n, int(W), int(H),

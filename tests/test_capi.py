@@ -74,8 +74,8 @@ def test_argument_errors(lib, call, needle):
 
 
 def test_workspace_sizes_grow(lib):
-    a = lib.msplat_sort_prepare_workspace_bytes(1000)
-    b = lib.msplat_sort_prepare_workspace_bytes(1_000_000)
+    a = lib.msplat_sort_prepare_workspace_bytes(1000, 1920, 1080)
+    b = lib.msplat_sort_prepare_workspace_bytes(1_000_000, 1920, 1080)
     assert 0 < a < b and b >= 1_000_000 * 28
     assert lib.msplat_sort_emit_workspace_bytes(10, 5_000_000, 1920, 1080) >= 5_000_000 * 16
 
