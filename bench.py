@@ -5,12 +5,18 @@ Workload (the headline configuration): N = 1,000,000 Gaussians in
 features = compute_sh(degree 3, [N,3,16] ~ N(0,0.3)) concatenated with the
 projected depth (C = 4), L2 loss against a U(0,1) target, backward to every
 Gaussian parameter.  A step is one forward + backward of that view; inputs
-are resident in HBM before the timed region.
+are resident in HBM before the timed region.  The blending launch parameters
+are chosen by msplat_amd.autotune before the warm-up (--no-autotune keeps
+the defaults).
 
-Prints ONE JSON line (metric Mpix/s = W*H*steps / seconds / 1e6), with the
-roofline figure of the dominant stage (HIP events on its launch stream
-inside the timed region) and a CPU baseline (the oracle, on this host).
-Per-stage timings go to stderr.
+Prints ONE JSON line (metric Mpix/s = W*H*steps / seconds / 1e6) with
+  roofline      of the kernel with the largest share of the timed region,
+                timed by HIP events the library records on its launch stream;
+  roofline_hbm  the same for the heaviest HBM-bound kernel;
+  cpu_baseline  the CPU oracle on this host over a bounded sample.
+Per-kernel timings go to stderr.  ``traffic`` (HBM bytes per launch) comes
+from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under
+profiles/ when it covers the kernel, else null.
 """
 
 from __future__ import annotations
@@ -26,7 +32,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP32_VALU_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "rasterize_1080p_pmc_traffic.txt")
+
+# Kernels the library times (msplat_timing_read names).
+KERNELS = ("preprocess_fwd", "preprocess_bwd", "sh_fwd", "sh_bwd", "depth_sort", "emit", "class_sort_pass",
+           "rasterize_fwd", "rasterize_bwd")
 
 
 def log(*a):
@@ -60,24 +71,61 @@ def build_step(cfg, dev):
     return step, P, cam
 
 
-def algorithmic_work(name, n, k, C, D):
-    """Algorithmic bytes moved by one call of a stage (HBM-bound stages)."""
+def algorithmic_work(name, w):
+    """(bound, work per launch, unit) of a library kernel for this workload.
+
+    w: n Gaussians, nvis of them binned, count cell entries, C rendered
+    channels, Csh / D SH channels / coefficients, E = sum of ncontrib.
+    Bytes are the algorithmic minimum each launch must move; FLOPs follow the
+    per-(pixel, Gaussian) operation counts of the blending forward (16 + 2C)
+    and adjoint (30 + 6C)."""
+    n, C, Csh, D = w["n"], w["C"], w["Csh"], w["D"]
     table = {
-        "msplat_project_point_forward": 24 * n,          # xyz in, uv + depth out
-        "msplat_project_point_backward": 40 * n,         # xyz, depth, dL/duv, dL/ddepth in; dL/dxyz out
-        "msplat_preprocess_forward": 72 * n,             # xyz, scale, rot in; uv, depth, conic, radius, tiles out
-        "msplat_preprocess_backward": 108 * n,           # + depth, radius, dL/duv, dL/dconic in; 3 grads out
-        "msplat_compute_sh_forward": n * (4 * C * D + 12 + 4 * C),
-        "msplat_compute_sh_backward": n * (8 * C * D + 24 + 4 * C),
-        # depth-key sort of n pairs: 4 passes x (hist read 4 + pair read 8 + pair write 8), + 28 B/Gaussian in/out
-        "msplat_sort_prepare": n * (28 + 4 * 20 + 24),
-        # per intersection: pair written once, 2 class passes x 20 B, range read 4; per Gaussian 28 B read
-        "msplat_sort_emit": k * (8 + 2 * 20 + 4) + 28 * n,
+        "preprocess_fwd": ("hbm", 72 * n),          # xyz, scale, rot in; uv, depth, conic, radius, tiles out
+        "preprocess_bwd": ("hbm", 108 * n),         # + depth, radius, dL/duv, dL/dconic in; 3 grads out
+        "sh_fwd": ("hbm", n * (4 * Csh * D + 12 + 4 * Csh)),
+        "sh_bwd": ("hbm", n * (8 * Csh * D + 24 + 4 * Csh)),
+        "depth_sort": ("hbm", n * 4 * 20),          # 4 passes x (key read for the histogram, pair read, pair write)
+        "emit": ("hbm", w["count"] * 8 + w["nvis"] * 28),
+        "class_sort_pass": ("hbm", w["count"] * 20),
+        "rasterize_fwd": ("flop", w["E"] * (16 + 2 * C)),
+        "rasterize_bwd": ("flop", w["E"] * (30 + 6 * C)),
     }
     return table.get(name)
 
 
-def cpu_baseline(cfg, budget_rows: int = 128) -> dict:
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (None if absent)."""
+    if not os.path.exists(PMC_SUMMARY):
+        return None
+    for line in open(PMC_SUMMARY):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == "traffic" and parts[1] == kernel:
+            kv = dict(p.split("=", 1) for p in parts[2:] if "=" in p)
+            if "bytes_per_launch" in kv:
+                return float(kv["bytes_per_launch"])
+    return None
+
+
+def roofline(name, st, w):
+    bound, work = algorithmic_work(name, w)
+    sec = st["avg_ms"] * 1e-3
+    if bound == "hbm":
+        ach = work / sec / 1e9
+        r = dict(kernel=name, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                 frac=round(ach / HBM_PEAK_GBS, 4))
+    else:
+        ach = work / sec / 1e12
+        r = dict(kernel=name, bound="mfma", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                 frac=round(ach / FP32_PEAK_TFLOPS, 4),
+                 pipe="fp32 vector ALU: blending is not a contraction, no matrix instructions are used")
+    r["avg_launch_ms"] = round(st["avg_ms"], 4)
+    r["work_per_launch"] = work
+    r["traffic"] = read_traffic(name)
+    return r
+
+
+def cpu_baseline(cfg, rows: int = 64) -> dict:
     """Time the CPU oracle (numpy + C, one thread) on a bounded sample of the same frame."""
     import numpy as np
 
@@ -98,26 +146,32 @@ def cpu_baseline(cfg, budget_rows: int = 128) -> dict:
     conic, radius, tiles = geometry.ewa_forward(g["xyz"], cov, intr, extr, uv, W, H, vis)
     idx, rng, _ = sort.sort_gaussian(uv, depth, radius, W, H)
     t_geo_fwd = time.perf_counter() - t0
-    # blending: a band of tile rows, forward + backward, scaled to the frame
-    rows = min(H, budget_rows)
-    band_rng = rng.reshape((H + 15) // 16, (W + 15) // 16, 2)[: (rows + 15) // 16].reshape(-1, 2)
+    # blending: a band of tile rows through the middle of the frame, forward +
+    # backward, scaled to the whole frame
+    gy, gx = (H + 15) // 16, (W + 15) // 16
+    band = max(1, rows // 16)
+    r0 = max(0, gy // 2 - band // 2)
+    keep = np.zeros((gy, gx, 2), np.int32)
+    keep[r0:r0 + band] = rng.reshape(gy, gx, 2)[r0:r0 + band]
     t1 = time.perf_counter()
-    out, fT, nc, _ = blend.blend_forward(uv, conic, g["opacity"], feats, idx, band_rng, 0.0, W, rows)
+    out, fT, nc, _ = blend.blend_forward(uv, conic, g["opacity"], feats, idx, keep.reshape(-1, 2), 0.0, W, H)
     dout = np.ones_like(out)
-    duv, dco, dop, dfe = blend.blend_backward(uv, conic, g["opacity"], feats, idx, band_rng, 0.0, W, rows, fT, nc,
-                                              dout)
+    duv, dco, dop, dfe = blend.blend_backward(uv, conic, g["opacity"], feats, idx, keep.reshape(-1, 2), 0.0, W, H,
+                                              fT, nc, dout)
     t_blend_band = time.perf_counter() - t1
     t2 = time.perf_counter()
-    dxyz_e, dcov, _, _ = geometry.ewa_backward(g["xyz"], cov, intr, extr, radius, dco)
+    _, dcov, _, _ = geometry.ewa_backward(g["xyz"], cov, intr, extr, radius, dco)
     geometry.cov3d_backward(g["scale"], g["rotate"], dcov, vis)
     geometry.project_backward(g["xyz"], intr, extr, depth, duv)
     sh.sh_backward(g["shs"], dirs, dfe[:, :3])
     t_geo_bwd = time.perf_counter() - t2
-    frame_s = t_geo_fwd + t_geo_bwd + t_blend_band * (H / rows)
-    return dict(value=W * H / frame_s / 1e6, unit="Mpix/s", cores=1, kind="port",
-                sample=(f"same frame (N={len(g['xyz'])}, {W}x{H}, C=4): projection, SH, covariance, EWA, sort and "
-                        f"their adjoints timed in full ({t_geo_fwd + t_geo_bwd:.1f} s); blending forward+backward "
-                        f"timed on the first {rows} rows ({t_blend_band:.1f} s) and scaled x{H / rows:.2f}"),
+    scale = gy / band
+    frame_s = t_geo_fwd + t_geo_bwd + t_blend_band * scale
+    return dict(value=round(W * H / frame_s / 1e6, 5), unit="Mpix/s", cores=1, kind="port",
+                sample=(f"same frame (N={len(g['xyz'])}, {W}x{H}, C=4): projection, SH, covariance, EWA, tile sort "
+                        f"and their adjoints timed in full ({t_geo_fwd + t_geo_bwd:.1f} s); blending forward + "
+                        f"backward (C oracle, one thread) timed on tile rows {r0}..{r0 + band - 1} of {gy} "
+                        f"({t_blend_band:.1f} s) and scaled x{scale:.1f}"),
                 seconds=time.perf_counter() - t0)
 
 
@@ -128,73 +182,114 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=128)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=64, help="pixel rows of the CPU blending sample")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (a PMC calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by view; "
-              "see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the rasterizer shards by view "
+              "across GPUs, which this one-GPU build does not run)", file=sys.stderr)
         sys.exit(2)
 
     import torch
 
-    from msplat_amd import scenes
-    from msplat_amd.profiling import stage_timer
+    from msplat_amd import _lib, autotune, scenes, tuning
+    from msplat_amd.binning import bin_gaussians, rasterize_forward
+    from msplat_amd.rasterization import preprocess_forward
 
     dev = torch.device("cuda:0")
     cfg = scenes.config_sh3_1080p(n=args.n)
     step, P, cam = build_step(cfg, dev)
     W, H = cam.W, cam.H
-    log(f"workload: N={args.n} {W}x{H} C=4 (SH3 RGB + depth); warmup {args.warmup}, steps {args.steps}")
+    C = 4
+    log(f"workload: N={args.n} {W}x{H} C={C} (SH3 RGB + depth); warmup {args.warmup}, steps {args.steps}")
+
+    if args.calibrate_copy_mib:
+        src = torch.ones(args.calibrate_copy_mib * (1 << 18), device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
+        del src, dst
+
+    if not args.no_autotune:
+        with torch.no_grad():
+            _, depth = preprocess_forward(P["xyz"], P["scale"], P["rotate"], cam.intr, cam.extr, W, H)[:2]
+            feat = torch.cat([torch.rand(args.n, 3, device=dev), depth], 1)
+        best, rep = autotune.autotune(P["xyz"], P["scale"], P["rotate"], P["opacity"], feat, cam.intr, cam.extr,
+                                      W, H, 0.0)
+        log(f"autotune: {best} ({rep[(best.cell, best.ppl_fwd, best.ppl_bwd)]['total']:.3f} ms bin+fwd+bwd)")
+    cfg_used = tuning.config_for(W, H, C)
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    with stage_timer() as st:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-    stages = st.summary()
+    lib = _lib.load()
+    for k in KERNELS:  # drop anything recorded before the timed region
+        _lib.timing_read(k)
+    lib.msplat_timing_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.msplat_timing_enable(0)
     sec = t1 - t0
     value = W * H * args.steps / sec / 1e6
+    kern = {}
+    for k in KERNELS:
+        total, launches = _lib.timing_read(k)
+        if launches:
+            kern[k] = dict(total_ms=total, launches=launches, avg_ms=total / launches)
 
-    # intersections of this view, for the sort's byte count
-    from msplat_amd.rasterization import preprocess_forward
+    # workload figures for the algorithmic work
     with torch.no_grad():
-        _, _, _, radius, tiles = preprocess_forward(P["xyz"], P["scale"], P["rotate"], cam.intr, cam.extr, W, H)
+        uv, depth, conic, radius, tiles = preprocess_forward(P["xyz"], P["scale"], P["rotate"], cam.intr,
+                                                             cam.extr, W, H)
         K = int(tiles.sum().item())
-    n, C, D = args.n, 3, 16
-    total_stage_ms = sum(v["total_ms"] for v in stages.values())
-    for name, v in sorted(stages.items(), key=lambda kv: -kv[1]["total_ms"]):
-        w = algorithmic_work(name, n, K, C, D)
-        bw = f"{w / (v['avg_ms'] * 1e-3) / 1e9:8.1f} GB/s" if w else ""
-        log(f"  {name:40s} x{v['count'] // args.steps:<2d} avg {v['avg_ms']:8.3f} ms  "
-            f"{100 * v['total_ms'] / total_stage_ms:5.1f}%  {bw}")
-    log(f"K = {K} intersections; step {1e3 * sec / args.steps:.2f} ms")
+        bins = bin_gaussians(uv, depth, W, H, radius, cfg_used.cell)
+        nvis = int(torch.unique(bins.cell_ids).numel())
+        feats = torch.zeros(args.n, C, device=dev)
+        _, _, ncontrib, _ = rasterize_forward(uv, conic, P["opacity"], feats, bins, 0.0, W, H, cfg_used.ppl_fwd)
+        E = int(ncontrib.sum().item())
+    w = dict(n=args.n, nvis=nvis, count=int(bins.cell_ids.numel()), C=C, Csh=3, D=16, E=E)
+    step_ms = 1e3 * sec / args.steps
+    for name, v in sorted(kern.items(), key=lambda kv: -kv[1]["total_ms"]):
+        bound, work = algorithmic_work(name, w)
+        rate = (f"{work / (v['avg_ms'] * 1e-3) / 1e9:8.1f} GB/s" if bound == "hbm"
+                else f"{work / (v['avg_ms'] * 1e-3) / 1e12:8.2f} TFLOP/s")
+        log(f"  {name:18s} x{v['launches'] // args.steps:<2d} avg {v['avg_ms']:8.4f} ms  "
+            f"{100 * v['total_ms'] / args.steps / step_ms:5.1f}% of step  {rate}")
+    log(f"K = {K} tile intersections (never materialised), {w['count']} cell entries, {nvis} binned Gaussians, "
+        f"sum ncontrib = {E}; step {step_ms:.3f} ms")
 
-    # dominant HBM-bound stage's roofline
-    hbm = [(v["total_ms"], name) for name, v in stages.items() if algorithmic_work(name, n, K, C, D)]
-    _, dom = max(hbm)
-    w = algorithmic_work(dom, n, K, C, D)
-    ach = w / (stages[dom]["avg_ms"] * 1e-3) / 1e9
-    roofline = dict(kernel=dom, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                    stage_share=round(stages[dom]["total_ms"] / total_stage_ms, 3))
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    roof = roofline(dom, kern[dom], w)
+    roof["step_share"] = round(kern[dom]["total_ms"] / args.steps / step_ms, 3)
+    hbm = [k for k in kern if algorithmic_work(k, w)[0] == "hbm"]
+    roof_hbm = None
+    if hbm:
+        hd = max(hbm, key=lambda k: kern[k]["total_ms"])
+        roof_hbm = roofline(hd, kern[hd], w)
 
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         cpu = cpu_baseline(cfg, args.cpu_rows)
-        log(f"cpu baseline: {cpu['value']:.4f} Mpix/s in {cpu.pop('seconds'):.1f} s")
+        log(f"cpu baseline: {cpu['value']} Mpix/s in {cpu.pop('seconds'):.1f} s")
 
     line = dict(metric="fwd+bwd Mpix/s at N=1M,1080p", value=round(value, 3), unit="Mpix/s", n_gpus=1,
-                steps=args.steps, warmup=args.warmup, ms_per_step=round(1e3 * sec / args.steps, 3),
+                steps=args.steps, warmup=args.warmup, ms_per_step=round(step_ms, 3),
                 higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32", data="synthetic",
                 config=dict(workload="N=1M Gaussians in [-3,3]^3, 1920x1080, SH deg 3 + depth (C=4), fwd+bwd L2",
-                            gaussians=n, width=W, height=H, channels=4, intersections=K, parallelism="single GPU"),
-                roofline=roofline, cpu_baseline=cpu,
-                stages_ms={k: round(v["avg_ms"], 4) for k, v in stages.items()})
+                            gaussians=args.n, width=W, height=H, channels=C, intersections=K,
+                            cell_entries=w["count"], blend=dict(cell=cfg_used.cell, ppl_fwd=cfg_used.ppl_fwd,
+                                                                ppl_bwd=cfg_used.ppl_bwd),
+                            parallelism="single GPU"),
+                roofline=roof, roofline_hbm=roof_hbm, cpu_baseline=cpu,
+                kernels_ms={k: round(v["avg_ms"], 4) for k, v in kern.items()})
     print(json.dumps(line), flush=True)
 
 

@@ -126,6 +126,16 @@ int msplat_compute_sh_backward(const float *shs, const float *dirs,
                                int64_t n, int C, int D, float *dL_dshs,
                                float *dL_ddirs, void *stream);
 
+/* ---- kernel timing (no reference counterpart; for benchmarks).  While
+ * enabled, the main kernel of each entry point is bracketed by HIP events on
+ * the caller's stream.  Kernel names: preprocess_fwd, preprocess_bwd, sh_fwd,
+ * sh_bwd, depth_sort, emit, class_sort_pass, blend_fwd, blend_bwd,
+ * rasterize_fwd, rasterize_bwd.  msplat_timing_enable returns the previous
+ * state; msplat_timing_read waits for the recorded events of one kernel,
+ * returns their total milliseconds and launch count, and forgets them. */
+int msplat_timing_enable(int on);
+int msplat_timing_read(const char *kernel, double *total_ms, int64_t *launches);
+
 /* ---- tile sort.  Replaces msplat/sort_gaussian.py:42-52 (cumsum, keygen
  * sort_gaussian.cu:74-113, torch.sort, gather, tile range
  * sort_gaussian.cu:115-142) with two calls around one 8-byte device read.
@@ -186,6 +196,68 @@ int msplat_alpha_blending_backward(
     const float *feature, int64_t n, int C, const int32_t *idx_sorted,
     const int32_t *tile_range, float bg, int W, int H, const float *final_T,
     const int32_t *ncontrib, const float *dL_drendered, float *dL_duv,
+    float *dL_dconic, float *dL_dopacity, float *dL_dfeature, void *stream);
+
+/* ---- rasterizer over cell lists: replaces the sort_gaussian + alpha_blending
+ * pair inside rasterization (msplat/__init__.py:83-91) with images, final_T,
+ * ncontrib and gradients identical to that pair's.
+ *
+ * Binning groups the tile grid into cells of `cell` x `cell` tiles and lists,
+ * per cell, every Gaussian whose tile rectangle (utils.h:17-37) touches it,
+ * in the reference's (depth bits, id) order; negative depths are left out, as
+ * the reference's keys put them in no tile.  A tile's reference list is its
+ * cell's list filtered by rectangle, so the K (tile, Gaussian) pairs are never
+ * materialised.  Two calls around one 8-byte device read, like the sort.
+ *
+ * Step 1 writes each Gaussian's packed tile rectangle to rect [n, 2] uint32
+ * (x0 | y0 << 16, x1 | y1 << 16; 8-byte aligned) and the number of cell
+ * entries to *d_count. */
+size_t msplat_bin_prepare_workspace_bytes(int64_t n, int W, int H, int cell);
+int msplat_bin_prepare(const float *uv, const float *depth,
+                       const int32_t *radius, int64_t n, int W, int H,
+                       int cell, uint32_t *rect, void *workspace,
+                       size_t workspace_bytes, int64_t *d_count, void *stream);
+
+/* Step 2 writes cell_ids [count] and cell_range [cells, 2] (cells =
+ * ceil(ceil(W/16)/cell) * ceil(ceil(H/16)/cell), row-major).  With d_count
+ * NULL, `count` is the step-1 count read back by the host.  With d_count =
+ * step 1's device counter, `count` is a capacity: grids are sized for it and
+ * the kernels read the true count on the device (no host synchronisation, so
+ * the two steps can be captured in a graph); if the true count exceeds the
+ * capacity, entries past it are dropped and *d_count still holds the true
+ * count, for the caller to detect. */
+size_t msplat_bin_emit_workspace_bytes(int64_t n, int64_t count, int W, int H,
+                                       int cell);
+int msplat_bin_emit(const float *uv, const float *depth, const int32_t *radius,
+                    int64_t n, int W, int H, int cell, const void *prepared,
+                    int64_t count, const int64_t *d_count, void *workspace,
+                    size_t workspace_bytes, int32_t *cell_ids,
+                    int32_t *cell_range, void *stream);
+
+/* Blending forward over cell lists (alpha_blending.cu:248-394 semantics).
+ * `ppl` = pixels per lane (1, 2 or 4; 0 picks the default).  `cull` != 0
+ * also skips, per tile, Gaussians that provably give every pixel of the tile
+ * alpha < 1/255 (rendered and final_T are unchanged; ncontrib then counts
+ * positions in the culled list).  walk [T, 2] int32 receives, per tile,
+ * where the adjoint's walk starts.  The adjoint must use the same cell and
+ * cull. */
+int msplat_rasterize_forward(const float *uv, const float *conic,
+                             const float *opacity, const float *feature,
+                             int64_t n, int C, const uint32_t *rect,
+                             const int32_t *cell_ids,
+                             const int32_t *cell_range, int cell, float bg,
+                             int W, int H, int ppl, int cull, float *rendered,
+                             float *final_T, int32_t *ncontrib, int32_t *walk,
+                             void *stream);
+
+/* Adjoint over cell lists (alpha_blending.cu:396-573 semantics); gradient
+ * outputs are overwritten. */
+int msplat_rasterize_backward(
+    const float *uv, const float *conic, const float *opacity,
+    const float *feature, int64_t n, int C, const uint32_t *rect,
+    const int32_t *cell_ids, const int32_t *cell_range, int cell, float bg,
+    int W, int H, int ppl, int cull, const float *final_T, const int32_t *ncontrib,
+    const int32_t *walk, const float *dL_drendered, float *dL_duv,
     float *dL_dconic, float *dL_dopacity, float *dL_dfeature, void *stream);
 
 #ifdef __cplusplus

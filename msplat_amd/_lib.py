@@ -47,6 +47,15 @@ SIGNATURES = {
     "msplat_alpha_blending_forward": (I32, [P, P, P, P, I64, I32, P, P, F32, I32, I32, P, P, P, P]),
     "msplat_alpha_blending_backward": (
         I32, [P, P, P, P, I64, I32, P, P, F32, I32, I32, P, P, P, P, P, P, P, P]),
+    "msplat_timing_enable": (I32, [I32]),
+    "msplat_timing_read": (I32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I64)]),
+    "msplat_bin_prepare_workspace_bytes": (SZ, [I64, I32, I32, I32]),
+    "msplat_bin_prepare": (I32, [P, P, P, I64, I32, I32, I32, P, P, SZ, P, P]),
+    "msplat_bin_emit_workspace_bytes": (SZ, [I64, I64, I32, I32, I32]),
+    "msplat_bin_emit": (I32, [P, P, P, I64, I32, I32, I32, P, I64, P, P, SZ, P, P, P]),
+    "msplat_rasterize_forward": (I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P]),
+    "msplat_rasterize_backward": (
+        I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P, P, P, P, P]),
 }
 
 _lib = None
@@ -87,6 +96,14 @@ def call(name: str, *args) -> int:
 
 def query(name: str, *args):
     return getattr(load(), name)(*args)
+
+
+def timing_read(kernel: str) -> tuple[float, int]:
+    """(total milliseconds, launches) the library recorded for `kernel` since the last read."""
+    total = ctypes.c_double(0.0)
+    launches = I64(0)
+    call("msplat_timing_read", kernel.encode(), ctypes.byref(total), ctypes.byref(launches))
+    return total.value, launches.value
 
 
 def stream_ptr(device: torch.device | None = None) -> int:

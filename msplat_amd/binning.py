@@ -1,0 +1,108 @@
+"""Cell binning and blending over cell lists: the rasterizer's replacement for
+the sort_gaussian + alpha_blending pair inside msplat.rasterization
+(msplat/__init__.py:83-91).
+
+The reference materialises every (tile, Gaussian) pair of the tile
+rectangles, sorts them by ``tile << 32 | depth bits`` and blends each tile's
+list front to back.  Most of those pairs lie behind pixels that saturate
+early and are never read.  Here Gaussians are binned into cells of
+``cell x cell`` tiles, each cell's list in (depth bits, id) order, and a tile
+walks its cell's list keeping the Gaussians whose tile rectangle covers it.
+That filtered sequence is exactly the tile's reference list, so rendered
+images, final_T and ncontrib are identical to sort_gaussian + alpha_blending
+(and gradients equal up to float summation order).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import torch
+
+from . import _lib
+from .sort_gaussian import tile_grid
+
+
+@dataclasses.dataclass
+class CellLists:
+    rect: torch.Tensor        # [n, 2] int32: packed tile rectangle per Gaussian
+    cell_ids: torch.Tensor    # [count] int32: Gaussian ids, cell-major, (depth, id) order within a cell
+    cell_range: torch.Tensor  # [cells, 2] int32: [start, end) per cell
+    cell: int                 # cell edge in tiles
+
+
+def cell_grid(W: int, H: int, cell: int) -> tuple[int, int]:
+    gx, gy = tile_grid(W, H)
+    return (gx + cell - 1) // cell, (gy + cell - 1) // cell
+
+
+def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8) -> CellLists:
+    u = _lib.f32("uv", uv)
+    d = _lib.f32("depth", depth).view(-1)
+    r = _lib.i32("radius", radius).view(-1)
+    n = u.shape[0]
+    dev = u.device
+    W, H, cell = int(W), int(H), int(cell)
+    ncx, ncy = cell_grid(W, H, cell)
+    with torch.cuda.device(dev):
+        stream = _lib.stream_ptr()
+        rect = torch.empty((n, 2), dtype=torch.int32, device=dev)
+        prep_bytes = int(_lib.query("msplat_bin_prepare_workspace_bytes", n, W, H, cell))
+        prep = torch.empty(max(prep_bytes, 1), dtype=torch.uint8, device=dev)
+        count = torch.empty(1, dtype=torch.int64, device=dev)
+        _lib.call("msplat_bin_prepare", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, W, H, cell, _lib.ptr(rect),
+                  _lib.ptr(prep), prep_bytes, _lib.ptr(count), stream)
+        k = int(count.item())  # the only host synchronisation of the binning
+        cell_ids = torch.empty(k, dtype=torch.int32, device=dev)
+        cell_range = torch.empty((ncx * ncy, 2), dtype=torch.int32, device=dev)
+        emit_bytes = int(_lib.query("msplat_bin_emit_workspace_bytes", n, k, W, H, cell))
+        ws = torch.empty(max(emit_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.call("msplat_bin_emit", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, W, H, cell, _lib.ptr(prep), k, None,
+                  _lib.ptr(ws), emit_bytes, _lib.ptr(cell_ids), _lib.ptr(cell_range), stream)
+    return CellLists(rect, cell_ids, cell_range, cell)
+
+
+def rasterize_forward(uv, conic, opacity, feature, bins: CellLists, bg: float, W: int, H: int, ppl: int = 0,
+                      cull: bool = True):
+    """``(rendered [C,H,W], final_T [H,W], ncontrib [H,W] int32, walk [T,2] int32)``.
+
+    ``cull`` skips, per tile, Gaussians that give no pixel of it alpha >= 1/255
+    (rendered and final_T are unchanged; ncontrib counts culled-list
+    positions).  The adjoint must be called with the same ``cull``."""
+    u = _lib.f32("uv", uv)
+    f = _lib.f32("feature", feature)
+    n, C = f.shape
+    dev = u.device
+    gx, gy = tile_grid(int(W), int(H))
+    out = torch.empty((C, H, W), dtype=torch.float32, device=dev)
+    final_T = torch.empty((H, W), dtype=torch.float32, device=dev)
+    ncontrib = torch.empty((H, W), dtype=torch.int32, device=dev)
+    walk = torch.empty((gx * gy, 2), dtype=torch.int32, device=dev)
+    with torch.cuda.device(dev):
+        _lib.call("msplat_rasterize_forward", _lib.ptr(u), _lib.ptr(_lib.f32("conic", conic)),
+                  _lib.ptr(_lib.f32("opacity", opacity)), _lib.ptr(f), n, C, _lib.ptr(bins.rect),
+                  _lib.ptr(bins.cell_ids), _lib.ptr(bins.cell_range), bins.cell, float(bg), int(W), int(H),
+                  int(ppl), int(bool(cull)), _lib.ptr(out), _lib.ptr(final_T), _lib.ptr(ncontrib), _lib.ptr(walk),
+                  _lib.stream_ptr())
+    return out, final_T, ncontrib, walk
+
+
+def rasterize_backward(uv, conic, opacity, feature, bins: CellLists, bg: float, W: int, H: int, final_T, ncontrib,
+                       walk, dL_drendered, ppl: int = 0, cull: bool = True):
+    """``(dL_duv [P,2], dL_dconic [P,3], dL_dopacity [P,1], dL_dfeature [P,C])``."""
+    u = _lib.f32("uv", uv)
+    f = _lib.f32("feature", feature)
+    n, C = f.shape
+    dev = u.device
+    duv = torch.empty((n, 2), dtype=torch.float32, device=dev)
+    dconic = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    dopac = torch.empty((n, 1), dtype=torch.float32, device=dev)
+    dfeat = torch.empty((n, C), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        _lib.call("msplat_rasterize_backward", _lib.ptr(u), _lib.ptr(_lib.f32("conic", conic)),
+                  _lib.ptr(_lib.f32("opacity", opacity)), _lib.ptr(f), n, C, _lib.ptr(bins.rect),
+                  _lib.ptr(bins.cell_ids), _lib.ptr(bins.cell_range), bins.cell, float(bg), int(W), int(H),
+                  int(ppl), int(bool(cull)), _lib.ptr(final_T), _lib.ptr(ncontrib), _lib.ptr(walk),
+                  _lib.ptr(_lib.f32("dL_drendered", dL_drendered)), _lib.ptr(duv), _lib.ptr(dconic),
+                  _lib.ptr(dopac), _lib.ptr(dfeat), _lib.stream_ptr())
+    return duv, dconic, dopac, dfeat

@@ -1,18 +1,29 @@
-// Front-to-back alpha blending over the sorted per-tile Gaussian lists,
-// forward and adjoint (msplat/src/alpha_blending.cu:16-246).
+// Front-to-back alpha blending, forward and adjoint
+// (msplat/src/alpha_blending.cu:16-246), over two kinds of Gaussian lists.
 //
-// Work decomposition (gfx950): a 16x16 tile is split into four 8x8
-// quadrants, each handled by a one-wavefront workgroup (64 lanes = 64
-// pixels).  A wave stages 64 Gaussians of the tile list at a time in LDS
-// (position, conic, opacity and all C feature channels) and every lane walks
-// them; the wave stops as soon as all 64 of its pixels are saturated (a
-// ballot), independent of the other quadrants, and needs no workgroup-wide
-// barriers.  The four quadrants of one tile read the same list, so their
-// workgroup ids are chosen congruent modulo 8: workgroups are dispatched
-// round-robin over the 8 XCDs, which puts the four on one XCD and its L2.
+//  * Tile lists (kWalk = false): idx_sorted / tile_range as sort_gaussian
+//    returns them, the reference's format (alpha_blending API).
+//  * Cell lists (kWalk = true): every visible Gaussian binned into cells of
+//    S x S tiles, each cell's list in (depth bits, id) order.  A tile walks its
+//    cell's list and keeps the Gaussians whose tile rectangle covers it: that
+//    is exactly its reference tile list, in the same order, so images,
+//    final_T and ncontrib are bit-identical -- but the K tile x Gaussian
+//    pairs (most of which lie behind saturated pixels and are never read) are
+//    neither written nor sorted.
 //
-// All C channels are blended in one walk (the reference re-walks the list
-// for every 32-channel chunk).
+// Work decomposition (gfx950): a one-wavefront workgroup covers 64 * PPL
+// pixels of a 16x16 tile: PPL = 1 an 8x8 quadrant, 2 a 16x8 half, 4 the whole
+// tile with 4 pixels per lane (rows r, r + 4, r + 8, r + 12).  The wave stages
+// up to 64 Gaussians at a time in LDS as 16-byte records and every lane walks
+// them for its PPL pixels; it stops when all its pixels are saturated (a
+// ballot) and needs no workgroup barrier.  More pixels per lane amortise the
+// staging reads and, in the adjoint, divide the per-Gaussian wave reductions
+// and global atomics by PPL.  Workgroups are dispatched round-robin over the
+// 8 XCDs; wave w = (b % 8) * per_xcd + b / 8 gives each XCD a contiguous band
+// of tiles, so neighbouring tiles (which share most Gaussians) share an L2.
+//
+// All C channels of a pass are blended in one walk (the reference re-walks
+// the list for every 32-channel chunk).
 #pragma once
 
 #include "common.hpp"
@@ -21,28 +32,57 @@ namespace msplat {
 
 constexpr int kBlendLanes = 64;
 
-struct QuadPix {
-  int tile, px, py;
-  bool active;  // workgroup maps to a real tile
+// Largest pixels-per-lane a channel bucket is instantiated for (the adjoint
+// keeps 3 * CMAX + 5 registers per pixel).
+MSPLAT_HD constexpr int blend_max_ppl(int cmax) { return cmax <= 8 ? 4 : (cmax <= 16 ? 2 : 1); }
+
+// Everything a blending kernel reads.
+struct BlendIn {
+  const float *uv, *conic, *opacity, *feature;
+  int C, c0;               // channels, first channel of this pass
+  const int32_t *list;     // idx_sorted, or cell lists
+  const int32_t *range;    // tile_range [T, 2], or cell_range [cells, 2]
+  const uint32_t *rect;    // cell lists: tile rectangle per Gaussian (x0 | y0 << 16, x1 | y1 << 16)
+  int cell, ncx;           // cell lists: cell edge in tiles, cells per row
+  int cull;                // cell lists: skip Gaussians that reach no pixel of the tile (may_touch)
+  float bg;
+  int W, H, gx, ntiles, per_xcd;
 };
 
-// blockIdx.x -> (tile, quadrant): xcd = b % 8 selects the tile within a group
-// of 8 tiles, (b / 8) % 4 the quadrant, b / 32 the group.
-MSPLAT_DEV QuadPix quad_pixel(int gx, int ntiles) {
+template <int PPL>
+struct PixBlock {
+  int tile, tx, ty;
+  bool active;
+  int px[PPL], py[PPL];
+};
+
+template <int PPL>
+MSPLAT_DEV PixBlock<PPL> pix_block(const BlendIn &in) {
+  constexpr int R = 4 / PPL;  // waves per tile
   const int b = blockIdx.x;
-  const int tile = (b >> 5) * 8 + (b & 7);
-  const int quad = (b >> 3) & 3;
-  QuadPix q;
-  q.tile = tile;
-  q.active = tile < ntiles;
-  const int tx = q.active ? tile % gx : 0, ty = q.active ? tile / gx : 0;
+  const int w = (b & 7) * in.per_xcd + (b >> 3);
+  PixBlock<PPL> p;
+  p.tile = w / R;
+  const int sub = w % R;
+  p.active = p.tile < in.ntiles;
+  p.tx = p.active ? p.tile % in.gx : 0;
+  p.ty = p.active ? p.tile / in.gx : 0;
   const int lane = threadIdx.x;
-  q.px = tx * kTile + (quad & 1) * 8 + (lane & 7);
-  q.py = ty * kTile + (quad >> 1) * 8 + (lane >> 3);
-  return q;
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    if constexpr (PPL == 1) {
+      p.px[i] = p.tx * kTile + (sub & 1) * 8 + (lane & 7);
+      p.py[i] = p.ty * kTile + (sub >> 1) * 8 + (lane >> 3);
+    } else {
+      p.px[i] = p.tx * kTile + (lane & 15);
+      p.py[i] = p.ty * kTile + sub * (kTile / R) + (lane >> 4) + 4 * i;
+    }
+  }
+  return p;
 }
 
-inline unsigned blend_grid(int ntiles) { return unsigned(((ntiles + 7) / 8) * 32); }
+// Workgroups per launch for a given pixels-per-lane (8 XCD bands of per_xcd).
+inline int blend_per_xcd(int ntiles, int ppl) { return (ntiles * (4 / ppl) + 7) / 8; }
 
 // exp(x) as 2^(x log2 e) on the hardware exp2 (v_exp_f32).
 MSPLAT_DEV float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
@@ -53,26 +93,27 @@ template <int CMAX>
 struct BlendStage {
   static constexpr int kQuads = (6 + CMAX + 3) / 4;  // float4s per record
   float4 rec[kBlendLanes * kQuads];
+  int32_t id[kBlendLanes];   // Gaussian of each slot
+  int32_t pos[kBlendLanes];  // cell lists: list index of each slot
 };
 
 // Stage Gaussian g into LDS slot `slot`: channels [c0, c0 + cc) of its row.
 template <int CMAX>
-MSPLAT_DEV void stage_gaussian(BlendStage<CMAX> &s, int slot, int g, const float *__restrict__ uv,
-                               const float *__restrict__ conic, const float *__restrict__ opacity,
-                               const float *__restrict__ feature, int C, int c0, int cc) {
+MSPLAT_DEV void stage_gaussian(BlendStage<CMAX> &s, int slot, int g, const BlendIn &in, int cc) {
   constexpr int NQ = BlendStage<CMAX>::kQuads;
   float r[NQ * 4];
-  r[0] = uv[2 * int64_t(g)];
-  r[1] = uv[2 * int64_t(g) + 1];
-  r[2] = conic[3 * int64_t(g)];
-  r[3] = conic[3 * int64_t(g) + 1];
-  r[4] = conic[3 * int64_t(g) + 2];
-  r[5] = opacity[g];
-  const float *fr = feature + int64_t(g) * C + c0;
+  r[0] = in.uv[2 * int64_t(g)];
+  r[1] = in.uv[2 * int64_t(g) + 1];
+  r[2] = in.conic[3 * int64_t(g)];
+  r[3] = in.conic[3 * int64_t(g) + 1];
+  r[4] = in.conic[3 * int64_t(g) + 2];
+  r[5] = in.opacity[g];
+  const float *fr = in.feature + int64_t(g) * in.C + in.c0;
 #pragma unroll
   for (int i = 6; i < NQ * 4; ++i) r[i] = (i - 6 < cc) ? fr[i - 6] : 0.f;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) s.rec[slot * NQ + q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+  s.id[slot] = g;
 }
 
 // Geometry of record j: (u, v, a, b) and (c, o, f0, f1).
@@ -100,184 +141,308 @@ MSPLAT_DEV void stage_features(const BlendStage<CMAX> &s, int j, const float4 &g
   }
 }
 
+// Lanes below this one in a ballot mask.
+MSPLAT_DEV int mask_rank(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+// Exact tile culling for cell lists: false only if no pixel centre of tile
+// (tx, ty) can reach alpha >= 1/255.  The Gaussian's largest power over the
+// tile's pixel-centre rectangle -- q = a dx^2 + 2 b dx dy + c dy^2 is convex,
+// so off-centre its minimum lies on an edge of the box -- must stay below
+// -ln(255 o) by a margin (0.05 in power, alpha x 0.95) far above the fp32
+// rounding of the blending's own power and exp2; degenerate or non-finite
+// parameters are never culled.  Culled Gaussians change neither colour nor
+// transmittance, so images, final_T and gradients are unchanged; only the
+// list positions counted in ncontrib differ.
+MSPLAT_DEV bool may_touch(const BlendIn &in, int g, int tx, int ty) {
+  const float u = in.uv[2 * int64_t(g)], v = in.uv[2 * int64_t(g) + 1];
+  const float a = in.conic[3 * int64_t(g)], b = in.conic[3 * int64_t(g) + 1], c = in.conic[3 * int64_t(g) + 2];
+  const float o = in.opacity[g];
+  if (!(a > 0.f && c > 0.f && a * c - b * b > 0.f && o > 0.f && o < 1e30f && fabsf(u) < 1e30f &&
+        fabsf(v) < 1e30f))
+    return true;
+  // d = mean - pixel over the pixel centres p in [16 t, 16 t + 15]
+  const float x0 = u - float(kTile * tx + kTile - 1), x1 = u - float(kTile * tx);
+  const float y0 = v - float(kTile * ty + kTile - 1), y1 = v - float(kTile * ty);
+  if (x0 <= 0.f && x1 >= 0.f && y0 <= 0.f && y1 >= 0.f) return true;
+  auto q = [&](float dx, float dy) { return a * dx * dx + 2.f * b * dx * dy + c * dy * dy; };
+  auto on_x = [&](float dx) { return q(dx, fminf(fmaxf(-b * dx / c, y0), y1)); };
+  auto on_y = [&](float dy) { return q(fminf(fmaxf(-b * dy / a, x0), x1), dy); };
+  const float qmin = fminf(fminf(on_x(x0), on_x(x1)), fminf(on_y(y0), on_y(y1)));
+  // alpha < 1/255  <=  power = -q / 2 < -ln(255 o) - 0.05
+  return !(qmin > 2.f * logf(255.f * o) + 0.1f);
+}
+
+// Cell-list filter: the Gaussian's tile rectangle covers the tile (and, with
+// culling, it may touch one of the tile's pixels).
+MSPLAT_DEV bool walk_keeps(const BlendIn &in, int g, int tx, int ty) {
+  if (!rect_covers(reinterpret_cast<const uint2 *>(in.rect)[g], tx, ty)) return false;
+  return !in.cull || may_touch(in, g, tx, ty);
+}
+
+// The list a tile walks: [start, end) of its tile list or of its cell's list.
+template <bool kWalk>
+MSPLAT_DEV int2 tile_list(const BlendIn &in, int tile, int tx, int ty) {
+  const int c = kWalk ? (ty / in.cell) * in.ncx + tx / in.cell : tile;
+  return make_int2(in.range[2 * c], in.range[2 * c + 1]);
+}
+
 // power = -0.5 (a dx^2 + c dy^2) - b dx dy with d = mean - pixel
 // (alpha_blending.cu:74-82).
 MSPLAT_DEV float gauss_power(float a, float b, float c, float dx, float dy) {
   return -0.5f * ((a * dx) * dx + (c * dy) * dy) - (b * dx) * dy;
 }
 
-template <int CMAX>
-__global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(
-    const float *__restrict__ uv, const float *__restrict__ conic, const float *__restrict__ opacity,
-    const float *__restrict__ feature, int C, int c0, const int32_t *__restrict__ idx_sorted,
-    const int32_t *__restrict__ tile_range, float bg, int W, int H, int gx, int ntiles,
-    float *__restrict__ out, float *__restrict__ final_T, int32_t *__restrict__ ncontrib) {
+// Forward.  Outputs rendered [C, H, W] (+ bg * T), final_T and ncontrib
+// [H, W] (the 1-based list position of each pixel's last contributor).  Cell
+// lists also record, per tile, the end of the walk that the adjoint starts
+// from: walk[2 t] = list index after the last contributor of any pixel of the
+// tile, walk[2 t + 1] = that contributor's position in the tile's list
+// (atomic maxima; the caller zero-fills walk).
+template <int CMAX, int PPL, bool kWalk>
+__global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, float *__restrict__ out,
+                                                                float *__restrict__ final_T,
+                                                                int32_t *__restrict__ ncontrib,
+                                                                int32_t *__restrict__ walk) {
   __shared__ BlendStage<CMAX> s;
-  const QuadPix q = quad_pixel(gx, ntiles);
-  if (!q.active) return;
+  const PixBlock<PPL> pb = pix_block<PPL>(in);
+  if (!pb.active) return;
   const int lane = threadIdx.x;
-  const bool inside = q.px < W && q.py < H;
-  const float pxf = float(q.px), pyf = float(q.py);
-  const int start = tile_range[2 * q.tile], end = tile_range[2 * q.tile + 1];
-  const int cc = C - c0 < CMAX ? C - c0 : CMAX;  // channels of this pass
+  const int cc = in.C - in.c0 < CMAX ? in.C - in.c0 : CMAX;  // channels of this pass
+  const int2 lst = tile_list<kWalk>(in, pb.tile, pb.tx, pb.ty);
+  int pos = lst.x;
+  const int end = lst.y;
 
-  bool done = !inside;
-  float T = 1.f;
-  int contributor = 0, last = 0;
-  float F[CMAX];
+  bool done[PPL];
+  float T[PPL];
+  int last[PPL], last_e[PPL];
+  float F[PPL][CMAX];
 #pragma unroll
-  for (int ch = 0; ch < CMAX; ++ch) F[ch] = 0.f;
+  for (int i = 0; i < PPL; ++i) {
+    done[i] = !(pb.px[i] < in.W && pb.py[i] < in.H);
+    T[i] = 1.f;
+    last[i] = 0;
+    last_e[i] = 0;
+#pragma unroll
+    for (int ch = 0; ch < CMAX; ++ch) F[i][ch] = 0.f;
+  }
 
-  for (int base = start; base < end; base += kBlendLanes) {
-    if (__ballot(!done) == 0ull) break;
-    const int cnt = end - base < kBlendLanes ? end - base : kBlendLanes;
+  int fpos = 0;  // position in the tile list of the first staged Gaussian
+  while (pos < end) {
+    bool alive = false;
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) alive |= !done[i];
+    if (__ballot(alive) == 0ull) break;
+    int cnt;
     __syncthreads();
-    if (lane < cnt) stage_gaussian<CMAX>(s, lane, idx_sorted[base + lane], uv, conic, opacity, feature, C, c0, cc);
+    if constexpr (kWalk) {
+      const int e = pos + lane;
+      int g = 0;
+      bool pass = false;
+      if (e < end) {
+        g = in.list[e];
+        pass = walk_keeps(in, g, pb.tx, pb.ty);
+      }
+      const uint64_t m = __ballot(pass);
+      cnt = __popcll(m);
+      if (pass) {
+        const int slot = mask_rank(m);
+        stage_gaussian<CMAX>(s, slot, g, in, cc);
+        s.pos[slot] = e;
+      }
+      pos += kBlendLanes;
+    } else {
+      cnt = end - pos < kBlendLanes ? end - pos : kBlendLanes;
+      if (lane < cnt) stage_gaussian<CMAX>(s, lane, in.list[pos + lane], in, cc);
+      pos += cnt;
+    }
     __syncthreads();
     for (int j = 0; j < cnt; ++j) {
-      if (!done) {
-        ++contributor;
-        float4 g0, g1;
-        stage_geom(s, j, g0, g1);
-        const float dx = g0.x - pxf, dy = g0.y - pyf;
-        const float power = gauss_power(g0.z, g0.w, g1.x, dx, dy);
-        if (!(power > 0.f)) {  // a NaN power is not skipped, as in the reference
-          const float alpha = fminf(0.99f, g1.y * fast_exp(power));
-          if (!(alpha < 1.f / 255.f)) {
-            const float nT = T * (1.f - alpha);
-            if (nT < 0.0001f) {
-              done = true;
-            } else {
-              float f[CMAX];
-              stage_features(s, j, g1, f);
+      float4 g0, g1;
+      stage_geom(s, j, g0, g1);
+      float f[CMAX];
+      stage_features(s, j, g1, f);
+      const int contributor = fpos + j + 1;
 #pragma unroll
-              for (int ch = 0; ch < CMAX; ++ch)
-                if (ch < cc) F[ch] += f[ch] * alpha * T;
-              T = nT;
-              last = contributor;
-            }
-          }
+      for (int i = 0; i < PPL; ++i) {
+        if (done[i]) continue;
+        const float dx = g0.x - float(pb.px[i]), dy = g0.y - float(pb.py[i]);
+        const float power = gauss_power(g0.z, g0.w, g1.x, dx, dy);
+        if (power > 0.f) continue;  // a NaN power is not skipped, as in the reference
+        const float alpha = fminf(0.99f, g1.y * fast_exp(power));
+        if (alpha < 1.f / 255.f) continue;
+        const float nT = T[i] * (1.f - alpha);
+        if (nT < 0.0001f) {
+          done[i] = true;
+          continue;
         }
+#pragma unroll
+        for (int ch = 0; ch < CMAX; ++ch)
+          if (ch < cc) F[i][ch] += f[ch] * alpha * T[i];
+        T[i] = nT;
+        last[i] = contributor;
+        if constexpr (kWalk) last_e[i] = s.pos[j] + 1;
       }
     }
+    fpos += cnt;
   }
-  if (inside) {
-    const int64_t pix = int64_t(q.py) * W + q.px;
-    const int64_t hw = int64_t(H) * W;
-    if (c0 == 0) {
-      final_T[pix] = T;
-      ncontrib[pix] = last;
+  const int64_t hw = int64_t(in.H) * in.W;
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    if (!(pb.px[i] < in.W && pb.py[i] < in.H)) continue;
+    const int64_t pix = int64_t(pb.py[i]) * in.W + pb.px[i];
+    if (in.c0 == 0) {
+      final_T[pix] = T[i];
+      ncontrib[pix] = last[i];
     }
 #pragma unroll
     for (int ch = 0; ch < CMAX; ++ch)
-      if (ch < cc) out[(c0 + ch) * hw + pix] = F[ch] + T * bg;
+      if (ch < cc) out[(in.c0 + ch) * hw + pix] = F[i][ch] + T[i] * in.bg;
+  }
+  if constexpr (kWalk) {
+    int wl = 0, we = 0;
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+      wl = imax(wl, last[i]);
+      we = imax(we, last_e[i]);
+    }
+    wl = wave_max(wl);
+    we = wave_max(we);
+    if (lane == 0 && in.c0 == 0 && wl > 0) {
+      atomicMax(walk + 2 * pb.tile, we);
+      atomicMax(walk + 2 * pb.tile + 1, wl);
+    }
   }
 }
 
 // Adjoint: walk each pixel's contributors back to front, recovering T by
-// division (alpha_blending.cu:155-245); per Gaussian, the wave sums its 64
-// lanes' contributions and issues one atomic per gradient value.
-template <int CMAX>
+// division (alpha_blending.cu:155-245).  Per Gaussian, every lane first sums
+// its PPL pixels' contributions, then the wave sums its 64 lanes and issues
+// one atomic per gradient value.
+template <int CMAX, int PPL, bool kWalk>
 __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
-    const float *__restrict__ uv, const float *__restrict__ conic, const float *__restrict__ opacity,
-    const float *__restrict__ feature, int C, int c0, const int32_t *__restrict__ idx_sorted,
-    const int32_t *__restrict__ tile_range, float bg, int W, int H, int gx, int ntiles,
-    const float *__restrict__ final_T, const int32_t *__restrict__ ncontrib, const float *__restrict__ dout,
-    float *__restrict__ duv, float *__restrict__ dconic, float *__restrict__ dopac, float *__restrict__ dfeat) {
+    BlendIn in, const float *__restrict__ final_T, const int32_t *__restrict__ ncontrib,
+    const int32_t *__restrict__ walk, const float *__restrict__ dout, float *__restrict__ duv,
+    float *__restrict__ dconic, float *__restrict__ dopac, float *__restrict__ dfeat) {
   __shared__ BlendStage<CMAX> s;
-  const QuadPix q = quad_pixel(gx, ntiles);
-  if (!q.active) return;
+  const PixBlock<PPL> pb = pix_block<PPL>(in);
+  if (!pb.active) return;
   const int lane = threadIdx.x;
-  const bool inside = q.px < W && q.py < H;
-  const float pxf = float(q.px), pyf = float(q.py);
-  const int start = tile_range[2 * q.tile];
-  const int64_t pix = int64_t(q.py) * W + q.px;
-  const int64_t hw = int64_t(H) * W;
+  const int cc = in.C - in.c0 < CMAX ? in.C - in.c0 : CMAX;  // channels of this pass
+  const int64_t hw = int64_t(in.H) * in.W;
 
-  const int cc = C - c0 < CMAX ? C - c0 : CMAX;  // channels of this pass
-  const float T_final = inside ? final_T[pix] : 0.f;
-  const int last = inside ? ncontrib[pix] : 0;
-  float g[CMAX];
-  float bg_dot = 0.f;
+  float T[PPL], T_final[PPL], bg_dot[PPL], last_alpha[PPL];
+  int last[PPL];
+  float g[PPL][CMAX], accum[PPL][CMAX], last_f[PPL][CMAX];
+  int wl = 0;
 #pragma unroll
-  for (int ch = 0; ch < CMAX; ++ch) {
-    g[ch] = (inside && ch < cc) ? dout[(c0 + ch) * hw + pix] : 0.f;
-    if (ch < cc) bg_dot += bg * g[ch];
+  for (int i = 0; i < PPL; ++i) {
+    const bool inside = pb.px[i] < in.W && pb.py[i] < in.H;
+    const int64_t pix = int64_t(pb.py[i]) * in.W + pb.px[i];
+    T_final[i] = inside ? final_T[pix] : 0.f;
+    last[i] = inside ? ncontrib[pix] : 0;
+    wl = imax(wl, last[i]);
+    bg_dot[i] = 0.f;
+#pragma unroll
+    for (int ch = 0; ch < CMAX; ++ch) {
+      g[i][ch] = (inside && ch < cc) ? dout[(in.c0 + ch) * hw + pix] : 0.f;
+      if (ch < cc) bg_dot[i] += in.bg * g[i][ch];
+      accum[i][ch] = last_f[i][ch] = 0.f;
+    }
+    T[i] = T_final[i];
+    last_alpha[i] = 0.f;
   }
-  const int wave_last = wave_max(last);
+  wl = wave_max(wl);
 
-  float T = T_final;
-  float accum[CMAX], last_f[CMAX];
-#pragma unroll
-  for (int ch = 0; ch < CMAX; ++ch) accum[ch] = last_f[ch] = 0.f;
-  float last_alpha = 0.f;
+  const int2 lst = tile_list<kWalk>(in, pb.tile, pb.tx, pb.ty);
+  const int lo_bound = lst.x;
+  int hi, fcount;  // walk backwards over [lo_bound, hi); fcount Gaussians of the tile list lie below hi
+  if constexpr (kWalk) {
+    hi = imin(walk[2 * pb.tile], lst.y);
+    fcount = walk[2 * pb.tile + 1];
+  } else {
+    hi = imin(lst.x + wl, lst.y);
+    fcount = hi - lst.x;
+  }
 
-  for (int hi = start + wave_last; hi > start; hi -= kBlendLanes) {
-    const int lo = hi - kBlendLanes > start ? hi - kBlendLanes : start;
-    const int cnt = hi - lo;
+  while (fcount > 0 && hi > lo_bound) {
+    const int lo = hi - kBlendLanes > lo_bound ? hi - kBlendLanes : lo_bound;
+    int cnt;
     __syncthreads();
-    if (lane < cnt) stage_gaussian<CMAX>(s, lane, idx_sorted[lo + lane], uv, conic, opacity, feature, C, c0, cc);
+    if constexpr (kWalk) {
+      const int e = lo + lane;
+      int gi = 0;
+      bool pass = false;
+      if (e < hi) {
+        gi = in.list[e];
+        pass = walk_keeps(in, gi, pb.tx, pb.ty);
+      }
+      const uint64_t m = __ballot(pass);
+      cnt = __popcll(m);
+      if (pass) stage_gaussian<CMAX>(s, mask_rank(m), gi, in, cc);
+    } else {
+      cnt = hi - lo;
+      if (lane < cnt) stage_gaussian<CMAX>(s, lane, in.list[lo + lane], in, cc);
+    }
+    const int fbase = fcount - cnt;  // tile-list position of slot 0
     __syncthreads();
     for (int j = cnt - 1; j >= 0; --j) {
-      const int rel = lo + j - start;  // 0-based position in the tile list
-      bool contrib = rel < last;
+      const int rel = fbase + j;  // 0-based position in the tile list
+      if (rel >= wl) continue;    // uniform: no pixel of this wave reaches it
       float4 g0, g1;
       stage_geom(s, j, g0, g1);
-      const float dx = g0.x - pxf, dy = g0.y - pyf;
       const float a = g0.z, b = g0.w, c = g1.x, op = g1.y;
-      float G = 0.f, alpha = 0.f;
-      if (contrib) {
-        const float power = gauss_power(a, b, c, dx, dy);
-        if (power > 0.f) {
-          contrib = false;
-        } else {
-          G = fast_exp(power);
-          alpha = fminf(0.99f, op * G);
-          if (alpha < 1.f / 255.f) contrib = false;
-        }
-      }
-      const uint64_t any = __ballot(contrib);
-      if (any == 0ull) continue;
+      float fe[CMAX];
+      stage_features(s, j, g1, fe);
       float vuv0 = 0.f, vuv1 = 0.f, vc0 = 0.f, vc1 = 0.f, vc2 = 0.f, vo = 0.f;
       float vf[CMAX];
 #pragma unroll
       for (int ch = 0; ch < CMAX; ++ch) vf[ch] = 0.f;
-      if (contrib) {
+      bool any = false;
+#pragma unroll
+      for (int i = 0; i < PPL; ++i) {
+        if (!(rel < last[i])) continue;
+        const float dx = g0.x - float(pb.px[i]), dy = g0.y - float(pb.py[i]);
+        const float power = gauss_power(a, b, c, dx, dy);
+        if (power > 0.f) continue;
+        const float G = fast_exp(power);
+        const float alpha = fminf(0.99f, op * G);
+        if (alpha < 1.f / 255.f) continue;
+        any = true;
         // 1 / (1 - alpha) on the hardware reciprocal (the reference divides
         // with fast-math); 1 - alpha >= 0.01, so it is well conditioned.
         const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-        T = T * inv;
-        const float dcol = alpha * T;
+        T[i] = T[i] * inv;
+        const float dcol = alpha * T[i];
         float dL_dalpha = 0.f;
-        float fe[CMAX];
-        stage_features(s, j, g1, fe);
 #pragma unroll
         for (int ch = 0; ch < CMAX; ++ch) {
           if (ch < cc) {
-            const float f = fe[ch];
-            accum[ch] = last_alpha * last_f[ch] + (1.f - last_alpha) * accum[ch];
-            last_f[ch] = f;
-            dL_dalpha += (f - accum[ch]) * g[ch];
-            vf[ch] = dcol * g[ch];
+            accum[i][ch] = last_alpha[i] * last_f[i][ch] + (1.f - last_alpha[i]) * accum[i][ch];
+            last_f[i][ch] = fe[ch];
+            dL_dalpha += (fe[ch] - accum[i][ch]) * g[i][ch];
+            vf[ch] += dcol * g[i][ch];
           }
         }
-        dL_dalpha *= T;
-        last_alpha = alpha;
-        dL_dalpha += (-T_final * inv) * bg_dot;
+        dL_dalpha *= T[i];
+        last_alpha[i] = alpha;
+        dL_dalpha += (-T_final[i] * inv) * bg_dot[i];
         const float dL_dG = op * dL_dalpha;
         const float gdx = ((-G) * dx) * a - (G * dy) * b;
         const float gdy = ((-G) * dy) * c - (G * dx) * b;
-        vuv0 = dL_dG * gdx;
-        vuv1 = dL_dG * gdy;
-        vc0 = (((-0.5f * G) * dx) * dx) * dL_dG;
-        vc1 = (((-G) * dx) * dy) * dL_dG;
-        vc2 = (((-0.5f * G) * dy) * dy) * dL_dG;
-        vo = G * dL_dalpha;
+        vuv0 += dL_dG * gdx;
+        vuv1 += dL_dG * gdy;
+        vc0 += (((-0.5f * G) * dx) * dx) * dL_dG;
+        vc1 += (((-G) * dx) * dy) * dL_dG;
+        vc2 += (((-0.5f * G) * dy) * dy) * dL_dG;
+        vo += G * dL_dalpha;
       }
+      if (__ballot(any) == 0ull) continue;
       // Sum the 6 + cc per-lane contributions over the wave in one transposed
       // reduction; lane 16 r + k then owns value 4 k' + kRowValue[r] of
       // vector k' = k + 16 q in atomic round q and issues its atomic.
-      const int gidx = idx_sorted[lo + j];
+      const int gidx = s.id[j];
       constexpr int NV = ((6 + CMAX) + 3) / 4 * 4;
       float vals[NV];
       vals[0] = vuv0;
@@ -304,11 +469,13 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
           if (v < 2) dst = duv + 2 * int64_t(gidx) + v;
           else if (v < 5) dst = dconic + 3 * int64_t(gidx) + (v - 2);
           else if (v == 5) dst = dopac + gidx;
-          else dst = dfeat + int64_t(gidx) * C + c0 + (v - 6);
+          else dst = dfeat + int64_t(gidx) * in.C + in.c0 + (v - 6);
         }
         if (dst) unsafeAtomicAdd(dst, mine);
       }
     }
+    fcount = fbase;
+    hi = lo;
   }
 }
 

@@ -58,6 +58,45 @@ MSPLAT_HD TileRect tile_rect(float px, float py, int r, int gx, int gy) {
   return t;
 }
 
+// Tile rectangle packed into 8 bytes (x0 | y0 << 16, x1 | y1 << 16); the
+// tile grid is at most 4096 x 4096.
+MSPLAT_HD uint2 pack_rect(const TileRect &t) {
+  return make_uint2(uint32_t(t.x0) | (uint32_t(t.y0) << 16), uint32_t(t.x1) | (uint32_t(t.y1) << 16));
+}
+
+MSPLAT_DEV bool rect_covers(uint2 r, int tx, int ty) {
+  const int x0 = int(r.x & 0xffffu), y0 = int(r.x >> 16), x1 = int(r.y & 0xffffu), y1 = int(r.y >> 16);
+  return x0 <= tx && tx < x1 && y0 <= ty && ty < y1;
+}
+
+// ---- views of one launch ---------------------------------------------------
+// Several cameras (of different image sizes) render in one launch over one
+// global tile space: view b owns global tiles [tile0[b], tile0[b+1]), cells
+// [cell0[b], cell0[b+1]) and pixels [pix0[b], pix0[b] + W H) of every
+// concatenated per-pixel plane.  Per-view Gaussian arrays are indexed
+// vg = b * n + g.  Passed by value as a kernel argument (under 2.5 KB).
+constexpr int kMaxViews = 64;
+
+struct Views {
+  int count;
+  int W[kMaxViews], H[kMaxViews];
+  int gx[kMaxViews], gy[kMaxViews];  // tile grid
+  int ncx[kMaxViews];                // cells per row (cell lists)
+  int tile0[kMaxViews + 1];
+  int cell0[kMaxViews + 1];
+  int64_t pix0[kMaxViews + 1];
+};
+
+// View owning global tile t (binary search over tile0).
+MSPLAT_DEV int view_of_tile(const Views &v, int t) {
+  int lo = 0, hi = v.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (v.tile0[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---- wave-level helpers (wave64) -------------------------------------------
 
 MSPLAT_DEV int lane_id() { return __lane_id(); }

@@ -5,9 +5,14 @@
 // caller's stream and reports launch errors; nothing here allocates.
 #include "../../include/msplat_amd.h"
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "blend.hpp"
 #include "common.hpp"
@@ -38,6 +43,66 @@ int launched(const char *what) {
 }
 
 hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+// ---- optional kernel timing: HIP events on the launch stream ---------------
+// While enabled, the main kernel of each entry point is bracketed by a pair of
+// events recorded on the caller's stream; msplat_timing_read synchronises on
+// them afterwards.  Disabled (the default), a launch pays one atomic load.
+
+struct TimingState {
+  std::atomic<bool> on{false};
+  std::mutex mu;
+  std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> recorded;
+  std::vector<hipEvent_t> pool;
+};
+
+TimingState &timing() {
+  static TimingState t;
+  return t;
+}
+
+hipEvent_t pooled_event(TimingState &t) {  // caller holds t.mu
+  if (!t.pool.empty()) {
+    hipEvent_t e = t.pool.back();
+    t.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+class Timed {
+ public:
+  Timed(const char *name, hipStream_t s) : name_(name), s_(s) {
+    TimingState &t = timing();
+    if (!t.on.load(std::memory_order_relaxed)) return;
+    std::lock_guard<std::mutex> g(t.mu);
+    a_ = pooled_event(t);
+    b_ = pooled_event(t);
+    if (a_ && b_ && hipEventRecord(a_, s_) == hipSuccess) return;
+    if (a_) t.pool.push_back(a_);
+    if (b_) t.pool.push_back(b_);
+    a_ = b_ = nullptr;
+  }
+  ~Timed() {
+    if (!a_) return;
+    TimingState &t = timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (hipEventRecord(b_, s_) == hipSuccess) {
+      t.recorded[name_].emplace_back(a_, b_);
+    } else {
+      t.pool.push_back(a_);
+      t.pool.push_back(b_);
+    }
+  }
+  Timed(const Timed &) = delete;
+  Timed &operator=(const Timed &) = delete;
+
+ private:
+  const char *name_;
+  hipStream_t s_;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
 
 #define REQUIRE(cond, ...)                                         \
   do {                                                             \
@@ -138,6 +203,8 @@ int sh_degree(int D) {
   return -1;
 }
 
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 // Workgroup shape of the LDS-staged SH kernels: G rows of odd stride S in at
 // most 64 KiB of LDS; G = 0 when a row is too long (direct kernels instead).
 ShGeom sh_geom(int C, int D) {
@@ -159,6 +226,13 @@ template <int L>
 void sh_fwd_launch(int64_t n, int C, const float *shs, const float *dirs, const uint8_t *vis, float *val,
                    hipStream_t s) {
   const ShGeom g = sh_geom(C, (L + 1) * (L + 1));
+  Timed timed_("sh_fwd", s);
+  if constexpr (L <= kShGcMaxDeg) {
+    if (C <= kWave && aligned16(shs)) {
+      sh_fwd_gc_kernel<L><<<grid_1d(n * C, 256), 256, 0, s>>>(n, C, shs, dirs, vis, val);
+      return;
+    }
+  }
   if (g.G)
     sh_fwd_lds_kernel<L><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dirs, vis, val, g);
   else
@@ -168,6 +242,14 @@ template <int L>
 void sh_bwd_launch(int64_t n, int C, const float *shs, const float *dirs, const uint8_t *vis, const float *dval,
                    float *dshs, float *ddirs, hipStream_t s) {
   const ShGeom g = sh_geom(C, (L + 1) * (L + 1));
+  Timed timed_("sh_bwd", s);
+  if constexpr (L <= kShGcMaxDeg) {
+    if (C <= kWave && aligned16(shs) && aligned16(dshs)) {
+      const int64_t waves = (n + kWave / C - 1) / (kWave / C);
+      sh_bwd_gc_kernel<L><<<grid_1d(waves * kWave, 256), 256, 0, s>>>(n, C, shs, dirs, vis, dval, dshs, ddirs);
+      return;
+    }
+  }
   if (g.G)
     sh_bwd_lds_kernel<L><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dirs, vis, dval, dshs,
                                                                               ddirs, g);
@@ -189,6 +271,197 @@ void sh_bwd_launch(int64_t n, int C, const float *shs, const float *dirs, const 
     case 9: CALL(9); break;         \
     default: CALL(10); break;       \
   }
+
+// Stable order of the prepared (depth bits, id) pairs by depth -- four 8-bit
+// LSD passes, ids end in L.va -- and the exclusive scan of the per-Gaussian
+// box counts in that order (total -> *d_count).
+void depth_order(void *ws, const PrepLayout &L, int64_t n, int64_t *d_count, hipStream_t s) {
+  uint32_t *ka = at<uint32_t>(ws, L.ka), *va = at<uint32_t>(ws, L.va);
+  uint32_t *kb = at<uint32_t>(ws, L.kb), *vb = at<uint32_t>(ws, L.vb);
+  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums)};
+  {
+    Timed timed_("depth_sort", s);
+    sortk::radix_pass(ka, va, kb, vb, n, 0, 8, rb, s);  // a -> b -> a -> b -> a
+    sortk::radix_pass(kb, vb, ka, va, n, 8, 8, rb, s);
+    sortk::radix_pass(ka, va, kb, vb, n, 16, 8, rb, s);
+    sortk::radix_pass(kb, vb, ka, va, n, 24, 8, rb, s);
+  }
+  sortk::exclusive_scan<int32_t, int64_t>(at<int32_t>(ws, L.area), va, n, at<int64_t>(ws, L.sums),
+                                          at<int64_t>(ws, L.offs), d_count, s);
+}
+
+// Emits the (class, id) pairs of every Gaussian in depth order -- classes are
+// box cells + 1 on a grid ncx wide, cells of S x S tiles -- and sorts them
+// stably by class (ncls + 1 classes).  The ids end in `ids`; returns the
+// sorted classes (in the workspace).
+const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t *radius, int64_t n, int gx, int gy,
+                              int S, int ncx, int ncls, const void *prepared, int64_t k, const int64_t *dlen,
+                              void *ws, const EmitLayout &L, uint32_t *ids, hipStream_t s) {
+  const PrepLayout P = prep_layout(n, 1, 1);
+  const uint32_t *order = at<uint32_t>(prepared, P.va);
+  const int64_t *offs = at<int64_t>(prepared, P.offs);
+  uint32_t *ca = at<uint32_t>(ws, L.ca), *ga = at<uint32_t>(ws, L.ga);
+  uint32_t *cb = at<uint32_t>(ws, L.cb), *gb = at<uint32_t>(ws, L.gb);
+  int64_t *owner = at<int64_t>(ws, L.owner);
+  sortk::emit_owner_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
+      at<int32_t>(prepared, P.area), order, offs, n, sortk::emit_chunks(k), owner);
+  {
+    Timed timed_("emit", s);
+    sortk::emit_chunk_kernel<<<dim3(unsigned(sortk::emit_chunks(k))), sortk::kThreads, 0, s>>>(
+        uv, depth, radius, order, offs, owner, n, k, dlen, gx, gy, S, ncx, ca, ga);
+  }
+  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums)};
+  const int bits = bit_length(uint32_t(ncls));  // classes 0..ncls
+  const int npass = (bits + 7) / 8;
+  const int db = (bits + npass - 1) / npass;
+  uint32_t *sk = ca, *sv = ga;
+  for (int p = 0; p < npass; ++p) {
+    const int shift = p * db;
+    const int pb = bits - shift < db ? bits - shift : db;
+    uint32_t *dk = (sk == ca) ? cb : ca;
+    uint32_t *dv = (p == npass - 1) ? ids : ((sv == ga) ? gb : ga);
+    {
+      Timed timed_("class_sort_pass", s);
+      sortk::radix_pass(sk, sv, dk, dv, k, shift, pb, rb, s, dlen);
+    }
+    sk = dk;
+    sv = dv;
+  }
+  return sk;
+}
+
+// Cells of S x S tiles on the tile grid of a W x H image.
+int cells_x(int W, int S) { return (tiles_x(W) + S - 1) / S; }
+int cells_y(int H, int S) { return (tiles_y(H) + S - 1) / S; }
+bool valid_cell(int S) { return S >= 1 && S <= 4096; }
+
+// ---- blending launchers (tile lists and cell lists) -----------------------
+
+int pick_ppl(int requested, int cm) {
+  int p = requested <= 0 ? 4 : (requested >= 4 ? 4 : (requested >= 2 ? 2 : 1));
+  const int mx = blend_max_ppl(cm);
+  return p < mx ? p : mx;
+}
+
+template <int CM, bool kWalk>
+void fwd_launch(int p, BlendIn in, float *out, float *final_T, int32_t *ncontrib, int32_t *walk, hipStream_t s) {
+  in.per_xcd = blend_per_xcd(in.ntiles, p);
+  Timed timed_(kWalk ? "rasterize_fwd" : "blend_fwd", s);
+  const dim3 grid(unsigned(8 * in.per_xcd));
+  if constexpr (blend_max_ppl(CM) >= 4) {
+    if (p == 4) {
+      blend_fwd_kernel<CM, 4, kWalk><<<grid, kBlendLanes, 0, s>>>(in, out, final_T, ncontrib, walk);
+      return;
+    }
+  }
+  if constexpr (blend_max_ppl(CM) >= 2) {
+    if (p == 2) {
+      blend_fwd_kernel<CM, 2, kWalk><<<grid, kBlendLanes, 0, s>>>(in, out, final_T, ncontrib, walk);
+      return;
+    }
+  }
+  blend_fwd_kernel<CM, 1, kWalk><<<grid, kBlendLanes, 0, s>>>(in, out, final_T, ncontrib, walk);
+}
+
+template <int CM, bool kWalk>
+void bwd_launch(int p, BlendIn in, const float *final_T, const int32_t *ncontrib, const int32_t *walk,
+                const float *dout, float *duv, float *dconic, float *dopac, float *dfeat, hipStream_t s) {
+  in.per_xcd = blend_per_xcd(in.ntiles, p);
+  Timed timed_(kWalk ? "rasterize_bwd" : "blend_bwd", s);
+  const dim3 grid(unsigned(8 * in.per_xcd));
+  if constexpr (blend_max_ppl(CM) >= 4) {
+    if (p == 4) {
+      blend_bwd_kernel<CM, 4, kWalk><<<grid, kBlendLanes, 0, s>>>(in, final_T, ncontrib, walk, dout, duv, dconic,
+                                                                  dopac, dfeat);
+      return;
+    }
+  }
+  if constexpr (blend_max_ppl(CM) >= 2) {
+    if (p == 2) {
+      blend_bwd_kernel<CM, 2, kWalk><<<grid, kBlendLanes, 0, s>>>(in, final_T, ncontrib, walk, dout, duv, dconic,
+                                                                  dopac, dfeat);
+      return;
+    }
+  }
+  blend_bwd_kernel<CM, 1, kWalk><<<grid, kBlendLanes, 0, s>>>(in, final_T, ncontrib, walk, dout, duv, dconic,
+                                                              dopac, dfeat);
+}
+
+#define MSPLAT_BLEND_DISPATCH(CM, CALL) \
+  switch (CM) {                         \
+    case 1: CALL(1); break;             \
+    case 2: CALL(2); break;             \
+    case 3: CALL(3); break;             \
+    case 4: CALL(4); break;             \
+    case 8: CALL(8); break;             \
+    case 16: CALL(16); break;           \
+    case 32: CALL(32); break;           \
+    default: CALL(64); break;           \
+  }
+
+// Forward over all channels, 64 per pass (each pass walks the lists again).
+template <bool kWalk>
+void blend_forward(BlendIn in, int ppl, float *out, float *final_T, int32_t *ncontrib, int32_t *walk,
+                   hipStream_t s) {
+  if (kWalk) launch_fill_zero(walk, 2 * int64_t(in.ntiles), s);
+  int c0 = 0;
+  do {
+    in.c0 = c0;
+    const int rem = in.C - c0 < 64 ? in.C - c0 : 64;
+    const int cm = channel_bucket(rem);
+    const int p = pick_ppl(ppl, cm);
+#define CALL(CMV) fwd_launch<CMV, kWalk>(p, in, out, final_T, ncontrib, walk, s)
+    MSPLAT_BLEND_DISPATCH(cm, CALL)
+#undef CALL
+    c0 += 64;
+  } while (c0 < in.C);
+}
+
+// Adjoint: zero-fills the gradients, then accumulates them pass by pass.
+template <bool kWalk>
+void blend_backward(BlendIn in, int ppl, int64_t n, const float *final_T, const int32_t *ncontrib,
+                    const int32_t *walk, const float *dout, float *duv, float *dconic, float *dopac, float *dfeat,
+                    hipStream_t s) {
+  launch_fill_zero(duv, 2 * n, s);
+  launch_fill_zero(dconic, 3 * n, s);
+  launch_fill_zero(dopac, n, s);
+  if (in.C == 0) return;  // no channel, no gradient
+  launch_fill_zero(dfeat, n * in.C, s);
+  for (int c0 = 0; c0 < in.C; c0 += 64) {
+    in.c0 = c0;
+    const int rem = in.C - c0 < 64 ? in.C - c0 : 64;
+    const int cm = channel_bucket(rem);
+    const int p = pick_ppl(ppl, cm);
+#define CALL(CMV) bwd_launch<CMV, kWalk>(p, in, final_T, ncontrib, walk, dout, duv, dconic, dopac, dfeat, s)
+    MSPLAT_BLEND_DISPATCH(cm, CALL)
+#undef CALL
+  }
+}
+
+BlendIn blend_inputs(const float *uv, const float *conic, const float *opacity, const float *feature, int C,
+                     const int32_t *list, const int32_t *range, const uint32_t *rect, int cell, int cull, float bg,
+                     int W, int H) {
+  BlendIn in{};
+  in.uv = uv;
+  in.conic = conic;
+  in.opacity = opacity;
+  in.feature = feature;
+  in.C = C;
+  in.c0 = 0;
+  in.list = list;
+  in.range = range;
+  in.rect = rect;
+  in.cell = cell;
+  in.ncx = cell > 0 ? cells_x(W, cell) : 0;
+  in.cull = cell > 0 && cull != 0;
+  in.bg = bg;
+  in.W = W;
+  in.H = H;
+  in.gx = tiles_x(W);
+  in.ntiles = tiles_x(W) * tiles_y(H);
+  in.per_xcd = 0;
+  return in;
+}
 }  // namespace
 
 extern "C" {
@@ -196,6 +469,34 @@ extern "C" {
 int msplat_version(void) { return 100; }
 
 const char *msplat_last_error(void) { return g_error.c_str(); }
+
+int msplat_timing_enable(int on) {
+  return timing().on.exchange(on != 0) ? 1 : 0;
+}
+
+int msplat_timing_read(const char *kernel, double *total_ms, int64_t *launches) {
+  REQUIRE(kernel && total_ms && launches, "timing_read: null pointer");
+  TimingState &t = timing();
+  std::lock_guard<std::mutex> g(t.mu);
+  double ms = 0.0;
+  int64_t cnt = 0;
+  auto it = t.recorded.find(kernel);
+  if (it != t.recorded.end()) {
+    for (auto &ev : it->second) {
+      float e = 0.f;
+      if (hipEventSynchronize(ev.second) == hipSuccess && hipEventElapsedTime(&e, ev.first, ev.second) == hipSuccess) {
+        ms += e;
+        ++cnt;
+      }
+      t.pool.push_back(ev.first);
+      t.pool.push_back(ev.second);
+    }
+    t.recorded.erase(it);
+  }
+  *total_ms = ms;
+  *launches = cnt;
+  return MSPLAT_OK;
+}
 
 // ---------------------------------------------------------------- projection
 
@@ -296,6 +597,7 @@ int msplat_preprocess_forward(const float *xyz, const float *scale, const float 
   if (n == 0) return MSPLAT_OK;
   REQUIRE(xyz && scale && rotate && intr && extr && uv && depth && conic && radius && tiles,
           "preprocess_forward: null pointer");
+  Timed timed_("preprocess_fwd", as_stream(stream));
   preprocess_fwd_kernel<<<grid_1d(n, kGeomThreads), kGeomThreads, 0, as_stream(stream)>>>(
       xyz, scale, rotate, n, intr, extr, W, H, tiles_x(W), tiles_y(H), nearest, extent, uv, depth, conic, radius,
       tiles);
@@ -316,6 +618,7 @@ int msplat_preprocess_backward(const float *xyz, const float *scale, const float
   REQUIRE(xyz && scale && rotate && intr && extr && depth && radius && dL_duv && dL_dconic && dL_dxyz &&
               dL_dscale && dL_drotate,
           "preprocess_backward: null pointer");
+  Timed timed_("preprocess_bwd", s);
   if (cam)
     preprocess_bwd_kernel<true><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
         xyz, scale, rotate, n, intr, extr, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
@@ -383,19 +686,10 @@ int msplat_sort_prepare(const float *uv, const float *depth, const int32_t *radi
   launch_fill_zero(flag, 1, s);
   if (n > 0) {
     REQUIRE(uv && depth && radius, "sort_prepare: null pointer");
-    int32_t *area = at<int32_t>(workspace, L.area);
-    uint32_t *ka = at<uint32_t>(workspace, L.ka), *va = at<uint32_t>(workspace, L.va);
-    uint32_t *kb = at<uint32_t>(workspace, L.kb), *vb = at<uint32_t>(workspace, L.vb);
     sortk::prepare_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
-        uv, depth, radius, tiles, n, tiles_x(W), tiles_y(H), area, ka, va, flag);
-    const sortk::RadixBuffers rb{at<uint32_t>(workspace, L.hist), at<int64_t>(workspace, L.sums)};
-    // four 8-bit passes over the depth bits: a -> b -> a -> b -> a
-    sortk::radix_pass(ka, va, kb, vb, n, 0, 8, rb, s);
-    sortk::radix_pass(kb, vb, ka, va, n, 8, 8, rb, s);
-    sortk::radix_pass(ka, va, kb, vb, n, 16, 8, rb, s);
-    sortk::radix_pass(kb, vb, ka, va, n, 24, 8, rb, s);
-    sortk::exclusive_scan<int32_t, int64_t>(area, va, n, at<int64_t>(workspace, L.sums),
-                                            at<int64_t>(workspace, L.offs), d_count, s);
+        uv, depth, radius, tiles, n, tiles_x(W), tiles_y(H), at<int32_t>(workspace, L.area),
+        at<uint32_t>(workspace, L.ka), at<uint32_t>(workspace, L.va), flag);
+    depth_order(workspace, L, n, d_count, s);
   }
   sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, d_count, n);
   return launched("sort_prepare");
@@ -424,35 +718,77 @@ int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius,
   launch_fill_zero(tile_range, 2 * int64_t(T), s);
   if (k == 0) return launched("sort_emit");
   REQUIRE(uv && depth && radius && prepared && workspace && idx_sorted, "sort_emit: null pointer");
-  const PrepLayout P = prep_layout(n, W, H);
-  const uint32_t *order = at<uint32_t>(prepared, P.va);
-  const int64_t *offs = at<int64_t>(prepared, P.offs);
-  uint32_t *ca = at<uint32_t>(workspace, L.ca), *ga = at<uint32_t>(workspace, L.ga);
-  uint32_t *cb = at<uint32_t>(workspace, L.cb), *gb = at<uint32_t>(workspace, L.gb);
-  int64_t *owner = at<int64_t>(workspace, L.owner);
-  sortk::emit_owner_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(at<int32_t>(prepared, P.area),
-                                                                                    order, offs, n, owner);
-  sortk::emit_chunk_kernel<<<dim3(unsigned(sortk::emit_chunks(k))), sortk::kThreads, 0, s>>>(
-      uv, depth, radius, order, offs, owner, n, k, gx, gy, ca, ga);
-  const sortk::RadixBuffers rb{at<uint32_t>(workspace, L.hist), at<int64_t>(workspace, L.sums)};
-  const int bits = bit_length(uint32_t(T));  // classes 0..T
-  const int npass = (bits + 7) / 8;
-  const int db = (bits + npass - 1) / npass;
-  uint32_t *sk = ca, *sv = ga;
-  for (int p = 0; p < npass; ++p) {
-    const int shift = p * db;
-    const int pb = bits - shift < db ? bits - shift : db;
-    uint32_t *dk = (sk == ca) ? cb : ca;
-    uint32_t *dv = (p == npass - 1) ? reinterpret_cast<uint32_t *>(idx_sorted) : ((sv == ga) ? gb : ga);
-    sortk::radix_pass(sk, sv, dk, dv, k, shift, pb, rb, s);
-    sk = dk;
-    sv = dv;
-  }
-  sortk::range_vec_kernel<<<grid_1d((k + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, k, tile_range);
+  const uint32_t *sk = emit_by_class(uv, depth, radius, n, gx, gy, 1, gx, T, prepared, k, nullptr, workspace, L,
+                                     reinterpret_cast<uint32_t *>(idx_sorted), s);
+  sortk::range_vec_kernel<<<grid_1d((k + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, k, nullptr,
+                                                                                           tile_range);
   if (key_sorted)
     sortk::sorted_keys_kernel<<<grid_1d(k, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, idx_sorted, depth, k,
                                                                                       key_sorted);
   return launched("sort_emit");
+}
+
+// ---------------------------------------------------------------- cell binning
+
+size_t msplat_bin_prepare_workspace_bytes(int64_t n, int W, int H, int cell) {
+  if (!valid_image(W, H) || !valid_cell(cell)) return 0;
+  return prep_layout(n < 0 ? 0 : n, W, H).total;
+}
+
+int msplat_bin_prepare(const float *uv, const float *depth, const int32_t *radius, int64_t n, int W, int H, int cell,
+                       uint32_t *rect, void *workspace, size_t workspace_bytes, int64_t *d_count, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX), "bin_prepare: n = %lld", (long long)n);
+  REQUIRE(valid_image(W, H), "bin_prepare: bad image size %dx%d", W, H);
+  REQUIRE(valid_cell(cell), "bin_prepare: cell = %d tiles", cell);
+  REQUIRE(d_count, "bin_prepare: null count pointer");
+  const PrepLayout L = prep_layout(n, W, H);
+  if (workspace_bytes < L.total)
+    return fail(MSPLAT_ERR_WORKSPACE, "bin_prepare: workspace %zu < %zu bytes", workspace_bytes, L.total);
+  REQUIRE(workspace, "bin_prepare: null workspace");
+  hipStream_t s = as_stream(stream);
+  int32_t *flag = at<int32_t>(workspace, L.flag);
+  launch_fill_zero(flag, 1, s);
+  if (n > 0) {
+    REQUIRE(uv && depth && radius && rect, "bin_prepare: null pointer");
+    REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "bin_prepare: rect must be 8-byte aligned");
+    sortk::bin_prepare_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
+        uv, depth, radius, n, tiles_x(W), tiles_y(H), cell, at<int32_t>(workspace, L.area),
+        at<uint32_t>(workspace, L.ka), at<uint32_t>(workspace, L.va), reinterpret_cast<uint2 *>(rect));
+    depth_order(workspace, L, n, d_count, s);
+  }
+  sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, d_count, n);
+  return launched("bin_prepare");
+}
+
+size_t msplat_bin_emit_workspace_bytes(int64_t n, int64_t count, int W, int H, int cell) {
+  (void)n;
+  if (!valid_image(W, H) || !valid_cell(cell)) return 0;
+  return emit_layout(count < 0 ? 0 : count, W, H).total;
+}
+
+int msplat_bin_emit(const float *uv, const float *depth, const int32_t *radius, int64_t n, int W, int H, int cell,
+                    const void *prepared, int64_t count, const int64_t *d_count, void *workspace,
+                    size_t workspace_bytes, int32_t *cell_ids, int32_t *cell_range, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX), "bin_emit: n = %lld", (long long)n);
+  REQUIRE(valid_image(W, H), "bin_emit: bad image size %dx%d", W, H);
+  REQUIRE(valid_cell(cell), "bin_emit: cell = %d tiles", cell);
+  REQUIRE(count >= 0, "bin_emit: count = %lld", (long long)count);
+  if (count > int64_t(INT32_MAX))
+    return fail(MSPLAT_ERR_CAPACITY, "bin_emit: %lld cell entries exceed int32 indexing", (long long)count);
+  REQUIRE(cell_range, "bin_emit: null cell_range");
+  const EmitLayout L = emit_layout(count, W, H);
+  if (workspace_bytes < L.total)
+    return fail(MSPLAT_ERR_WORKSPACE, "bin_emit: workspace %zu < %zu bytes", workspace_bytes, L.total);
+  hipStream_t s = as_stream(stream);
+  const int ncx = cells_x(W, cell), ncells = ncx * cells_y(H, cell);
+  launch_fill_zero(cell_range, 2 * int64_t(ncells), s);
+  if (count == 0) return launched("bin_emit");
+  REQUIRE(uv && depth && radius && prepared && workspace && cell_ids, "bin_emit: null pointer");
+  const uint32_t *sk = emit_by_class(uv, depth, radius, n, tiles_x(W), tiles_y(H), cell, ncx, ncells, prepared,
+                                     count, d_count, workspace, L, reinterpret_cast<uint32_t *>(cell_ids), s);
+  sortk::range_vec_kernel<<<grid_1d((count + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, count, d_count,
+                                                                                               cell_range);
+  return launched("bin_emit");
 }
 
 int msplat_gaussian_keys(const float *uv, const float *depth, const int32_t *radius, const int32_t *offsets,
@@ -489,20 +825,6 @@ int msplat_tile_range_from_keys(const int64_t *key_sorted, int64_t k, int W, int
 
 // ---------------------------------------------------------------- blending
 
-namespace {
-#define MSPLAT_BLEND_DISPATCH(CM, CALL) \
-  switch (CM) {                         \
-    case 1: CALL(1); break;             \
-    case 2: CALL(2); break;             \
-    case 3: CALL(3); break;             \
-    case 4: CALL(4); break;             \
-    case 8: CALL(8); break;             \
-    case 16: CALL(16); break;           \
-    case 32: CALL(32); break;           \
-    default: CALL(64); break;           \
-  }
-}  // namespace
-
 int msplat_alpha_blending_forward(const float *uv, const float *conic, const float *opacity, const float *feature,
                                   int64_t n, int C, const int32_t *idx_sorted, const int32_t *tile_range, float bg,
                                   int W, int H, float *rendered, float *final_T, int32_t *ncontrib, void *stream) {
@@ -511,20 +833,8 @@ int msplat_alpha_blending_forward(const float *uv, const float *conic, const flo
   REQUIRE(valid_image(W, H), "alpha_blending_forward: bad image size %dx%d", W, H);
   REQUIRE(tile_range && final_T && ncontrib && (C == 0 || rendered), "alpha_blending_forward: null output");
   REQUIRE(n == 0 || (uv && conic && opacity && (C == 0 || feature)), "alpha_blending_forward: null input");
-  hipStream_t s = as_stream(stream);
-  const int gx = tiles_x(W), T = gx * tiles_y(H);
-  const dim3 grid(blend_grid(T));
-  int c0 = 0;
-  do {
-    const int rem = C - c0 < 64 ? C - c0 : 64;
-    const int cm = channel_bucket(rem);
-#define CALL(CMV)                                                                                              \
-  blend_fwd_kernel<CMV><<<grid, kBlendLanes, 0, s>>>(uv, conic, opacity, feature, C, c0, idx_sorted, tile_range, \
-                                                     bg, W, H, gx, T, rendered, final_T, ncontrib)
-    MSPLAT_BLEND_DISPATCH(cm, CALL)
-#undef CALL
-    c0 += 64;
-  } while (c0 < C);
+  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, idx_sorted, tile_range, nullptr, 0, 0, bg, W, H);
+  blend_forward<false>(in, 0, rendered, final_T, ncontrib, nullptr, as_stream(stream));
   return launched("alpha_blending_forward");
 }
 
@@ -536,29 +846,53 @@ int msplat_alpha_blending_backward(const float *uv, const float *conic, const fl
   REQUIRE(n >= 0 && n <= int64_t(INT32_MAX) && C >= 0, "alpha_blending_backward: n = %lld, C = %d", (long long)n,
           C);
   REQUIRE(valid_image(W, H), "alpha_blending_backward: bad image size %dx%d", W, H);
-  hipStream_t s = as_stream(stream);
   if (n == 0) return MSPLAT_OK;
   REQUIRE(uv && conic && opacity && idx_sorted && tile_range && final_T && ncontrib && dL_duv && dL_dconic &&
               dL_dopacity && (C == 0 || (feature && dL_drendered && dL_dfeature)),
           "alpha_blending_backward: null pointer");
-  launch_fill_zero(dL_duv, 2 * n, s);
-  launch_fill_zero(dL_dconic, 3 * n, s);
-  launch_fill_zero(dL_dopacity, n, s);
-  if (C > 0) launch_fill_zero(dL_dfeature, n * C, s);
-  if (C == 0) return launched("alpha_blending_backward");  // no channel, no gradient
-  const int gx = tiles_x(W), T = gx * tiles_y(H);
-  const dim3 grid(blend_grid(T));
-  for (int c0 = 0; c0 < C; c0 += 64) {
-    const int rem = C - c0 < 64 ? C - c0 : 64;
-    const int cm = channel_bucket(rem);
-#define CALL(CMV)                                                                                                  \
-  blend_bwd_kernel<CMV><<<grid, kBlendLanes, 0, s>>>(uv, conic, opacity, feature, C, c0, idx_sorted, tile_range,   \
-                                                     bg, W, H, gx, T, final_T, ncontrib, dL_drendered, dL_duv,     \
-                                                     dL_dconic, dL_dopacity, dL_dfeature)
-    MSPLAT_BLEND_DISPATCH(cm, CALL)
-#undef CALL
-  }
+  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, idx_sorted, tile_range, nullptr, 0, 0, bg, W, H);
+  blend_backward<false>(in, 0, n, final_T, ncontrib, nullptr, dL_drendered, dL_duv, dL_dconic, dL_dopacity,
+                        dL_dfeature, as_stream(stream));
   return launched("alpha_blending_backward");
+}
+
+// ---------------------------------------------------------------- rasterizer (cell lists)
+
+int msplat_rasterize_forward(const float *uv, const float *conic, const float *opacity, const float *feature,
+                             int64_t n, int C, const uint32_t *rect, const int32_t *cell_ids,
+                             const int32_t *cell_range, int cell, float bg, int W, int H, int ppl, int cull,
+                             float *rendered,
+                             float *final_T, int32_t *ncontrib, int32_t *walk, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX) && C >= 0, "rasterize_forward: n = %lld, C = %d", (long long)n, C);
+  REQUIRE(valid_image(W, H), "rasterize_forward: bad image size %dx%d", W, H);
+  REQUIRE(valid_cell(cell), "rasterize_forward: cell = %d tiles", cell);
+  REQUIRE(cell_range && final_T && ncontrib && walk && (C == 0 || rendered), "rasterize_forward: null output");
+  REQUIRE(n == 0 || (uv && conic && opacity && rect && cell_ids && (C == 0 || feature)),
+          "rasterize_forward: null input");
+  REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "rasterize_forward: rect must be 8-byte aligned");
+  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, W, H);
+  blend_forward<true>(in, ppl, rendered, final_T, ncontrib, walk, as_stream(stream));
+  return launched("rasterize_forward");
+}
+
+int msplat_rasterize_backward(const float *uv, const float *conic, const float *opacity, const float *feature,
+                              int64_t n, int C, const uint32_t *rect, const int32_t *cell_ids,
+                              const int32_t *cell_range, int cell, float bg, int W, int H, int ppl, int cull,
+                              const float *final_T, const int32_t *ncontrib, const int32_t *walk,
+                              const float *dL_drendered, float *dL_duv, float *dL_dconic, float *dL_dopacity,
+                              float *dL_dfeature, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX) && C >= 0, "rasterize_backward: n = %lld, C = %d", (long long)n, C);
+  REQUIRE(valid_image(W, H), "rasterize_backward: bad image size %dx%d", W, H);
+  REQUIRE(valid_cell(cell), "rasterize_backward: cell = %d tiles", cell);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(uv && conic && opacity && rect && cell_ids && cell_range && final_T && ncontrib && walk && dL_duv &&
+              dL_dconic && dL_dopacity && (C == 0 || (feature && dL_drendered && dL_dfeature)),
+          "rasterize_backward: null pointer");
+  REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "rasterize_backward: rect must be 8-byte aligned");
+  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, W, H);
+  blend_backward<true>(in, ppl, n, final_T, ncontrib, walk, dL_drendered, dL_duv, dL_dconic, dL_dopacity,
+                       dL_dfeature, as_stream(stream));
+  return launched("rasterize_backward");
 }
 
 }  // extern "C"

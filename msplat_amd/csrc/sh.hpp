@@ -139,6 +139,29 @@ MSPLAT_DEV void sh_basis(float x, float y, float z, Emit &&emit) {
 
 constexpr int kShThreads = 256;
 
+// dL/ddir = sum_c sum_k (g_c row[c, k]) grad Y_k: one basis walk per channel,
+// the inner sum in basis order and channels added first to last (the
+// rounding order every SH adjoint kernel shares).
+template <int L>
+MSPLAT_DEV void sh_dir_grad(float x, float y, float z, const float *row, const float *g, int C, float &gx,
+                            float &gy, float &gz) {
+  constexpr int D = (L + 1) * (L + 1);
+  for (int c = 0; c < C; ++c) {
+    const float gc = g[c];
+    const float *rc = row + c * D;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    sh_basis<L>(x, y, z, [&](int k, float, float dx, float dy, float dz) {
+      const float w = gc * rc[k];
+      px += w * dx;
+      py += w * dy;
+      pz += w * dz;
+    });
+    gx += px;
+    gy += py;
+    gz += pz;
+  }
+}
+
 // ---- LDS-staged kernels ------------------------------------------------------
 // A workgroup of G threads owns G consecutive Gaussians, whose coefficient
 // rows (R = C*D floats each) form one contiguous block of HBM.  The block is
@@ -230,29 +253,15 @@ __global__ __launch_bounds__(256) void sh_bwd_lds_kernel(int64_t n, int C, const
     if (live) {
       const float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
       const float *g = dval + i * C;
-      float w[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) w[k] = 0.f;
+      sh_dir_grad<L>(x, y, z, row, g, C, gx, gy, gz);
+      // the row is consumed: overwrite it with dL/dshs = Y_k g_c
+      float Y[D];
+      sh_basis<L>(x, y, z, [&](int k, float v, float, float, float) { Y[k] = v; });
       for (int c = 0; c < C; ++c) {
         const float gc = g[c];
 #pragma unroll
-        for (int k = 0; k < D; ++k) w[k] += gc * row[c * D + k];
+        for (int k = 0; k < D; ++k) row[c * D + k] = Y[k] * gc;
       }
-      // the row is consumed: overwrite it with dL/dshs = Y_k g_c
-      {
-        float Y[D];
-        sh_basis<L>(x, y, z, [&](int k, float v, float, float, float) { Y[k] = v; });
-        for (int c = 0; c < C; ++c) {
-          const float gc = g[c];
-#pragma unroll
-          for (int k = 0; k < D; ++k) row[c * D + k] = Y[k] * gc;
-        }
-      }
-      sh_basis<L>(x, y, z, [&](int k, float, float dx, float dy, float dz) {
-        gx += w[k] * dx;
-        gy += w[k] * dy;
-        gz += w[k] * dz;
-      });
     } else {
       for (int r = 0; r < sg.R; ++r) row[r] = 0.f;
     }
@@ -262,6 +271,128 @@ __global__ __launch_bounds__(256) void sh_bwd_lds_kernel(int64_t n, int C, const
   }
   __syncthreads();
   sh_block_copy<false>(lds, dshs, g0, rows, sg);
+}
+
+// ---- one thread per (Gaussian, channel) ----------------------------------------
+// Thread (i, c) evaluates the basis at Gaussian i's direction and works on row
+// c of its coefficients: D consecutive floats.  Consecutive threads own
+// consecutive rows, so a wave reads and writes one contiguous block (16-byte
+// accesses when D is a multiple of 4 and the arrays are 16-byte aligned) with
+// no LDS and full occupancy.  The basis is evaluated C times per Gaussian,
+// which costs less than the memory traffic it avoids staging.  The adjoint
+// keeps the C threads of a Gaussian in one wave (G = 64 / C Gaussians per
+// wave) and sums their dL/ddir partials, channel 0 first, with lane shuffles.
+// Used for degrees <= 6 and C <= 64 (the basis and a row stay in registers).
+
+constexpr int kShGcMaxDeg = 6;
+
+template <int D>
+MSPLAT_DEV void sh_load_row(const float *__restrict__ row, float (&r)[D]) {
+  if constexpr (D % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < D / 4; ++q) {
+      const float4 v = reinterpret_cast<const float4 *>(row)[q];
+      r[4 * q] = v.x;
+      r[4 * q + 1] = v.y;
+      r[4 * q + 2] = v.z;
+      r[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; ++k) r[k] = row[k];
+  }
+}
+
+template <int D>
+MSPLAT_DEV void sh_store_row(float *__restrict__ row, const float (&r)[D]) {
+  if constexpr (D % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < D / 4; ++q)
+      reinterpret_cast<float4 *>(row)[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; ++k) row[k] = r[k];
+  }
+}
+
+// value[i, c] = sum_k Y_k(dir_i) shs[i, c, k], k ascending.
+template <int L>
+__global__ __launch_bounds__(256) void sh_fwd_gc_kernel(int64_t n, int C, const float *__restrict__ shs,
+                                                        const float *__restrict__ dirs,
+                                                        const uint8_t *__restrict__ vis,
+                                                        float *__restrict__ value) {
+  constexpr int D = (L + 1) * (L + 1);
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;  // row (i, c) = i * C + c
+  if (t >= n * C) return;
+  const int64_t i = t / C;
+  if (vis && !vis[i]) {
+    value[t] = 0.f;
+    return;
+  }
+  float Y[D];
+  sh_basis<L>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2],
+              [&](int k, float v, float, float, float) { Y[k] = v; });
+  float r[D];
+  sh_load_row<D>(shs + t * D, r);
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < D; ++k) acc += Y[k] * r[k];
+  value[t] = acc;
+}
+
+// dL/dshs[i,c,k] = Y_k g_c;  dL/ddir = sum_c sum_k (g_c shs[i,c,k]) grad Y_k,
+// the inner sum in basis order.
+template <int L>
+__global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const float *__restrict__ shs,
+                                                        const float *__restrict__ dirs,
+                                                        const uint8_t *__restrict__ vis,
+                                                        const float *__restrict__ dval,
+                                                        float *__restrict__ dshs, float *__restrict__ ddirs) {
+  constexpr int D = (L + 1) * (L + 1);
+  const int G = kWave / C;  // Gaussians per wave
+  const int lane = threadIdx.x % kWave;
+  const int64_t wave = (int64_t(blockIdx.x) * 256 + threadIdx.x) / kWave;
+  const int j = lane / C, c = lane - j * C;
+  const int64_t i = wave * G + j;
+  const bool active = j < G && i < n;
+  float gx = 0.f, gy = 0.f, gz = 0.f;
+  if (active) {
+    const int64_t t = i * C + c;
+    if (vis && !vis[i]) {
+      float z[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) z[k] = 0.f;
+      sh_store_row<D>(dshs + t * D, z);
+    } else {
+      const float x = dirs[3 * i], y = dirs[3 * i + 1], zz = dirs[3 * i + 2];
+      const float gc = dval[t];
+      float r[D];
+      sh_load_row<D>(shs + t * D, r);
+      float Y[D];
+      sh_basis<L>(x, y, zz, [&](int k, float v, float dx, float dy, float dz) {
+        Y[k] = v * gc;
+        const float w = gc * r[k];
+        gx += w * dx;
+        gy += w * dy;
+        gz += w * dz;
+      });
+      sh_store_row<D>(dshs + t * D, Y);
+    }
+  }
+  // channel partials of one Gaussian sit in lanes j*C .. j*C + C - 1
+  float sx = gx, sy = gy, sz = gz;
+  for (int cc = 1; cc < C; ++cc) {
+    const int src = imin(lane + cc, kWave - 1);
+    const float ox = __shfl(gx, src, kWave), oy = __shfl(gy, src, kWave), oz = __shfl(gz, src, kWave);
+    sx += ox;
+    sy += oy;
+    sz += oz;
+  }
+  if (active && c == 0) {
+    ddirs[3 * i] = sx;
+    ddirs[3 * i + 1] = sy;
+    ddirs[3 * i + 2] = sz;
+  }
 }
 
 // value[i, c] = sum_k Y_k(dir_i) * shs[i, c, k]  (compute_sh.cu:1600-1637)
@@ -322,20 +453,8 @@ __global__ __launch_bounds__(kShThreads) void sh_bwd_kernel(int64_t n, int C, co
       for (int k = 0; k < D; ++k) drow[c * D + k] = Y[k] * gc;
     }
   }
-  float w[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) w[k] = 0.f;
-  for (int c = 0; c < C; ++c) {
-    const float gc = g[c];
-#pragma unroll
-    for (int k = 0; k < D; ++k) w[k] += gc * row[c * D + k];
-  }
   float gx = 0.f, gy = 0.f, gz = 0.f;
-  sh_basis<L>(x, y, z, [&](int k, float, float dx, float dy, float dz) {
-    gx += w[k] * dx;
-    gy += w[k] * dy;
-    gz += w[k] * dz;
-  });
+  sh_dir_grad<L>(x, y, z, row, g, C, gx, gy, gz);
   ddirs[3 * i] = gx;
   ddirs[3 * i + 1] = gy;
   ddirs[3 * i + 2] = gz;

@@ -35,6 +35,15 @@ MSPLAT_HD int64_t scan_tiles(int64_t len) { return (len + kScanTile - 1) / kScan
 
 MSPLAT_DEV uint32_t fbits(float v) { return __float_as_uint(v); }
 
+// Length of a sequence whose buffers hold `cap` elements: *dlen (clamped to
+// cap) when the length lives on the device, else cap.  Grids are sized for
+// cap, so a launch needs no host read of the length.
+MSPLAT_DEV int64_t dev_len(const int64_t *dlen, int64_t cap) {
+  if (!dlen) return cap;
+  const int64_t v = *dlen;
+  return v < 0 ? 0 : (v < cap ? v : cap);
+}
+
 // ---- device-wide exclusive scan (reduce, scan of partials, apply) ------------
 
 // Value at position i of the scanned sequence: in[perm[i]] or in[i].
@@ -139,10 +148,12 @@ inline void exclusive_scan(const In *in, const uint32_t *perm, int64_t len, int6
 // Per-workgroup digit histogram, stored digit-major (hist[d * tiles + t]) so
 // that an exclusive scan of the flat array gives every (digit, workgroup) its
 // first output slot in stable order.
-__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__restrict__ keys, int64_t len,
-                                                              int shift, int bits, uint32_t *__restrict__ hist,
+__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__restrict__ keys, int64_t cap,
+                                                              const int64_t *__restrict__ dlen, int shift,
+                                                              int bits, uint32_t *__restrict__ hist,
                                                               int64_t ntiles) {
   __shared__ uint32_t h[256];
+  const int64_t len = dev_len(dlen, cap);  // tiles past it store zero counts
   h[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t mask = (1u << bits) - 1u;
@@ -165,9 +176,11 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
 //  3. the tile is written out digit run by digit run, so consecutive lanes
 //     store consecutive addresses (coalesced), at offsets from the global scan.
 __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
-    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t len, int shift, int bits,
-    const uint32_t *__restrict__ offsets, int64_t ntiles, uint32_t *__restrict__ kout,
-    uint32_t *__restrict__ vout) {
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t cap,
+    const int64_t *__restrict__ dlen, int shift, int bits, const uint32_t *__restrict__ offsets, int64_t ntiles,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout) {
+  const int64_t len = dev_len(dlen, cap);
+  if (int64_t(blockIdx.x) * kRadixTile >= len) return;  // whole workgroup: past the data
   constexpr int kWaves = kThreads / kWave;
   constexpr int kPerWave = kRadixTile / kWaves;  // 1024 keys, contiguous
   __shared__ uint32_t s_key[kRadixTile];
@@ -254,16 +267,18 @@ struct RadixBuffers {
   int64_t *sums;   // scan_tiles(256 * radix_tiles(len))
 };
 
-// One stable pass over bits [shift, shift + bits).
-inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t len,
-                       int shift, int bits, const RadixBuffers &rb, hipStream_t s) {
-  const int64_t nt = radix_tiles(len);
+// One stable pass over bits [shift, shift + bits) of the first `cap` pairs,
+// or of the first *dlen (<= cap) when dlen is given.
+inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t cap,
+                       int shift, int bits, const RadixBuffers &rb, hipStream_t s,
+                       const int64_t *dlen = nullptr) {
+  const int64_t nt = radix_tiles(cap);
   if (nt == 0) return;
-  radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, len, shift, bits, rb.hist, nt);
+  radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, cap, dlen, shift, bits, rb.hist, nt);
   const int64_t hlen = int64_t(1 << bits) * nt;
   exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
-  radix_scatter_tile_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, vin, len, shift, bits, rb.hist,
-                                                                           nt, kout, vout);
+  radix_scatter_tile_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, vin, cap, dlen, shift, bits,
+                                                                           rb.hist, nt, kout, vout);
 }
 
 // ---- sort-specific kernels ---------------------------------------------------
@@ -288,12 +303,43 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(const float *__restri
   if (tiles && tiles[i] != a) atomicOr(flag, 1);
 }
 
+// Cell binning for the rasterizer: the number of S x S-tile cells that each
+// Gaussian's tile rectangle touches (0 for a negative depth, whose reference
+// key sorts into no tile, and for an empty rectangle), depth bits as the first
+// sort key, identity as the value, and the packed tile rectangle.
+__global__ __launch_bounds__(kThreads) void bin_prepare_kernel(const float *__restrict__ uv,
+                                                               const float *__restrict__ depth,
+                                                               const int32_t *__restrict__ radius, int64_t n,
+                                                               int gx, int gy, int S, int32_t *__restrict__ area,
+                                                               uint32_t *__restrict__ keys,
+                                                               uint32_t *__restrict__ vals,
+                                                               uint2 *__restrict__ rect) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int r = radius[i];
+  const uint32_t db = fbits(depth[i]);
+  int a = 0;
+  TileRect t{0, 0, 0, 0};
+  if (r > 0 && !(db >> 31)) {
+    t = tile_rect(uv[2 * i], uv[2 * i + 1], r, gx, gy);
+    if (t.area() > 0) a = ((t.x1 + S - 1) / S - t.x0 / S) * ((t.y1 + S - 1) / S - t.y0 / S);
+  }
+  area[i] = a;
+  // Gaussians in no cell sort after every listed one (listed keys have the
+  // sign bit clear), so the emission never walks past them.
+  keys[i] = a > 0 ? db : 0xffffffffu;
+  vals[i] = uint32_t(i);
+  rect[i] = pack_rect(t);
+}
+
 // tile_range[c-1] = [first, last+1) of class c in the sorted classes; class 0
 // (negative depth) has no tile.  Each thread reads 4 consecutive classes with
 // one 16-byte load and compares them with their neighbours.  The caller
 // zero-fills tile_range (unset tiles stay (0, 0)).
-__global__ __launch_bounds__(kThreads) void range_vec_kernel(const uint32_t *__restrict__ cls, int64_t k,
+__global__ __launch_bounds__(kThreads) void range_vec_kernel(const uint32_t *__restrict__ cls, int64_t cap,
+                                                             const int64_t *__restrict__ dlen,
                                                              int32_t *__restrict__ range) {
+  const int64_t k = dev_len(dlen, cap);
   const int64_t i0 = (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4;
   if (i0 >= k) return;
   uint32_t c[6];  // c[0] = class at i0 - 1, c[1..4] = i0..i0+3, c[5] = i0 + 4 (0xffffffff outside)
@@ -339,24 +385,29 @@ MSPLAT_HD int64_t emit_chunks(int64_t k) { return (k + kEmitChunk - 1) / kEmitCh
 __global__ __launch_bounds__(kThreads) void emit_owner_kernel(const int32_t *__restrict__ area,
                                                               const uint32_t *__restrict__ order,
                                                               const int64_t *__restrict__ offs, int64_t n,
-                                                              int64_t *__restrict__ owner) {
+                                                              int64_t nchunks, int64_t *__restrict__ owner) {
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n) return;
   const int a = area[order[i]];
   if (a == 0) return;
   const int64_t o = offs[i];
-  for (int64_t c = (o + kEmitChunk - 1) / kEmitChunk; c * kEmitChunk < o + a; ++c) owner[c] = i;
+  for (int64_t c = (o + kEmitChunk - 1) / kEmitChunk; c * kEmitChunk < o + a && c < nchunks; ++c) owner[c] = i;
 }
 
 // Slots are written p = tid, tid + 256, ... (coalesced).  The workgroup stages
 // 256 consecutive Gaussians (starting at the chunk's owner) at a time: start
-// offsets relative to the chunk, tile box and class; a thread finds the
-// Gaussian holding its first slot by binary search and later ones by walking
-// forward.
+// offsets relative to the chunk, box and class; a thread finds the Gaussian
+// holding its first slot by binary search and later ones by walking forward.
+// The box is the tile rectangle (S = 1, classes are tiles + 1, ncx = gx) or
+// the cells of S x S tiles it touches (classes are cells + 1, ncx cells per
+// row).
 __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
     const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius,
     const uint32_t *__restrict__ order, const int64_t *__restrict__ offs, const int64_t *__restrict__ owner,
-    int64_t n, int64_t k, int gx, int gy, uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
+    int64_t n, int64_t cap, const int64_t *__restrict__ dlen, int gx, int gy, int S, int ncx,
+    uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
+  const int64_t k = dev_len(dlen, cap);  // slots past it (or past cap) are not written
+  if (int64_t(blockIdx.x) * kEmitChunk >= k) return;
   __shared__ int32_t s_rel[kThreads + 1];
   __shared__ int32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
   __shared__ uint32_t s_g[kThreads], s_c[kThreads];
@@ -379,9 +430,10 @@ __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
       s_g[tid] = g;
       if (r > 0) {
         const TileRect t = tile_rect(uv[2 * g], uv[2 * g + 1], r, gx, gy);
-        s_x0[tid] = t.x0;
-        s_y0[tid] = t.y0;
-        s_w[tid] = imax(1, t.x1 - t.x0);
+        const int x0 = t.x0 / S, x1 = (t.x1 + S - 1) / S;  // cell columns (tiles when S = 1)
+        s_x0[tid] = x0;
+        s_y0[tid] = t.y0 / S;
+        s_w[tid] = imax(1, x1 - x0);
         // depth sign bit set: the reference key's upper word is all ones
         s_c[tid] = (fbits(depth[g]) >> 31) ? 0u : 1u;
       }
@@ -406,7 +458,7 @@ __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
         int q = int(float(t) * __builtin_amdgcn_rcpf(float(w)));
         int rr = t - q * w;
         if (rr < 0) { --q; rr += w; } else if (rr >= w) { ++q; rr -= w; }
-        cls[c0 + p] = s_c[lo] ? uint32_t((s_y0[lo] + q) * gx + s_x0[lo] + rr) + 1u : 0u;
+        cls[c0 + p] = s_c[lo] ? uint32_t((s_y0[lo] + q) * ncx + s_x0[lo] + rr) + 1u : 0u;
         gid[c0 + p] = s_g[lo];
       }
     }

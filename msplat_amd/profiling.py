@@ -52,3 +52,99 @@ def stage_timer():
 
 def active() -> "StageTimer | None":
     return _active
+
+
+# ---- rocprofv3 output summaries (python -m msplat_amd.profiling ...) ----------
+#
+#   stats DIR                 kernel-trace statistics of a `rocprofv3 --kernel-trace
+#                             --stats --output-format csv -d DIR` run
+#   pmc FETCH_DIR WRITE_DIR MIB
+#                             HBM bytes per launch of the library's kernels from two
+#                             `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
+#                             `bench.py --calibrate-copy-mib MIB`; each counter is scaled
+#                             by what it reads for that copy of known size.
+
+import csv as _csv
+import glob as _glob
+import re as _re
+import sys as _sys
+
+# library kernel -> device function pattern (in rocprof's demangled names)
+KERNEL_PATTERNS = {
+    "rasterize_bwd": r"blend_bwd_kernel<\d+, \d+, true>",
+    "rasterize_fwd": r"blend_fwd_kernel<\d+, \d+, true>",
+    "blend_bwd": r"blend_bwd_kernel<\d+, \d+, false>",
+    "blend_fwd": r"blend_fwd_kernel<\d+, \d+, false>",
+    "sh_fwd": r"sh_fwd_(lds_)?kernel",
+    "sh_bwd": r"sh_bwd_(lds_)?kernel",
+    "preprocess_fwd": r"preprocess_fwd_kernel",
+    "preprocess_bwd": r"preprocess_bwd_kernel",
+    "emit": r"emit_chunk_kernel",
+}
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def _rows(d, suffix):
+    out = []
+    for f in sorted(_glob.glob(f"{d}/**/*{suffix}", recursive=True)):
+        out.extend(_csv.DictReader(open(f)))
+    return out
+
+
+def summarize_stats(d, top=25):
+    rows = _rows(d, "kernel_stats.csv")
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    lines = [f"{'kernel':72s} {'calls':>6s} {'avg_us':>10s} {'total_ms':>10s} {'share':>6s}"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
+        name = _re.sub(r"\(.*", "", r["Name"])[:72]
+        lines.append(f"{name:72s} {r['Calls']:>6s} {float(r['AverageNs']) / 1e3:10.1f} "
+                     f"{float(r['TotalDurationNs']) / 1e6:10.3f} {100 * float(r['TotalDurationNs']) / total:5.1f}%")
+    return "\n".join(lines)
+
+
+def _per_dispatch(d, counter):
+    """{dispatch id: (kernel name, summed counter value)} of one PMC run."""
+    acc = {}
+    for r in _rows(d, "counter_collection.csv"):
+        if r.get("Counter_Name") != counter:
+            continue
+        key = r["Dispatch_Id"]
+        name, v = acc.get(key, (r["Kernel_Name"], 0.0))
+        acc[key] = (name, v + float(r["Counter_Value"]))
+    return acc
+
+
+def summarize_pmc(fetch_dir, write_dir, mib):
+    copy_bytes = mib * (1 << 20)
+    lines = []
+    factors = {}
+    per = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        disp = _per_dispatch(d, counter)
+        copies = [v for name, v in disp.values() if COPY_KERNEL in name]
+        if not copies:
+            raise SystemExit(f"{counter}: no {COPY_KERNEL} dispatch in {d}")
+        kib = max(copies)  # the calibration copy is the largest one
+        factors[counter] = copy_bytes / (kib * 1024.0)
+        lines.append(f"calibration {counter}: copy of {mib} MiB reads {kib:.0f} KiB -> factor {factors[counter]:.4f}")
+        for k, pat in KERNEL_PATTERNS.items():
+            vals = [v for name, v in disp.values() if _re.search(pat, name)]
+            if vals:
+                per.setdefault(k, {})[counter] = (sum(vals) / len(vals) * 1024.0 * factors[counter], len(vals))
+    for k, c in per.items():
+        f = c.get("FETCH_SIZE", (None, 0))[0]
+        w = c.get("WRITE_SIZE", (None, 0))[0]
+        if f is None or w is None:
+            continue
+        lines.append(f"traffic {k} bytes_per_launch={f + w:.0f} fetch={f:.0f} write={w:.0f} "
+                     f"launches={c['FETCH_SIZE'][1]}")
+    return "\n".join(lines)
+
+
+if __name__ == "__main__":
+    if len(_sys.argv) >= 3 and _sys.argv[1] == "stats":
+        print(summarize_stats(_sys.argv[2]))
+    elif len(_sys.argv) >= 5 and _sys.argv[1] == "pmc":
+        print(summarize_pmc(_sys.argv[2], _sys.argv[3], int(_sys.argv[4])))
+    else:
+        raise SystemExit(__doc__)

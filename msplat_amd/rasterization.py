@@ -2,9 +2,12 @@
 
 The reference chains five autograd ops (project_point, compute_cov3d with
 visible = depth != 0, ewa_project, sort_gaussian, alpha_blending).  Here the
-three per-Gaussian stages run as one fused kernel in each direction and the
-whole chain is a single autograd node with a hand-written adjoint; results
-equal the chain's (the fused kernels share the per-stage device code).
+three per-Gaussian stages run as one fused kernel in each direction, the
+sort + blend pair runs over cell lists (msplat_amd.binning: no (tile,
+Gaussian) pairs are materialised), and the whole chain is a single autograd
+node with a hand-written adjoint.  Results equal the chain's: the fused
+kernels share the per-stage device code and the cell walk visits each tile's
+reference list in order.
 """
 
 from __future__ import annotations
@@ -12,9 +15,9 @@ from __future__ import annotations
 import torch
 
 from . import _lib
-from .alpha_blending import blend_backward, blend_forward
+from . import tuning
+from .binning import CellLists, bin_gaussians, rasterize_backward, rasterize_forward
 from .project_point import camera_grad_shape
-from .sort_gaussian import sort_gaussian
 
 
 def preprocess_forward(xyz, scale, rotate, intr, extr, W: int, H: int, nearest: float = 0.0,
@@ -71,21 +74,24 @@ def rasterization(xyz, scale, rotate, opacity, feature, intr, extr, W: int, H: i
 class _Rasterize(torch.autograd.Function):
     @staticmethod
     def forward(ctx, xyz, scale, rotate, opacity, feature, intr, extr, W, H, bg, ndc):
+        cfg = tuning.config_for(W, H, feature.shape[1])
         uv, depth, conic, radius, tiles = preprocess_forward(xyz, scale, rotate, intr, extr, W, H)
-        idx_sorted, tile_range = sort_gaussian(uv, depth, W, H, radius, None)
-        out, final_T, ncontrib = blend_forward(uv, conic, opacity, feature, idx_sorted, tile_range, bg, W, H)
-        ctx.W, ctx.H, ctx.bg = W, H, bg
+        bins = bin_gaussians(uv, depth, W, H, radius, cfg.cell)
+        out, final_T, ncontrib, walk = rasterize_forward(uv, conic, opacity, feature, bins, bg, W, H, cfg.ppl_fwd,
+                                                         cfg.cull)
+        ctx.W, ctx.H, ctx.bg, ctx.cell, ctx.ppl_bwd, ctx.cull = W, H, bg, cfg.cell, cfg.ppl_bwd, cfg.cull
         ctx.has_ndc = ndc is not None
         ctx.save_for_backward(xyz, scale, rotate, opacity, feature, intr, extr, uv, depth, conic, radius,
-                              idx_sorted, tile_range, final_T, ncontrib)
+                              bins.rect, bins.cell_ids, bins.cell_range, final_T, ncontrib, walk)
         return out
 
     @staticmethod
     def backward(ctx, dL_dout):
-        (xyz, scale, rotate, opacity, feature, intr, extr, uv, depth, conic, radius, idx_sorted, tile_range,
-         final_T, ncontrib) = ctx.saved_tensors
-        duv, dconic, dopac, dfeat = blend_backward(uv, conic, opacity, feature, idx_sorted, tile_range, ctx.bg,
-                                                   ctx.W, ctx.H, final_T, ncontrib, dL_dout)
+        (xyz, scale, rotate, opacity, feature, intr, extr, uv, depth, conic, radius, rect, cell_ids, cell_range,
+         final_T, ncontrib, walk) = ctx.saved_tensors
+        bins = CellLists(rect, cell_ids, cell_range, ctx.cell)
+        duv, dconic, dopac, dfeat = rasterize_backward(uv, conic, opacity, feature, bins, ctx.bg, ctx.W, ctx.H,
+                                                       final_T, ncontrib, walk, dL_dout, ctx.ppl_bwd, ctx.cull)
         need_geom = ctx.needs_input_grad[0] or ctx.needs_input_grad[1] or ctx.needs_input_grad[2] \
             or intr.requires_grad or extr.requires_grad
         dxyz = dscale = drot = dintr = dextr = None

@@ -132,12 +132,14 @@ def sh_backward(shs, dirs, dvalue, visible=None, dtype=np.float32):
     n, C, D = s.shape
     Y, dY, order = basis(dirs, degree_of(D), dtype, grad=True)
     dshs = Y[:, None, :] * g[:, :, None]
-    w = np.zeros((n, D), dt)
-    for c in range(C):
-        w = w + g[:, c:c + 1] * s[:, c, :]
+    # dL/ddir = sum_c sum_k (g_c shs[c,k]) grad Y_k: per channel in basis
+    # order, channels summed first to last (the kernels' rounding order)
     dd = np.zeros((n, 3), dt)
-    for k in order:
-        dd = dd + w[:, k:k + 1] * dY[:, k]
+    for c in range(C):
+        part = np.zeros((n, 3), dt)
+        for k in order:
+            part = part + (g[:, c:c + 1] * s[:, c, k:k + 1]) * dY[:, k]
+        dd = dd + part
     if visible is not None:
         vis = np.asarray(visible).reshape(-1)
         dshs = np.where(vis[:, None, None], dshs, dt(0))

@@ -74,3 +74,42 @@ def sort_gaussian(uv, depth, radius, W, H):
     keys, ids, _ = gaussian_keys(uv, depth, radius, W, H)
     ks, idx = sort_keys(keys, ids)
     return idx, tile_range(ks, W, H), ks
+
+
+def cell_lists(uv, depth, radius, W, H, cell):
+    """Cell lists of the rasterizer (no reference counterpart; they replace the
+    sorted tile lists inside rasterization, msplat/__init__.py:83-91).
+
+    Every Gaussian with radius > 0, a non-negative depth bit pattern (sign bit
+    clear: the reference key of a negative depth sorts into no tile) and a
+    non-empty tile rectangle is listed in each cell of ``cell x cell`` tiles its
+    rectangle touches, cells row-major, each cell's list ordered by (depth bits
+    as unsigned, id).  Returns (rect [n,4] int64 = x0, y0, x1, y1; cell_ids
+    [count] int32; cell_range [cells,2] int32).
+    """
+    gx, gy = grid(W, H)
+    ncx, ncy = (gx + cell - 1) // cell, (gy + cell - 1) // cell
+    uv = np.asarray(uv, np.float32)
+    r = np.asarray(radius).reshape(-1).astype(np.int64)
+    dbits = np.asarray(depth, np.float32).reshape(-1).view(np.uint32).astype(np.int64)
+    x0, y0, x1, y1 = tile_rect(uv[:, 0], uv[:, 1], r, gx, gy, np.float32)
+    live = (r > 0) & (dbits >> 31 == 0) & ((x1 - x0) * (y1 - y0) > 0)
+    cx0, cy0 = x0 // cell, y0 // cell
+    cx1, cy1 = (x1 + cell - 1) // cell, (y1 + cell - 1) // cell
+    cells, ids = [], []
+    for g in np.nonzero(live)[0]:
+        for cy in range(cy0[g], cy1[g]):
+            for cx in range(cx0[g], cx1[g]):
+                cells.append(cy * ncx + cx)
+                ids.append(g)
+    cells = np.asarray(cells, np.int64)
+    ids = np.asarray(ids, np.int64)
+    order = np.lexsort((ids, dbits[ids] if len(ids) else ids, cells))
+    cells, ids = cells[order], ids[order]
+    rng = np.zeros((ncx * ncy, 2), np.int32)
+    for c in np.unique(cells):
+        w = np.nonzero(cells == c)[0]
+        rng[c] = (w[0], w[-1] + 1)
+    rect = np.stack([x0, y0, x1, y1], 1).astype(np.int64)
+    rect[~live] = 0
+    return rect, ids.astype(np.int32), rng

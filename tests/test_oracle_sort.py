@@ -96,3 +96,40 @@ def test_invariants():
         s, e = trange[t]
         assert np.all((keys[s:e] >> 32) == t)
         assert np.all(np.diff(depth[idx[s:e]]) >= 0)  # front to back within a tile
+
+
+@pytest.mark.parametrize("cell", [1, 2, 3, 8])
+def test_cell_lists_filter_to_tile_lists(cell):
+    """A tile's list is its cell's list filtered by tile rectangle, for every tile."""
+    rng_ = np.random.default_rng(cell)
+    n, W, H = 300, 200, 136
+    uv = rng_.uniform(-40, 240, (n, 2)).astype(np.float32)
+    depth = rng_.uniform(-1, 8, n).astype(np.float32)
+    depth[:5] = depth[5:10]  # ties in depth
+    depth[10] = -0.0         # sign bit set: no tile
+    radius = rng_.integers(-2, 60, n)
+    idx, trng, _ = osort.sort_gaussian(uv, depth, radius, W, H)
+    rect, ids, crng = osort.cell_lists(uv, depth, radius, W, H, cell)
+    gx, gy = osort.grid(W, H)
+    ncx = (gx + cell - 1) // cell
+    assert 10 not in ids.tolist()
+    for t in range(gx * gy):
+        tx, ty = t % gx, t // gx
+        c = (ty // cell) * ncx + tx // cell
+        s, e = crng[c]
+        walk = [g for g in ids[s:e] if rect[g, 0] <= tx < rect[g, 2] and rect[g, 1] <= ty < rect[g, 3]]
+        a, b = trng[t]
+        assert walk == idx[a:b].tolist(), f"tile {t}"
+
+
+def test_cell_lists_single_cell_is_depth_sort():
+    """With one cell covering the grid the list is the stable depth sort of the live Gaussians."""
+    rng_ = np.random.default_rng(7)
+    n, W, H = 200, 64, 48
+    uv = rng_.uniform(0, 1, (n, 2)).astype(np.float32) * np.float32([64, 48])  # inside the image
+    depth = rng_.choice(np.float32([0.5, 1.0, 2.0, 3.0]), n)
+    radius = rng_.integers(1, 20, n)
+    _, ids, crng = osort.cell_lists(uv, depth, radius, W, H, 8)
+    assert crng.shape == (1, 2) and tuple(crng[0]) == (0, n)
+    expect = sorted(range(n), key=lambda g: (float(depth[g]), g))
+    assert ids.tolist() == expect
