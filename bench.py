@@ -175,6 +175,84 @@ def cpu_baseline(cfg, rows: int = 64) -> dict:
                 seconds=time.perf_counter() - t0)
 
 
+def _time_steps(fn, steps: int, warmup: int) -> float:
+    """Average milliseconds of fn() over `steps` after `warmup` (synchronised)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def bench_multiview(dev, steps: int = 5, warmup: int = 2) -> dict:
+    """N = 2M Gaussians, 16 cameras of seeded sizes in 400x300..1920x1080, C = 3, fwd + bwd (L2) with
+    gradients for the Gaussians and all 16 extrinsics: a loop of single-view calls, and one many-view
+    launch per stage."""
+    import torch
+
+    import msplat_amd as ms
+    from msplat_amd import scenes, views
+
+    cfg = scenes.config_multiview()
+    P = {k: v.to(dev).contiguous().requires_grad_() for k, v in cfg["gaussians"].items()}
+    cams = [c.to(dev) for c in cfg["cameras"]]
+    sizes = [(c.W, c.H) for c in cams]
+    intrs = torch.stack([c.intr for c in cams])
+    extrs = torch.stack([c.extr for c in cams]).requires_grad_()
+    keys = ("xyz", "scale", "rotate", "opacity", "feature")
+    args = [P[k] for k in keys]
+
+    def loop_step():
+        loss = 0.0
+        for b, c in enumerate(cams):
+            out = ms.rasterization(*args, intrs[b], extrs[b], c.W, c.H, 0.0)
+            loss = loss + (out * out).sum()
+        return torch.autograd.grad(loss, args + [extrs])
+
+    def batched_step():
+        imgs = views.rasterization_views(*args, intrs, extrs, sizes, 0.0)
+        loss = sum((im * im).sum() for im in imgs)
+        return torch.autograd.grad(loss, args + [extrs])
+
+    pix = sum(w * h for w, h in sizes)
+    log(f"multiview: N={len(P['xyz'])}, {len(sizes)} views, {pix / 1e6:.2f} Mpix per step")
+    t_loop = _time_steps(loop_step, steps, warmup)
+    t_batch = _time_steps(batched_step, steps, warmup)
+    log(f"multiview: loop {t_loop:.2f} ms, batched {t_batch:.2f} ms per step")
+    return dict(workload="N=2M, 16 views 400x300..1920x1080, C=3, fwd+bwd L2, grads to Gaussians and extrinsics",
+                pixels_per_step=pix, per_view_loop_ms=round(t_loop, 3), batched_ms=round(t_batch, 3),
+                per_view_loop_mpix_s=round(pix / t_loop / 1e3, 2), batched_mpix_s=round(pix / t_batch / 1e3, 2),
+                speedup_batched=round(t_loop / t_batch, 3), graph_capture="disabled (see msplat_amd/graphs.py)")
+
+
+def bench_serving(dev, rate: float, duration: float) -> dict:
+    """4 resident scenes (100k, 500k, 1.5M, 3M Gaussians, C = 3), Poisson requests for a uniformly chosen
+    scene with a random camera at 640x480, 1280x720 or 1920x1080, forward only."""
+    from msplat_amd import scenes, serving
+
+    store = serving.SceneStore(dev, channels=3)
+    for name, g in scenes.config_serving().items():
+        store.add(name, g["xyz"], g["scale"], g["rotate"], g["opacity"], g["feature"])
+    names = sorted(store.scenes)
+
+    def make(rng):
+        W, H = rng.choice(scenes.SERVING_SIZES)
+        cam = scenes.serving_camera(rng, W, H)
+        return serving.Request(rng.choice(names), cam.intr, cam.extr, W, H)
+
+    server = serving.RenderServer(store, hbm_budget=32 << 30, max_views=16, streams=3)
+    stats = serving.run_poisson(server, make, rate=rate, duration=duration, seed=5)
+    log(f"serving: {stats}")
+    stats["workload"] = ("4 resident scenes (100k, 500k, 1.5M, 3M), Poisson arrivals, 640x480 / 1280x720 / "
+                         "1920x1080, forward C=3, HBM budget 32 GiB, <= 16 views per launch")
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +264,10 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=64, help="pixel rows of the CPU blending sample")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a PMC calibration kernel)")
+    ap.add_argument("--no-multiview", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the rasterizer shards by view "
@@ -274,6 +356,9 @@ def main():
         hd = max(hbm, key=lambda k: kern[k]["total_ms"])
         roof_hbm = roofline(hd, kern[hd], w)
 
+    multiview = None if args.no_multiview else bench_multiview(dev)
+    serving_stats = None if args.no_serving else bench_serving(dev, args.serve_rate, args.serve_seconds)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
@@ -288,7 +373,7 @@ def main():
                             cell_entries=w["count"], blend=dict(cell=cfg_used.cell, ppl_fwd=cfg_used.ppl_fwd,
                                                                 ppl_bwd=cfg_used.ppl_bwd),
                             parallelism="single GPU"),
-                roofline=roof, roofline_hbm=roof_hbm, cpu_baseline=cpu,
+                roofline=roof, roofline_hbm=roof_hbm, cpu_baseline=cpu, multiview=multiview, serving=serving_stats,
                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in kern.items()})
     print(json.dumps(line), flush=True)
 

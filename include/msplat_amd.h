@@ -260,6 +260,72 @@ int msplat_rasterize_backward(
     const int32_t *walk, const float *dL_drendered, float *dL_duv,
     float *dL_dconic, float *dL_dopacity, float *dL_dfeature, void *stream);
 
+/* ---- many views in one launch (no reference counterpart: the reference
+ * renders one camera per rasterization call, msplat/__init__.py:22-93).
+ *
+ * B <= 64 cameras, view b of W[b] x H[b] pixels (W, H: host arrays), intr
+ * [B,4] and extr [B,12] (row-major [R|t]) on the device.  View b draws the
+ * Gaussians [first[b], first[b] + count[b]) of the shared per-Gaussian arrays
+ * (xyz, scale, rotate, opacity, shared features; host arrays first / count,
+ * so views of several resident scenes can share a launch), or all n of them
+ * when first = count = NULL.  Per-view arrays (uv, depth, conic, radius,
+ * tiles, rect, their gradients) hold one row per (view, Gaussian): view b's
+ * rows start at R_b = sum of the earlier views' counts (b * n with NULL).
+ * Feature rows are shared ([n, C], row first[b] + g) or per view ([R, C]) when
+ * feat_per_view.  Images are concatenated: view b's rendered [C, H_b, W_b]
+ * starts at C * P_b and its final_T / ncontrib [H_b, W_b] at P_b, with P_b =
+ * sum over earlier views of W*H.  Cells and tiles of all views form one
+ * global space; walk has one row per global tile.  Each view's results equal
+ * the single-view calls' on that view.  The adjoints take views of one scene
+ * (first = count = NULL); gradients of the shared arrays are summed over the
+ * views. */
+int msplat_preprocess_views_forward(const float *xyz, const float *scale,
+                                    const float *rotate, int64_t n, int B,
+                                    const float *intr, const float *extr,
+                                    const int *W, const int *H,
+                                    const int64_t *first, const int64_t *count,
+                                    float nearest, float extent, float *uv,
+                                    float *depth, float *conic, int32_t *radius,
+                                    int32_t *tiles, void *stream);
+/* dL_dintr [B,4] and dL_dextr [B,12] may be NULL (= not requested). */
+int msplat_preprocess_views_backward(
+    const float *xyz, const float *scale, const float *rotate, int64_t n, int B,
+    const float *intr, const float *extr, const float *depth,
+    const int32_t *radius, const float *dL_duv, const float *dL_ddepth,
+    const float *dL_dconic, float *dL_dxyz, float *dL_dscale,
+    float *dL_drotate, float *dL_dintr, float *dL_dextr, void *stream);
+size_t msplat_bin_views_prepare_workspace_bytes(int64_t rows);
+int msplat_bin_views_prepare(const float *uv, const float *depth,
+                             const int32_t *radius, int64_t n, int B,
+                             const int *W, const int *H, const int64_t *first,
+                             const int64_t *count, int cell, uint32_t *rect,
+                             void *workspace, size_t workspace_bytes,
+                             int64_t *d_count, void *stream);
+size_t msplat_bin_views_emit_workspace_bytes(int64_t count);
+int msplat_bin_views_emit(const float *uv, const float *depth,
+                          const int32_t *radius, int64_t n, int B,
+                          const int *W, const int *H, const int64_t *first,
+                          const int64_t *vcount, int cell, const void *prepared,
+                          int64_t count, const int64_t *d_count,
+                          void *workspace, size_t workspace_bytes,
+                          int32_t *cell_ids, int32_t *cell_range, void *stream);
+int msplat_rasterize_views_forward(
+    const float *uv, const float *conic, const float *opacity,
+    const float *feature, int64_t n, int C, int feat_per_view, int B,
+    const int *W, const int *H, const int64_t *first, const int64_t *count,
+    const uint32_t *rect, const int32_t *cell_ids,
+    const int32_t *cell_range, int cell, float bg, int ppl, int cull,
+    float *rendered, float *final_T, int32_t *ncontrib, int32_t *walk,
+    void *stream);
+int msplat_rasterize_views_backward(
+    const float *uv, const float *conic, const float *opacity,
+    const float *feature, int64_t n, int C, int feat_per_view, int B,
+    const int *W, const int *H, const uint32_t *rect, const int32_t *cell_ids,
+    const int32_t *cell_range, int cell, float bg, int ppl, int cull,
+    const float *final_T, const int32_t *ncontrib, const int32_t *walk,
+    const float *dL_drendered, float *dL_duv, float *dL_dconic,
+    float *dL_dopacity, float *dL_dfeature, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,16 @@ SIGNATURES = {
     "msplat_bin_prepare": (I32, [P, P, P, I64, I32, I32, I32, P, P, SZ, P, P]),
     "msplat_bin_emit_workspace_bytes": (SZ, [I64, I64, I32, I32, I32]),
     "msplat_bin_emit": (I32, [P, P, P, I64, I32, I32, I32, P, I64, P, P, SZ, P, P, P]),
+    "msplat_preprocess_views_forward": (I32, [P, P, P, I64, I32, P, P, P, P, P, P, F32, F32, P, P, P, P, P, P]),
+    "msplat_preprocess_views_backward": (I32, [P, P, P, I64, I32, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "msplat_bin_views_prepare_workspace_bytes": (SZ, [I64]),
+    "msplat_bin_views_prepare": (I32, [P, P, P, I64, I32, P, P, P, P, I32, P, P, SZ, P, P]),
+    "msplat_bin_views_emit_workspace_bytes": (SZ, [I64]),
+    "msplat_bin_views_emit": (I32, [P, P, P, I64, I32, P, P, P, P, I32, P, I64, P, P, SZ, P, P, P]),
+    "msplat_rasterize_views_forward": (
+        I32, [P, P, P, P, I64, I32, I32, I32, P, P, P, P, P, P, P, I32, F32, I32, I32, P, P, P, P, P]),
+    "msplat_rasterize_views_backward": (
+        I32, [P, P, P, P, I64, I32, I32, I32, P, P, P, P, P, I32, F32, I32, I32, P, P, P, P, P, P, P, P, P]),
     "msplat_rasterize_forward": (I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P]),
     "msplat_rasterize_backward": (
         I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P, P, P, P, P]),

@@ -26,9 +26,37 @@ from .sort_gaussian import tile_grid
 @dataclasses.dataclass
 class CellLists:
     rect: torch.Tensor        # [n, 2] int32: packed tile rectangle per Gaussian
-    cell_ids: torch.Tensor    # [count] int32: Gaussian ids, cell-major, (depth, id) order within a cell
+    cell_ids: torch.Tensor    # [capacity] int32: Gaussian ids, cell-major, (depth, id) order within a cell
     cell_range: torch.Tensor  # [cells, 2] int32: [start, end) per cell
     cell: int                 # cell edge in tiles
+    count: torch.Tensor | None = None  # [1] int64 on the device: the true number of entries
+
+    def overflowed(self) -> bool:
+        """True if the entries did not fit the capacity (synchronises)."""
+        return self.count is not None and int(self.count.item()) > self.cell_ids.numel()
+
+
+# Capacity plan of a graph capture in progress (msplat_amd.graphs): binning
+# calls without an explicit capacity record their counts (warm-up) or take a
+# preset capacity (capture).
+_plan = None
+
+
+def set_plan(plan):
+    global _plan
+    prev, _plan = _plan, plan
+    return prev
+
+
+def _planned(capacity):
+    if capacity is not None or _plan is None:
+        return capacity
+    return _plan.capacity()
+
+
+def _observe(count: torch.Tensor, capacity):
+    if _plan is not None:
+        _plan.observe(int(count.item()) if capacity is None else -1, count)
 
 
 def cell_grid(W: int, H: int, cell: int) -> tuple[int, int]:
@@ -36,13 +64,17 @@ def cell_grid(W: int, H: int, cell: int) -> tuple[int, int]:
     return (gx + cell - 1) // cell, (gy + cell - 1) // cell
 
 
-def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8) -> CellLists:
+def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8, capacity: int | None = None) -> CellLists:
+    """Cell lists of one view.  With ``capacity`` the entry count stays on the
+    device (no host synchronisation: the call can be captured in a graph);
+    ``overflowed()`` tells afterwards whether the capacity was too small."""
     u = _lib.f32("uv", uv)
     d = _lib.f32("depth", depth).view(-1)
     r = _lib.i32("radius", radius).view(-1)
     n = u.shape[0]
     dev = u.device
     W, H, cell = int(W), int(H), int(cell)
+    capacity = _planned(capacity)
     ncx, ncy = cell_grid(W, H, cell)
     with torch.cuda.device(dev):
         stream = _lib.stream_ptr()
@@ -52,14 +84,17 @@ def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8) -> CellLists
         count = torch.empty(1, dtype=torch.int64, device=dev)
         _lib.call("msplat_bin_prepare", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, W, H, cell, _lib.ptr(rect),
                   _lib.ptr(prep), prep_bytes, _lib.ptr(count), stream)
-        k = int(count.item())  # the only host synchronisation of the binning
+        # the only host synchronisation of the binning, unless a capacity is given
+        k = int(count.item()) if capacity is None else int(capacity)
+        _observe(count, capacity)
         cell_ids = torch.empty(k, dtype=torch.int32, device=dev)
         cell_range = torch.empty((ncx * ncy, 2), dtype=torch.int32, device=dev)
         emit_bytes = int(_lib.query("msplat_bin_emit_workspace_bytes", n, k, W, H, cell))
         ws = torch.empty(max(emit_bytes, 1), dtype=torch.uint8, device=dev)
-        _lib.call("msplat_bin_emit", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, W, H, cell, _lib.ptr(prep), k, None,
-                  _lib.ptr(ws), emit_bytes, _lib.ptr(cell_ids), _lib.ptr(cell_range), stream)
-    return CellLists(rect, cell_ids, cell_range, cell)
+        _lib.call("msplat_bin_emit", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, W, H, cell, _lib.ptr(prep), k,
+                  None if capacity is None else _lib.ptr(count), _lib.ptr(ws), emit_bytes, _lib.ptr(cell_ids),
+                  _lib.ptr(cell_range), stream)
+    return CellLists(rect, cell_ids, cell_range, cell, count)
 
 
 def rasterize_forward(uv, conic, opacity, feature, bins: CellLists, bg: float, W: int, H: int, ppl: int = 0,

@@ -36,22 +36,31 @@ constexpr int kBlendLanes = 64;
 // keeps 3 * CMAX + 5 registers per pixel).
 MSPLAT_HD constexpr int blend_max_ppl(int cmax) { return cmax <= 8 ? 4 : (cmax <= 16 ? 2 : 1); }
 
-// Everything a blending kernel reads.
+// Everything a blending kernel reads.  Per-view arrays (uv, conic, rect, the
+// lists' entries and gradients of those) are indexed by the per-view row vg;
+// opacity by the Gaussian gbase[b] + (vg - vbase[b]); feature rows by vg when
+// feat_per_view, else by that Gaussian.
 struct BlendIn {
   const float *uv, *conic, *opacity, *feature;
   int C, c0;               // channels, first channel of this pass
   const int32_t *list;     // idx_sorted, or cell lists
   const int32_t *range;    // tile_range [T, 2], or cell_range [cells, 2]
   const uint32_t *rect;    // cell lists: tile rectangle per Gaussian (x0 | y0 << 16, x1 | y1 << 16)
-  int cell, ncx;           // cell lists: cell edge in tiles, cells per row
+  int cell;                // cell lists: cell edge in tiles
   int cull;                // cell lists: skip Gaussians that reach no pixel of the tile (may_touch)
   float bg;
-  int W, H, gx, ntiles, per_xcd;
+  int feat_per_view;
+  int ntiles, per_xcd;     // global tiles of all views; workgroups per XCD band
+  Views views;
 };
 
 template <int PPL>
 struct PixBlock {
-  int tile, tx, ty;
+  int tile, view, tx, ty;  // global tile, its view, tile coordinates in the view
+  int W, H;
+  int64_t pix0;            // first pixel of the view in the concatenated planes
+  int64_t vbase;           // first per-view row of the view
+  int64_t gshift;          // Gaussian of per-view row vg = vg + gshift
   bool active;
   int px[PPL], py[PPL];
 };
@@ -65,8 +74,16 @@ MSPLAT_DEV PixBlock<PPL> pix_block(const BlendIn &in) {
   p.tile = w / R;
   const int sub = w % R;
   p.active = p.tile < in.ntiles;
-  p.tx = p.active ? p.tile % in.gx : 0;
-  p.ty = p.active ? p.tile / in.gx : 0;
+  p.view = p.active ? view_of_tile(in.views, p.tile) : 0;
+  const int local = p.active ? p.tile - in.views.tile0[p.view] : 0;
+  const int gx = in.views.gx[p.view];
+  p.tx = local % gx;
+  p.ty = local / gx;
+  p.W = in.views.W[p.view];
+  p.H = in.views.H[p.view];
+  p.pix0 = in.views.pix0[p.view];
+  p.vbase = in.views.vbase[p.view];
+  p.gshift = in.views.gbase[p.view] - p.vbase;
   const int lane = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
@@ -97,23 +114,28 @@ struct BlendStage {
   int32_t pos[kBlendLanes];  // cell lists: list index of each slot
 };
 
-// Stage Gaussian g into LDS slot `slot`: channels [c0, c0 + cc) of its row.
+// Feature row of per-view row vg (gshift: Gaussian = vg + gshift).
+MSPLAT_DEV int64_t feature_row(const BlendIn &in, int64_t vg, int64_t gshift) {
+  return in.feat_per_view ? vg : vg + gshift;
+}
+
+// Stage list entry vg into LDS slot `slot`: channels [c0, c0 + cc) of its row.
 template <int CMAX>
-MSPLAT_DEV void stage_gaussian(BlendStage<CMAX> &s, int slot, int g, const BlendIn &in, int cc) {
+MSPLAT_DEV void stage_gaussian(BlendStage<CMAX> &s, int slot, int vg, const BlendIn &in, int cc, int64_t gshift) {
   constexpr int NQ = BlendStage<CMAX>::kQuads;
   float r[NQ * 4];
-  r[0] = in.uv[2 * int64_t(g)];
-  r[1] = in.uv[2 * int64_t(g) + 1];
-  r[2] = in.conic[3 * int64_t(g)];
-  r[3] = in.conic[3 * int64_t(g) + 1];
-  r[4] = in.conic[3 * int64_t(g) + 2];
-  r[5] = in.opacity[g];
-  const float *fr = in.feature + int64_t(g) * in.C + in.c0;
+  r[0] = in.uv[2 * int64_t(vg)];
+  r[1] = in.uv[2 * int64_t(vg) + 1];
+  r[2] = in.conic[3 * int64_t(vg)];
+  r[3] = in.conic[3 * int64_t(vg) + 1];
+  r[4] = in.conic[3 * int64_t(vg) + 2];
+  r[5] = in.opacity[vg + gshift];
+  const float *fr = in.feature + feature_row(in, vg, gshift) * in.C + in.c0;
 #pragma unroll
   for (int i = 6; i < NQ * 4; ++i) r[i] = (i - 6 < cc) ? fr[i - 6] : 0.f;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) s.rec[slot * NQ + q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
-  s.id[slot] = g;
+  s.id[slot] = vg;
 }
 
 // Geometry of record j: (u, v, a, b) and (c, o, f0, f1).
@@ -155,10 +177,10 @@ MSPLAT_DEV int mask_rank(uint64_t m) {
 // parameters are never culled.  Culled Gaussians change neither colour nor
 // transmittance, so images, final_T and gradients are unchanged; only the
 // list positions counted in ncontrib differ.
-MSPLAT_DEV bool may_touch(const BlendIn &in, int g, int tx, int ty) {
-  const float u = in.uv[2 * int64_t(g)], v = in.uv[2 * int64_t(g) + 1];
-  const float a = in.conic[3 * int64_t(g)], b = in.conic[3 * int64_t(g) + 1], c = in.conic[3 * int64_t(g) + 2];
-  const float o = in.opacity[g];
+MSPLAT_DEV bool may_touch(const BlendIn &in, int vg, int64_t gshift, int tx, int ty) {
+  const float u = in.uv[2 * int64_t(vg)], v = in.uv[2 * int64_t(vg) + 1];
+  const float a = in.conic[3 * int64_t(vg)], b = in.conic[3 * int64_t(vg) + 1], c = in.conic[3 * int64_t(vg) + 2];
+  const float o = in.opacity[vg + gshift];
   if (!(a > 0.f && c > 0.f && a * c - b * b > 0.f && o > 0.f && o < 1e30f && fabsf(u) < 1e30f &&
         fabsf(v) < 1e30f))
     return true;
@@ -176,15 +198,16 @@ MSPLAT_DEV bool may_touch(const BlendIn &in, int g, int tx, int ty) {
 
 // Cell-list filter: the Gaussian's tile rectangle covers the tile (and, with
 // culling, it may touch one of the tile's pixels).
-MSPLAT_DEV bool walk_keeps(const BlendIn &in, int g, int tx, int ty) {
-  if (!rect_covers(reinterpret_cast<const uint2 *>(in.rect)[g], tx, ty)) return false;
-  return !in.cull || may_touch(in, g, tx, ty);
+MSPLAT_DEV bool walk_keeps(const BlendIn &in, int vg, int64_t gshift, int tx, int ty) {
+  if (!rect_covers(reinterpret_cast<const uint2 *>(in.rect)[vg], tx, ty)) return false;
+  return !in.cull || may_touch(in, vg, gshift, tx, ty);
 }
 
 // The list a tile walks: [start, end) of its tile list or of its cell's list.
-template <bool kWalk>
-MSPLAT_DEV int2 tile_list(const BlendIn &in, int tile, int tx, int ty) {
-  const int c = kWalk ? (ty / in.cell) * in.ncx + tx / in.cell : tile;
+template <bool kWalk, int PPL>
+MSPLAT_DEV int2 tile_list(const BlendIn &in, const PixBlock<PPL> &pb) {
+  const int c = kWalk ? in.views.cell0[pb.view] + (pb.ty / in.cell) * in.views.ncx[pb.view] + pb.tx / in.cell
+                      : pb.tile;
   return make_int2(in.range[2 * c], in.range[2 * c + 1]);
 }
 
@@ -210,7 +233,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
   if (!pb.active) return;
   const int lane = threadIdx.x;
   const int cc = in.C - in.c0 < CMAX ? in.C - in.c0 : CMAX;  // channels of this pass
-  const int2 lst = tile_list<kWalk>(in, pb.tile, pb.tx, pb.ty);
+  const int2 lst = tile_list<kWalk>(in, pb);
   int pos = lst.x;
   const int end = lst.y;
 
@@ -220,7 +243,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
   float F[PPL][CMAX];
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
-    done[i] = !(pb.px[i] < in.W && pb.py[i] < in.H);
+    done[i] = !(pb.px[i] < pb.W && pb.py[i] < pb.H);
     T[i] = 1.f;
     last[i] = 0;
     last_e[i] = 0;
@@ -242,19 +265,19 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
       bool pass = false;
       if (e < end) {
         g = in.list[e];
-        pass = walk_keeps(in, g, pb.tx, pb.ty);
+        pass = walk_keeps(in, g, pb.gshift, pb.tx, pb.ty);
       }
       const uint64_t m = __ballot(pass);
       cnt = __popcll(m);
       if (pass) {
         const int slot = mask_rank(m);
-        stage_gaussian<CMAX>(s, slot, g, in, cc);
+        stage_gaussian<CMAX>(s, slot, g, in, cc, pb.gshift);
         s.pos[slot] = e;
       }
       pos += kBlendLanes;
     } else {
       cnt = end - pos < kBlendLanes ? end - pos : kBlendLanes;
-      if (lane < cnt) stage_gaussian<CMAX>(s, lane, in.list[pos + lane], in, cc);
+      if (lane < cnt) stage_gaussian<CMAX>(s, lane, in.list[pos + lane], in, cc, pb.gshift);
       pos += cnt;
     }
     __syncthreads();
@@ -264,41 +287,47 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
       float f[CMAX];
       stage_features(s, j, g1, f);
       const int contributor = fpos + j + 1;
+      int epos = 0;
+      if constexpr (kWalk) epos = s.pos[j] + 1;
 #pragma unroll
       for (int i = 0; i < PPL; ++i) {
-        if (done[i]) continue;
+        // Branch-free: every quantity is computed and the reference's
+        // early-outs become selects, so each update happens exactly where the
+        // reference's would (with the same operations) and nowhere else.
         const float dx = g0.x - float(pb.px[i]), dy = g0.y - float(pb.py[i]);
         const float power = gauss_power(g0.z, g0.w, g1.x, dx, dy);
-        if (power > 0.f) continue;  // a NaN power is not skipped, as in the reference
         const float alpha = fminf(0.99f, g1.y * fast_exp(power));
-        if (alpha < 1.f / 255.f) continue;
+        // a NaN power is not skipped, as in the reference
+        const bool hit = !done[i] && !(power > 0.f) && !(alpha < 1.f / 255.f);
         const float nT = T[i] * (1.f - alpha);
-        if (nT < 0.0001f) {
-          done[i] = true;
-          continue;
-        }
+        const bool sat = hit && nT < 0.0001f;
+        const bool add = hit && !sat;
 #pragma unroll
-        for (int ch = 0; ch < CMAX; ++ch)
-          if (ch < cc) F[i][ch] += f[ch] * alpha * T[i];
-        T[i] = nT;
-        last[i] = contributor;
-        if constexpr (kWalk) last_e[i] = s.pos[j] + 1;
+        for (int ch = 0; ch < CMAX; ++ch) {  // channels >= cc are staged as 0 and never stored
+          const float nf = F[i][ch] + f[ch] * alpha * T[i];
+          F[i][ch] = add ? nf : F[i][ch];
+        }
+        T[i] = add ? nT : T[i];
+        last[i] = add ? contributor : last[i];
+        if constexpr (kWalk) last_e[i] = add ? epos : last_e[i];
+        done[i] = done[i] || sat;
       }
     }
     fpos += cnt;
   }
-  const int64_t hw = int64_t(in.H) * in.W;
+  const int64_t hw = int64_t(pb.H) * pb.W;
+  float *vout = out + int64_t(in.C) * pb.pix0;  // this view's [C, H, W]
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
-    if (!(pb.px[i] < in.W && pb.py[i] < in.H)) continue;
-    const int64_t pix = int64_t(pb.py[i]) * in.W + pb.px[i];
+    if (!(pb.px[i] < pb.W && pb.py[i] < pb.H)) continue;
+    const int64_t pix = int64_t(pb.py[i]) * pb.W + pb.px[i];
     if (in.c0 == 0) {
-      final_T[pix] = T[i];
-      ncontrib[pix] = last[i];
+      final_T[pb.pix0 + pix] = T[i];
+      ncontrib[pb.pix0 + pix] = last[i];
     }
 #pragma unroll
     for (int ch = 0; ch < CMAX; ++ch)
-      if (ch < cc) out[(in.c0 + ch) * hw + pix] = F[i][ch] + T[i] * in.bg;
+      if (ch < cc) vout[(in.c0 + ch) * hw + pix] = F[i][ch] + T[i] * in.bg;
   }
   if constexpr (kWalk) {
     int wl = 0, we = 0;
@@ -330,7 +359,8 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   if (!pb.active) return;
   const int lane = threadIdx.x;
   const int cc = in.C - in.c0 < CMAX ? in.C - in.c0 : CMAX;  // channels of this pass
-  const int64_t hw = int64_t(in.H) * in.W;
+  const int64_t hw = int64_t(pb.H) * pb.W;
+  const float *vdout = dout + int64_t(in.C) * pb.pix0;  // this view's [C, H, W]
 
   float T[PPL], T_final[PPL], bg_dot[PPL], last_alpha[PPL];
   int last[PPL];
@@ -338,15 +368,15 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   int wl = 0;
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
-    const bool inside = pb.px[i] < in.W && pb.py[i] < in.H;
-    const int64_t pix = int64_t(pb.py[i]) * in.W + pb.px[i];
-    T_final[i] = inside ? final_T[pix] : 0.f;
-    last[i] = inside ? ncontrib[pix] : 0;
+    const bool inside = pb.px[i] < pb.W && pb.py[i] < pb.H;
+    const int64_t pix = int64_t(pb.py[i]) * pb.W + pb.px[i];
+    T_final[i] = inside ? final_T[pb.pix0 + pix] : 0.f;
+    last[i] = inside ? ncontrib[pb.pix0 + pix] : 0;
     wl = imax(wl, last[i]);
     bg_dot[i] = 0.f;
 #pragma unroll
     for (int ch = 0; ch < CMAX; ++ch) {
-      g[i][ch] = (inside && ch < cc) ? dout[(in.c0 + ch) * hw + pix] : 0.f;
+      g[i][ch] = (inside && ch < cc) ? vdout[(in.c0 + ch) * hw + pix] : 0.f;
       if (ch < cc) bg_dot[i] += in.bg * g[i][ch];
       accum[i][ch] = last_f[i][ch] = 0.f;
     }
@@ -355,7 +385,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   }
   wl = wave_max(wl);
 
-  const int2 lst = tile_list<kWalk>(in, pb.tile, pb.tx, pb.ty);
+  const int2 lst = tile_list<kWalk>(in, pb);
   const int lo_bound = lst.x;
   int hi, fcount;  // walk backwards over [lo_bound, hi); fcount Gaussians of the tile list lie below hi
   if constexpr (kWalk) {
@@ -376,14 +406,14 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
       bool pass = false;
       if (e < hi) {
         gi = in.list[e];
-        pass = walk_keeps(in, gi, pb.tx, pb.ty);
+        pass = walk_keeps(in, gi, pb.gshift, pb.tx, pb.ty);
       }
       const uint64_t m = __ballot(pass);
       cnt = __popcll(m);
-      if (pass) stage_gaussian<CMAX>(s, mask_rank(m), gi, in, cc);
+      if (pass) stage_gaussian<CMAX>(s, mask_rank(m), gi, in, cc, pb.gshift);
     } else {
       cnt = hi - lo;
-      if (lane < cnt) stage_gaussian<CMAX>(s, lane, in.list[lo + lane], in, cc);
+      if (lane < cnt) stage_gaussian<CMAX>(s, lane, in.list[lo + lane], in, cc, pb.gshift);
     }
     const int fbase = fcount - cnt;  // tile-list position of slot 0
     __syncthreads();
@@ -402,41 +432,41 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
       bool any = false;
 #pragma unroll
       for (int i = 0; i < PPL; ++i) {
-        if (!(rel < last[i])) continue;
+        // Branch-free, as in the forward: the reference's skips become
+        // selects; a pixel's state and sums change only where it contributes.
         const float dx = g0.x - float(pb.px[i]), dy = g0.y - float(pb.py[i]);
         const float power = gauss_power(a, b, c, dx, dy);
-        if (power > 0.f) continue;
         const float G = fast_exp(power);
         const float alpha = fminf(0.99f, op * G);
-        if (alpha < 1.f / 255.f) continue;
-        any = true;
+        const bool hit = rel < last[i] && !(power > 0.f) && !(alpha < 1.f / 255.f);
+        any = any || hit;
         // 1 / (1 - alpha) on the hardware reciprocal (the reference divides
         // with fast-math); 1 - alpha >= 0.01, so it is well conditioned.
         const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-        T[i] = T[i] * inv;
+        const float Tn = T[i] * inv;
+        T[i] = hit ? Tn : T[i];
         const float dcol = alpha * T[i];
         float dL_dalpha = 0.f;
 #pragma unroll
-        for (int ch = 0; ch < CMAX; ++ch) {
-          if (ch < cc) {
-            accum[i][ch] = last_alpha[i] * last_f[i][ch] + (1.f - last_alpha[i]) * accum[i][ch];
-            last_f[i][ch] = fe[ch];
-            dL_dalpha += (fe[ch] - accum[i][ch]) * g[i][ch];
-            vf[ch] += dcol * g[i][ch];
-          }
+        for (int ch = 0; ch < CMAX; ++ch) {  // channels >= cc carry zeros
+          const float acc = last_alpha[i] * last_f[i][ch] + (1.f - last_alpha[i]) * accum[i][ch];
+          accum[i][ch] = hit ? acc : accum[i][ch];
+          last_f[i][ch] = hit ? fe[ch] : last_f[i][ch];
+          dL_dalpha += (fe[ch] - accum[i][ch]) * g[i][ch];
+          vf[ch] += hit ? dcol * g[i][ch] : 0.f;
         }
         dL_dalpha *= T[i];
-        last_alpha[i] = alpha;
+        last_alpha[i] = hit ? alpha : last_alpha[i];
         dL_dalpha += (-T_final[i] * inv) * bg_dot[i];
         const float dL_dG = op * dL_dalpha;
         const float gdx = ((-G) * dx) * a - (G * dy) * b;
         const float gdy = ((-G) * dy) * c - (G * dx) * b;
-        vuv0 += dL_dG * gdx;
-        vuv1 += dL_dG * gdy;
-        vc0 += (((-0.5f * G) * dx) * dx) * dL_dG;
-        vc1 += (((-G) * dx) * dy) * dL_dG;
-        vc2 += (((-0.5f * G) * dy) * dy) * dL_dG;
-        vo += G * dL_dalpha;
+        vuv0 += hit ? dL_dG * gdx : 0.f;
+        vuv1 += hit ? dL_dG * gdy : 0.f;
+        vc0 += hit ? (((-0.5f * G) * dx) * dx) * dL_dG : 0.f;
+        vc1 += hit ? (((-G) * dx) * dy) * dL_dG : 0.f;
+        vc2 += hit ? (((-0.5f * G) * dy) * dy) * dL_dG : 0.f;
+        vo += hit ? G * dL_dalpha : 0.f;
       }
       if (__ballot(any) == 0ull) continue;
       // Sum the 6 + cc per-lane contributions over the wave in one transposed
@@ -468,8 +498,8 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         if (vec < NV / 4 && v < 6 + cc) {
           if (v < 2) dst = duv + 2 * int64_t(gidx) + v;
           else if (v < 5) dst = dconic + 3 * int64_t(gidx) + (v - 2);
-          else if (v == 5) dst = dopac + gidx;
-          else dst = dfeat + int64_t(gidx) * in.C + in.c0 + (v - 6);
+          else if (v == 5) dst = dopac + (gidx + pb.gshift);
+          else dst = dfeat + feature_row(in, gidx, pb.gshift) * in.C + in.c0 + (v - 6);
         }
         if (dst) unsafeAtomicAdd(dst, mine);
       }

@@ -70,21 +70,25 @@ MSPLAT_DEV bool rect_covers(uint2 r, int tx, int ty) {
 }
 
 // ---- views of one launch ---------------------------------------------------
-// Several cameras (of different image sizes) render in one launch over one
-// global tile space: view b owns global tiles [tile0[b], tile0[b+1]), cells
-// [cell0[b], cell0[b+1]) and pixels [pix0[b], pix0[b] + W H) of every
-// concatenated per-pixel plane.  Per-view Gaussian arrays are indexed
-// vg = b * n + g.  Passed by value as a kernel argument (under 2.5 KB).
+// Several cameras (of different image sizes, possibly of different scenes)
+// render in one launch over one global tile space: view b owns global tiles
+// [tile0[b], tile0[b+1]), cells [cell0[b], cell0[b+1]), pixels [pix0[b],
+// pix0[b] + W H) of every concatenated per-pixel plane, and per-view
+// Gaussian rows [vbase[b], vbase[b+1]); its row vbase[b] + g is Gaussian
+// gbase[b] + g of the shared (per-Gaussian) arrays.  Passed by value as a
+// kernel argument (about 3 KB).
 constexpr int kMaxViews = 64;
 
 struct Views {
   int count;
   int W[kMaxViews], H[kMaxViews];
-  int gx[kMaxViews], gy[kMaxViews];  // tile grid
-  int ncx[kMaxViews];                // cells per row (cell lists)
+  uint16_t gx[kMaxViews], gy[kMaxViews];  // tile grid (<= 4096 x 4096)
+  uint16_t ncx[kMaxViews];                // cells per row (cell lists)
   int tile0[kMaxViews + 1];
   int cell0[kMaxViews + 1];
   int64_t pix0[kMaxViews + 1];
+  int64_t vbase[kMaxViews + 1];
+  int64_t gbase[kMaxViews];
 };
 
 // View owning global tile t (binary search over tile0).
@@ -93,6 +97,16 @@ MSPLAT_DEV int view_of_tile(const Views &v, int t) {
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (v.tile0[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// View owning per-view row i (binary search over vbase).
+MSPLAT_DEV int view_of_row(const Views &v, int64_t i) {
+  int lo = 0, hi = v.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (v.vbase[mid] <= i) lo = mid; else hi = mid - 1;
   }
   return lo;
 }

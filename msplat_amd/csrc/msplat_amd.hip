@@ -124,7 +124,7 @@ struct Carver {
 };
 
 struct PrepLayout {
-  size_t area, ka, va, kb, vb, offs, hist, sums, flag, total;
+  size_t area, ka, va, kb, vb, offs, hist, sums, flag, nlisted, total;
 };
 
 // (W, H are part of the size query so that the layout may depend on the
@@ -145,6 +145,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
   if (sortk::scan_tiles(n) > ns) ns = sortk::scan_tiles(n);
   L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
   L.flag = c.take(4);
+  L.nlisted = c.take(8);
   (void)W;
   (void)H;
   L.total = c.off;
@@ -204,6 +205,8 @@ int sh_degree(int D) {
 }
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+int64_t imax64(int64_t a, int64_t b) { return a > b ? a : b; }
 
 // Workgroup shape of the LDS-staged SH kernels: G rows of odd stride S in at
 // most 64 KiB of LDS; G = 0 when a row is too long (direct kernels instead).
@@ -273,42 +276,49 @@ void sh_bwd_launch(int64_t n, int C, const float *shs, const float *dirs, const 
   }
 
 // Stable order of the prepared (depth bits, id) pairs by depth -- four 8-bit
-// LSD passes, ids end in L.va -- and the exclusive scan of the per-Gaussian
-// box counts in that order (total -> *d_count).
-void depth_order(void *ws, const PrepLayout &L, int64_t n, int64_t *d_count, hipStream_t s) {
+// LSD passes, ids end in L.va; only the first *dlen pairs when dlen is given
+// -- and the exclusive scan of the per-Gaussian box counts in that order
+// (total -> *d_count).
+void depth_order(void *ws, const PrepLayout &L, int64_t n, int64_t *d_count, hipStream_t s,
+                 const int64_t *dlen = nullptr) {
   uint32_t *ka = at<uint32_t>(ws, L.ka), *va = at<uint32_t>(ws, L.va);
   uint32_t *kb = at<uint32_t>(ws, L.kb), *vb = at<uint32_t>(ws, L.vb);
   const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums)};
   {
     Timed timed_("depth_sort", s);
-    sortk::radix_pass(ka, va, kb, vb, n, 0, 8, rb, s);  // a -> b -> a -> b -> a
-    sortk::radix_pass(kb, vb, ka, va, n, 8, 8, rb, s);
-    sortk::radix_pass(ka, va, kb, vb, n, 16, 8, rb, s);
-    sortk::radix_pass(kb, vb, ka, va, n, 24, 8, rb, s);
+    sortk::radix_pass(ka, va, kb, vb, n, 0, 8, rb, s, dlen);  // a -> b -> a -> b -> a
+    sortk::radix_pass(kb, vb, ka, va, n, 8, 8, rb, s, dlen);
+    sortk::radix_pass(ka, va, kb, vb, n, 16, 8, rb, s, dlen);
+    sortk::radix_pass(kb, vb, ka, va, n, 24, 8, rb, s, dlen);
   }
   sortk::exclusive_scan<int32_t, int64_t>(at<int32_t>(ws, L.area), va, n, at<int64_t>(ws, L.sums),
                                           at<int64_t>(ws, L.offs), d_count, s);
 }
 
-// Emits the (class, id) pairs of every Gaussian in depth order -- classes are
-// box cells + 1 on a grid ncx wide, cells of S x S tiles -- and sorts them
-// stably by class (ncls + 1 classes).  The ids end in `ids`; returns the
-// sorted classes (in the workspace).
-const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t *radius, int64_t n, int gx, int gy,
-                              int S, int ncx, int ncls, const void *prepared, int64_t k, const int64_t *dlen,
-                              void *ws, const EmitLayout &L, uint32_t *ids, hipStream_t s) {
+// Emits the (class, id) pairs of every entry in depth order -- classes are
+// 1 + global cell (cells of S x S tiles, per-view grids and rows from `vw`)
+// -- and sorts them stably by class.  The ids end
+// in `ids`; returns the sorted classes (in the workspace).
+const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t *radius, int64_t n,
+                              const Views &vw, int S, const void *prepared, int64_t k, const int64_t *dlen,
+                              void *ws, const EmitLayout &L, uint32_t *ids, hipStream_t s, bool direct = false) {
+  const int ncls = vw.cell0[vw.count];
   const PrepLayout P = prep_layout(n, 1, 1);
   const uint32_t *order = at<uint32_t>(prepared, P.va);
   const int64_t *offs = at<int64_t>(prepared, P.offs);
   uint32_t *ca = at<uint32_t>(ws, L.ca), *ga = at<uint32_t>(ws, L.ga);
   uint32_t *cb = at<uint32_t>(ws, L.cb), *gb = at<uint32_t>(ws, L.gb);
   int64_t *owner = at<int64_t>(ws, L.owner);
-  sortk::emit_owner_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
-      at<int32_t>(prepared, P.area), order, offs, n, sortk::emit_chunks(k), owner);
-  {
+  if (direct) {  // a few entries per Gaussian: one thread each
+    Timed timed_("emit", s);
+    sortk::emit_direct_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
+        uv, depth, radius, order, offs, at<int32_t>(prepared, P.area), n, k, dlen, vw, S, ca, ga);
+  } else {  // equal output chunks per workgroup (tile rectangles of thousands of tiles)
+    sortk::emit_owner_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
+        at<int32_t>(prepared, P.area), order, offs, n, sortk::emit_chunks(k), owner);
     Timed timed_("emit", s);
     sortk::emit_chunk_kernel<<<dim3(unsigned(sortk::emit_chunks(k))), sortk::kThreads, 0, s>>>(
-        uv, depth, radius, order, offs, owner, n, k, dlen, gx, gy, S, ncx, ca, ga);
+        uv, depth, radius, order, offs, owner, n, k, dlen, vw, S, ca, ga);
   }
   const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums)};
   const int bits = bit_length(uint32_t(ncls));  // classes 0..ncls
@@ -334,6 +344,53 @@ const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t
 int cells_x(int W, int S) { return (tiles_x(W) + S - 1) / S; }
 int cells_y(int H, int S) { return (tiles_y(H) + S - 1) / S; }
 bool valid_cell(int S) { return S >= 1 && S <= 4096; }
+
+// The Views table of B images (cells of S tiles; S = 1 makes cells = tiles).
+// View b draws Gaussians [first[b], first[b] + count[b]) of the shared arrays
+// into per-view rows vbase[b]..; first = count = NULL means every view draws
+// all n Gaussians (rows b * n + g).  Returns an empty string or what is wrong.
+std::string make_views(int B, const int *W, const int *H, int S, const int64_t *first, const int64_t *count,
+                       int64_t n, Views &v) {
+  if (B < 1 || B > kMaxViews) return "view count " + std::to_string(B) + " outside 1.." + std::to_string(kMaxViews);
+  if (!W || !H) return "null image sizes";
+  if ((first == nullptr) != (count == nullptr)) return "first and count must both be given or both be NULL";
+  v = Views{};
+  v.count = B;
+  int64_t tiles = 0, cells = 0, pix = 0, rows = 0;
+  for (int b = 0; b < B; ++b) {
+    const int64_t f = first ? first[b] : 0, c = count ? count[b] : n;
+    if (f < 0 || c < 0) return "negative Gaussian range of view " + std::to_string(b);
+    v.vbase[b] = rows;
+    v.gbase[b] = f;
+    rows += c;
+    if (!valid_image(W[b], H[b]))
+      return "bad image size " + std::to_string(W[b]) + "x" + std::to_string(H[b]) + " of view " + std::to_string(b);
+    v.W[b] = W[b];
+    v.H[b] = H[b];
+    v.gx[b] = uint16_t(tiles_x(W[b]));
+    v.gy[b] = uint16_t(tiles_y(H[b]));
+    v.ncx[b] = uint16_t(cells_x(W[b], S));
+    v.tile0[b] = int(tiles);
+    v.cell0[b] = int(cells);
+    v.pix0[b] = pix;
+    tiles += int64_t(v.gx[b]) * v.gy[b];
+    cells += int64_t(v.ncx[b]) * cells_y(H[b], S);
+    pix += int64_t(W[b]) * H[b];
+    if (tiles >= INT32_MAX || cells >= INT32_MAX) return "too many tiles";
+  }
+  if (rows > int64_t(INT32_MAX)) return "more than 2^31 per-view rows";
+  v.tile0[B] = int(tiles);
+  v.cell0[B] = int(cells);
+  v.pix0[B] = pix;
+  v.vbase[B] = rows;
+  return "";
+}
+
+Views one_view(int W, int H, int S, int64_t n) {
+  Views v;
+  make_views(1, &W, &H, S, nullptr, nullptr, n, v);
+  return v;
+}
 
 // ---- blending launchers (tile lists and cell lists) -----------------------
 
@@ -418,15 +475,17 @@ void blend_forward(BlendIn in, int ppl, float *out, float *final_T, int32_t *nco
 }
 
 // Adjoint: zero-fills the gradients, then accumulates them pass by pass.
+// m = per-view entries (uv / conic rows), n = Gaussians (opacity rows),
+// nf = feature rows.
 template <bool kWalk>
-void blend_backward(BlendIn in, int ppl, int64_t n, const float *final_T, const int32_t *ncontrib,
-                    const int32_t *walk, const float *dout, float *duv, float *dconic, float *dopac, float *dfeat,
-                    hipStream_t s) {
-  launch_fill_zero(duv, 2 * n, s);
-  launch_fill_zero(dconic, 3 * n, s);
+void blend_backward(BlendIn in, int ppl, int64_t m, int64_t n, int64_t nf, const float *final_T,
+                    const int32_t *ncontrib, const int32_t *walk, const float *dout, float *duv, float *dconic,
+                    float *dopac, float *dfeat, hipStream_t s) {
+  launch_fill_zero(duv, 2 * m, s);
+  launch_fill_zero(dconic, 3 * m, s);
   launch_fill_zero(dopac, n, s);
   if (in.C == 0) return;  // no channel, no gradient
-  launch_fill_zero(dfeat, n * in.C, s);
+  launch_fill_zero(dfeat, nf * in.C, s);
   for (int c0 = 0; c0 < in.C; c0 += 64) {
     in.c0 = c0;
     const int rem = in.C - c0 < 64 ? in.C - c0 : 64;
@@ -440,7 +499,7 @@ void blend_backward(BlendIn in, int ppl, int64_t n, const float *final_T, const 
 
 BlendIn blend_inputs(const float *uv, const float *conic, const float *opacity, const float *feature, int C,
                      const int32_t *list, const int32_t *range, const uint32_t *rect, int cell, int cull, float bg,
-                     int W, int H) {
+                     const Views &vw, int feat_per_view) {
   BlendIn in{};
   in.uv = uv;
   in.conic = conic;
@@ -452,13 +511,11 @@ BlendIn blend_inputs(const float *uv, const float *conic, const float *opacity, 
   in.range = range;
   in.rect = rect;
   in.cell = cell;
-  in.ncx = cell > 0 ? cells_x(W, cell) : 0;
   in.cull = cell > 0 && cull != 0;
   in.bg = bg;
-  in.W = W;
-  in.H = H;
-  in.gx = tiles_x(W);
-  in.ntiles = tiles_x(W) * tiles_y(H);
+  in.feat_per_view = feat_per_view;
+  in.views = vw;
+  in.ntiles = vw.tile0[vw.count];
   in.per_xcd = 0;
   return in;
 }
@@ -718,8 +775,8 @@ int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius,
   launch_fill_zero(tile_range, 2 * int64_t(T), s);
   if (k == 0) return launched("sort_emit");
   REQUIRE(uv && depth && radius && prepared && workspace && idx_sorted, "sort_emit: null pointer");
-  const uint32_t *sk = emit_by_class(uv, depth, radius, n, gx, gy, 1, gx, T, prepared, k, nullptr, workspace, L,
-                                     reinterpret_cast<uint32_t *>(idx_sorted), s);
+  const uint32_t *sk = emit_by_class(uv, depth, radius, n, one_view(W, H, 1, n), 1, prepared, k, nullptr, workspace,
+                                     L, reinterpret_cast<uint32_t *>(idx_sorted), s);
   sortk::range_vec_kernel<<<grid_1d((k + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, k, nullptr,
                                                                                            tile_range);
   if (key_sorted)
@@ -737,27 +794,8 @@ size_t msplat_bin_prepare_workspace_bytes(int64_t n, int W, int H, int cell) {
 
 int msplat_bin_prepare(const float *uv, const float *depth, const int32_t *radius, int64_t n, int W, int H, int cell,
                        uint32_t *rect, void *workspace, size_t workspace_bytes, int64_t *d_count, void *stream) {
-  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX), "bin_prepare: n = %lld", (long long)n);
-  REQUIRE(valid_image(W, H), "bin_prepare: bad image size %dx%d", W, H);
-  REQUIRE(valid_cell(cell), "bin_prepare: cell = %d tiles", cell);
-  REQUIRE(d_count, "bin_prepare: null count pointer");
-  const PrepLayout L = prep_layout(n, W, H);
-  if (workspace_bytes < L.total)
-    return fail(MSPLAT_ERR_WORKSPACE, "bin_prepare: workspace %zu < %zu bytes", workspace_bytes, L.total);
-  REQUIRE(workspace, "bin_prepare: null workspace");
-  hipStream_t s = as_stream(stream);
-  int32_t *flag = at<int32_t>(workspace, L.flag);
-  launch_fill_zero(flag, 1, s);
-  if (n > 0) {
-    REQUIRE(uv && depth && radius && rect, "bin_prepare: null pointer");
-    REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "bin_prepare: rect must be 8-byte aligned");
-    sortk::bin_prepare_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
-        uv, depth, radius, n, tiles_x(W), tiles_y(H), cell, at<int32_t>(workspace, L.area),
-        at<uint32_t>(workspace, L.ka), at<uint32_t>(workspace, L.va), reinterpret_cast<uint2 *>(rect));
-    depth_order(workspace, L, n, d_count, s);
-  }
-  sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, d_count, n);
-  return launched("bin_prepare");
+  return msplat_bin_views_prepare(uv, depth, radius, n, 1, &W, &H, nullptr, nullptr, cell, rect, workspace,
+                                  workspace_bytes, d_count, stream);
 }
 
 size_t msplat_bin_emit_workspace_bytes(int64_t n, int64_t count, int W, int H, int cell) {
@@ -769,23 +807,77 @@ size_t msplat_bin_emit_workspace_bytes(int64_t n, int64_t count, int W, int H, i
 int msplat_bin_emit(const float *uv, const float *depth, const int32_t *radius, int64_t n, int W, int H, int cell,
                     const void *prepared, int64_t count, const int64_t *d_count, void *workspace,
                     size_t workspace_bytes, int32_t *cell_ids, int32_t *cell_range, void *stream) {
-  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX), "bin_emit: n = %lld", (long long)n);
-  REQUIRE(valid_image(W, H), "bin_emit: bad image size %dx%d", W, H);
+  return msplat_bin_views_emit(uv, depth, radius, n, 1, &W, &H, nullptr, nullptr, cell, prepared, count, d_count,
+                               workspace, workspace_bytes, cell_ids, cell_range, stream);
+}
+
+size_t msplat_bin_views_prepare_workspace_bytes(int64_t rows) {
+  if (rows < 0) return 0;
+  return prep_layout(rows, 1, 1).total;
+}
+
+int msplat_bin_views_prepare(const float *uv, const float *depth, const int32_t *radius, int64_t n, int B,
+                             const int *W, const int *H, const int64_t *first, const int64_t *count, int cell,
+                             uint32_t *rect, void *workspace, size_t workspace_bytes, int64_t *d_count,
+                             void *stream) {
+  REQUIRE(n >= 0 && B >= 1, "bin_prepare: n = %lld, views = %d", (long long)n, B);
+  REQUIRE(valid_cell(cell), "bin_prepare: cell = %d tiles", cell);
+  Views vw;
+  const std::string why = make_views(B, W, H, cell, first, count, n, vw);
+  REQUIRE(why.empty(), "bin_prepare: %s", why.c_str());
+  REQUIRE(d_count, "bin_prepare: null count pointer");
+  const int64_t m = vw.vbase[B];  // per-view rows
+  const PrepLayout L = prep_layout(m, 1, 1);
+  if (workspace_bytes < L.total)
+    return fail(MSPLAT_ERR_WORKSPACE, "bin_prepare: workspace %zu < %zu bytes", workspace_bytes, L.total);
+  REQUIRE(workspace, "bin_prepare: null workspace");
+  hipStream_t s = as_stream(stream);
+  int32_t *flag = at<int32_t>(workspace, L.flag);
+  launch_fill_zero(flag, 1, s);
+  if (m > 0) {
+    REQUIRE(uv && depth && radius && rect, "bin_prepare: null pointer");
+    REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "bin_prepare: rect must be 8-byte aligned");
+    int32_t *area = at<int32_t>(workspace, L.area);
+    uint32_t *listed = at<uint32_t>(workspace, L.kb), *pos = at<uint32_t>(workspace, L.vb);
+    int64_t *nlisted = at<int64_t>(workspace, L.nlisted);
+    sortk::bin_prepare_kernel<<<grid_1d(m, sortk::kThreads), sortk::kThreads, 0, s>>>(
+        uv, depth, radius, m, vw, cell, area, listed, reinterpret_cast<uint2 *>(rect));
+    // listed rows first (stable), then the depth sort of those only
+    sortk::exclusive_scan<uint32_t, uint32_t>(listed, nullptr, m, at<int64_t>(workspace, L.sums), pos, nlisted, s);
+    sortk::partition_kernel<<<grid_1d(m, sortk::kThreads), sortk::kThreads, 0, s>>>(
+        depth, area, pos, m, nlisted, at<uint32_t>(workspace, L.ka), at<uint32_t>(workspace, L.va));
+    depth_order(workspace, L, m, d_count, s, nlisted);
+  }
+  sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, d_count, m);
+  return launched("bin_prepare");
+}
+
+size_t msplat_bin_views_emit_workspace_bytes(int64_t count) {
+  return emit_layout(count < 0 ? 0 : count, 1, 1).total;
+}
+
+int msplat_bin_views_emit(const float *uv, const float *depth, const int32_t *radius, int64_t n, int B,
+                          const int *W, const int *H, const int64_t *first, const int64_t *vcount, int cell,
+                          const void *prepared, int64_t count, const int64_t *d_count, void *workspace,
+                          size_t workspace_bytes, int32_t *cell_ids, int32_t *cell_range, void *stream) {
+  REQUIRE(n >= 0 && B >= 1, "bin_emit: n = %lld, views = %d", (long long)n, B);
   REQUIRE(valid_cell(cell), "bin_emit: cell = %d tiles", cell);
+  Views vw;
+  const std::string why = make_views(B, W, H, cell, first, vcount, n, vw);
+  REQUIRE(why.empty(), "bin_emit: %s", why.c_str());
   REQUIRE(count >= 0, "bin_emit: count = %lld", (long long)count);
   if (count > int64_t(INT32_MAX))
     return fail(MSPLAT_ERR_CAPACITY, "bin_emit: %lld cell entries exceed int32 indexing", (long long)count);
   REQUIRE(cell_range, "bin_emit: null cell_range");
-  const EmitLayout L = emit_layout(count, W, H);
+  const EmitLayout L = emit_layout(count, 1, 1);
   if (workspace_bytes < L.total)
     return fail(MSPLAT_ERR_WORKSPACE, "bin_emit: workspace %zu < %zu bytes", workspace_bytes, L.total);
   hipStream_t s = as_stream(stream);
-  const int ncx = cells_x(W, cell), ncells = ncx * cells_y(H, cell);
-  launch_fill_zero(cell_range, 2 * int64_t(ncells), s);
+  launch_fill_zero(cell_range, 2 * int64_t(vw.cell0[B]), s);
   if (count == 0) return launched("bin_emit");
   REQUIRE(uv && depth && radius && prepared && workspace && cell_ids, "bin_emit: null pointer");
-  const uint32_t *sk = emit_by_class(uv, depth, radius, n, tiles_x(W), tiles_y(H), cell, ncx, ncells, prepared,
-                                     count, d_count, workspace, L, reinterpret_cast<uint32_t *>(cell_ids), s);
+  const uint32_t *sk = emit_by_class(uv, depth, radius, vw.vbase[B], vw, cell, prepared, count, d_count, workspace,
+                                     L, reinterpret_cast<uint32_t *>(cell_ids), s, true);
   sortk::range_vec_kernel<<<grid_1d((count + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, count, d_count,
                                                                                                cell_range);
   return launched("bin_emit");
@@ -833,7 +925,8 @@ int msplat_alpha_blending_forward(const float *uv, const float *conic, const flo
   REQUIRE(valid_image(W, H), "alpha_blending_forward: bad image size %dx%d", W, H);
   REQUIRE(tile_range && final_T && ncontrib && (C == 0 || rendered), "alpha_blending_forward: null output");
   REQUIRE(n == 0 || (uv && conic && opacity && (C == 0 || feature)), "alpha_blending_forward: null input");
-  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, idx_sorted, tile_range, nullptr, 0, 0, bg, W, H);
+  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, idx_sorted, tile_range, nullptr, 0, 0, bg,
+                                  one_view(W, H, 1, n), 0);
   blend_forward<false>(in, 0, rendered, final_T, ncontrib, nullptr, as_stream(stream));
   return launched("alpha_blending_forward");
 }
@@ -850,8 +943,9 @@ int msplat_alpha_blending_backward(const float *uv, const float *conic, const fl
   REQUIRE(uv && conic && opacity && idx_sorted && tile_range && final_T && ncontrib && dL_duv && dL_dconic &&
               dL_dopacity && (C == 0 || (feature && dL_drendered && dL_dfeature)),
           "alpha_blending_backward: null pointer");
-  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, idx_sorted, tile_range, nullptr, 0, 0, bg, W, H);
-  blend_backward<false>(in, 0, n, final_T, ncontrib, nullptr, dL_drendered, dL_duv, dL_dconic, dL_dopacity,
+  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, idx_sorted, tile_range, nullptr, 0, 0, bg,
+                                  one_view(W, H, 1, n), 0);
+  blend_backward<false>(in, 0, n, n, n, final_T, ncontrib, nullptr, dL_drendered, dL_duv, dL_dconic, dL_dopacity,
                         dL_dfeature, as_stream(stream));
   return launched("alpha_blending_backward");
 }
@@ -861,18 +955,10 @@ int msplat_alpha_blending_backward(const float *uv, const float *conic, const fl
 int msplat_rasterize_forward(const float *uv, const float *conic, const float *opacity, const float *feature,
                              int64_t n, int C, const uint32_t *rect, const int32_t *cell_ids,
                              const int32_t *cell_range, int cell, float bg, int W, int H, int ppl, int cull,
-                             float *rendered,
-                             float *final_T, int32_t *ncontrib, int32_t *walk, void *stream) {
-  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX) && C >= 0, "rasterize_forward: n = %lld, C = %d", (long long)n, C);
-  REQUIRE(valid_image(W, H), "rasterize_forward: bad image size %dx%d", W, H);
-  REQUIRE(valid_cell(cell), "rasterize_forward: cell = %d tiles", cell);
-  REQUIRE(cell_range && final_T && ncontrib && walk && (C == 0 || rendered), "rasterize_forward: null output");
-  REQUIRE(n == 0 || (uv && conic && opacity && rect && cell_ids && (C == 0 || feature)),
-          "rasterize_forward: null input");
-  REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "rasterize_forward: rect must be 8-byte aligned");
-  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, W, H);
-  blend_forward<true>(in, ppl, rendered, final_T, ncontrib, walk, as_stream(stream));
-  return launched("rasterize_forward");
+                             float *rendered, float *final_T, int32_t *ncontrib, int32_t *walk, void *stream) {
+  return msplat_rasterize_views_forward(uv, conic, opacity, feature, n, C, 0, 1, &W, &H, nullptr, nullptr, rect,
+                                        cell_ids, cell_range, cell, bg, ppl, cull, rendered, final_T, ncontrib, walk,
+                                        stream);
 }
 
 int msplat_rasterize_backward(const float *uv, const float *conic, const float *opacity, const float *feature,
@@ -881,18 +967,104 @@ int msplat_rasterize_backward(const float *uv, const float *conic, const float *
                               const float *final_T, const int32_t *ncontrib, const int32_t *walk,
                               const float *dL_drendered, float *dL_duv, float *dL_dconic, float *dL_dopacity,
                               float *dL_dfeature, void *stream) {
-  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX) && C >= 0, "rasterize_backward: n = %lld, C = %d", (long long)n, C);
-  REQUIRE(valid_image(W, H), "rasterize_backward: bad image size %dx%d", W, H);
+  return msplat_rasterize_views_backward(uv, conic, opacity, feature, n, C, 0, 1, &W, &H, rect, cell_ids,
+                                         cell_range, cell, bg, ppl, cull, final_T, ncontrib, walk, dL_drendered,
+                                         dL_duv, dL_dconic, dL_dopacity, dL_dfeature, stream);
+}
+
+int msplat_rasterize_views_forward(const float *uv, const float *conic, const float *opacity, const float *feature,
+                                   int64_t n, int C, int feat_per_view, int B, const int *W, const int *H,
+                                   const int64_t *first, const int64_t *count, const uint32_t *rect,
+                                   const int32_t *cell_ids, const int32_t *cell_range, int cell, float bg, int ppl,
+                                   int cull, float *rendered, float *final_T, int32_t *ncontrib, int32_t *walk,
+                                   void *stream) {
+  REQUIRE(n >= 0 && C >= 0 && B >= 1, "rasterize_forward: n = %lld, C = %d", (long long)n, C);
+  REQUIRE(valid_cell(cell), "rasterize_forward: cell = %d tiles", cell);
+  Views vw;
+  const std::string why = make_views(B, W, H, cell, first, count, n, vw);
+  REQUIRE(why.empty(), "rasterize_forward: %s", why.c_str());
+  REQUIRE(cell_range && final_T && ncontrib && walk && (C == 0 || rendered), "rasterize_forward: null output");
+  REQUIRE(n == 0 || (uv && conic && opacity && rect && cell_ids && (C == 0 || feature)),
+          "rasterize_forward: null input");
+  REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "rasterize_forward: rect must be 8-byte aligned");
+  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, vw,
+                                  feat_per_view != 0);
+  blend_forward<true>(in, ppl, rendered, final_T, ncontrib, walk, as_stream(stream));
+  return launched("rasterize_forward");
+}
+
+int msplat_rasterize_views_backward(const float *uv, const float *conic, const float *opacity, const float *feature,
+                                    int64_t n, int C, int feat_per_view, int B, const int *W, const int *H,
+                                    const uint32_t *rect, const int32_t *cell_ids, const int32_t *cell_range,
+                                    int cell, float bg, int ppl, int cull, const float *final_T,
+                                    const int32_t *ncontrib, const int32_t *walk, const float *dL_drendered,
+                                    float *dL_duv, float *dL_dconic, float *dL_dopacity, float *dL_dfeature,
+                                    void *stream) {
+  REQUIRE(n >= 0 && C >= 0 && B >= 1, "rasterize_backward: n = %lld, C = %d", (long long)n, C);
   REQUIRE(valid_cell(cell), "rasterize_backward: cell = %d tiles", cell);
+  Views vw;
+  const std::string why = make_views(B, W, H, cell, nullptr, nullptr, n, vw);
+  REQUIRE(why.empty(), "rasterize_backward: %s", why.c_str());
   if (n == 0) return MSPLAT_OK;
   REQUIRE(uv && conic && opacity && rect && cell_ids && cell_range && final_T && ncontrib && walk && dL_duv &&
               dL_dconic && dL_dopacity && (C == 0 || (feature && dL_drendered && dL_dfeature)),
           "rasterize_backward: null pointer");
   REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "rasterize_backward: rect must be 8-byte aligned");
-  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, W, H);
-  blend_backward<true>(in, ppl, n, final_T, ncontrib, walk, dL_drendered, dL_duv, dL_dconic, dL_dopacity,
-                       dL_dfeature, as_stream(stream));
+  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, vw,
+                                  feat_per_view != 0);
+  blend_backward<true>(in, ppl, vw.vbase[B], n, feat_per_view ? vw.vbase[B] : n, final_T, ncontrib, walk,
+                       dL_drendered, dL_duv, dL_dconic, dL_dopacity, dL_dfeature, as_stream(stream));
   return launched("rasterize_backward");
+}
+
+// ---------------------------------------------------------------- many views: preprocessing
+
+int msplat_preprocess_views_forward(const float *xyz, const float *scale, const float *rotate, int64_t n, int B,
+                                    const float *intr, const float *extr, const int *W, const int *H,
+                                    const int64_t *first, const int64_t *count, float nearest, float extent,
+                                    float *uv, float *depth, float *conic, int32_t *radius, int32_t *tiles,
+                                    void *stream) {
+  REQUIRE(n >= 0 && B >= 1, "preprocess_forward: n = %lld, views = %d", (long long)n, B);
+  Views vw;
+  const std::string why = make_views(B, W, H, 1, first, count, n, vw);
+  REQUIRE(why.empty(), "preprocess_forward: %s", why.c_str());
+  int64_t widest = 0;
+  for (int b = 0; b < B; ++b) widest = imax64(widest, vw.vbase[b + 1] - vw.vbase[b]);
+  if (widest == 0) return MSPLAT_OK;
+  REQUIRE(xyz && scale && rotate && intr && extr && uv && depth && conic && radius && tiles,
+          "preprocess_forward: null pointer");
+  Timed timed_("preprocess_fwd", as_stream(stream));
+  preprocess_fwd_views_kernel<<<dim3(grid_1d(widest, kGeomThreads), unsigned(B)), kGeomThreads, 0,
+                                as_stream(stream)>>>(xyz, scale, rotate, intr, extr, vw, nearest, extent, uv, depth,
+                                                     conic, radius, tiles);
+  return launched("preprocess_forward");
+}
+
+int msplat_preprocess_views_backward(const float *xyz, const float *scale, const float *rotate, int64_t n, int B,
+                                     const float *intr, const float *extr, const float *depth,
+                                     const int32_t *radius, const float *dL_duv, const float *dL_ddepth,
+                                     const float *dL_dconic, float *dL_dxyz, float *dL_dscale, float *dL_drotate,
+                                     float *dL_dintr, float *dL_dextr, void *stream) {
+  REQUIRE(n >= 0 && B >= 1 && B <= kMaxViews && n * B <= int64_t(INT32_MAX),
+          "preprocess_backward: n = %lld, views = %d", (long long)n, B);
+  hipStream_t s = as_stream(stream);
+  const bool cam = dL_dintr || dL_dextr;
+  if (dL_dintr) launch_fill_zero(dL_dintr, 4 * int64_t(B), s);
+  if (dL_dextr) launch_fill_zero(dL_dextr, 12 * int64_t(B), s);
+  if (n == 0) return launched("preprocess_backward");
+  REQUIRE(xyz && scale && rotate && intr && extr && depth && radius && dL_duv && dL_dconic && dL_dxyz &&
+              dL_dscale && dL_drotate,
+          "preprocess_backward: null pointer");
+  Timed timed_("preprocess_bwd", s);
+  if (cam)
+    preprocess_bwd_views_kernel<true><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+        xyz, scale, rotate, n, intr, extr, B, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
+        dL_drotate, dL_dintr, dL_dextr);
+  else
+    preprocess_bwd_views_kernel<false><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+        xyz, scale, rotate, n, intr, extr, B, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
+        dL_drotate, nullptr, nullptr);
+  return launched("preprocess_backward");
 }
 
 }  // extern "C"

@@ -549,3 +549,111 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_bwd_kernel(
 }
 
 }  // namespace msplat
+
+namespace msplat {
+
+// ---- many views in one launch -------------------------------------------------
+// Forward: grid.y = view; thread (b, g) projects Gaussian gbase[b] + g into
+// view b and writes per-view row vbase[b] + g.  Per view the values are those
+// of preprocess_fwd_kernel (same device code, same order).
+__global__ __launch_bounds__(kGeomThreads) void preprocess_fwd_views_kernel(
+    const float *__restrict__ xyz, const float *__restrict__ scale, const float *__restrict__ rot,
+    const float *__restrict__ intr, const float *__restrict__ extr, Views vw, float nearest, float extent,
+    float *__restrict__ uv, float *__restrict__ depth, float *__restrict__ conic, int32_t *__restrict__ radius,
+    int32_t *__restrict__ tiles) {
+  const int b = blockIdx.y;
+  const int64_t g = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
+  if (g >= vw.vbase[b + 1] - vw.vbase[b]) return;
+  const int64_t i = vw.gbase[b] + g;
+  const int64_t o = vw.vbase[b] + g;
+  const Camera c = load_camera(intr + 4 * b, extr + 12 * b);
+  const float3 p = ld3(xyz, i);
+  const float3 t = to_camera(c, p.x, p.y, p.z);
+  float u, v;
+  const bool keep = project_core(c, t, vw.W[b], vw.H[b], nearest, extent, u, v);
+  u = keep ? u : 0.f;
+  v = keep ? v : 0.f;
+  const float d = keep ? t.z : 0.f;
+  uv[2 * o] = u;
+  uv[2 * o + 1] = v;
+  depth[o] = d;
+  float3 co = make_float3(0.f, 0.f, 0.f);
+  int r = 0, nt = 0;
+  if (d != 0.f) {
+    float cv[6];
+    cov3d_core(ld3(scale, i), ld4(rot, i), cv);
+    if (!ewa_core(c, t, cv, u, v, vw.gx[b], vw.gy[b], co, r, nt)) {
+      co = make_float3(0.f, 0.f, 0.f);
+      r = 0;
+      nt = 0;
+    }
+  }
+  st3(conic, o, co.x, co.y, co.z);
+  radius[o] = r;
+  tiles[o] = nt;
+}
+
+// Adjoint over views that share one scene (per-view rows b * n + g): thread g
+// sums Gaussian g's gradients over the views (view 0 first) in registers and
+// stores them once; the covariance gradient of every view is summed before
+// the one covariance adjoint; camera partials are reduced per view (dintr +
+// 4 b, dextr + 12 b).
+template <bool kCam>
+__global__ __launch_bounds__(kGeomThreads) void preprocess_bwd_views_kernel(
+    const float *__restrict__ xyz, const float *__restrict__ scale, const float *__restrict__ rot, int64_t n,
+    const float *__restrict__ intr, const float *__restrict__ extr, int nviews, const float *__restrict__ depth,
+    const int32_t *__restrict__ radius, const float *__restrict__ duv, const float *__restrict__ ddepth,
+    const float *__restrict__ dconic, float *__restrict__ dxyz, float *__restrict__ dscale,
+    float *__restrict__ drot, float *__restrict__ dintr, float *__restrict__ dextr) {
+  const int64_t i = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
+  const bool live = i < n;
+  float3 p = make_float3(0.f, 0.f, 0.f), s = p;
+  float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (live) {
+    p = ld3(xyz, i);
+    s = ld3(scale, i);
+    q = ld4(rot, i);
+  }
+  float cv[6];
+  bool have_cov = false;
+  float gp[3] = {0.f, 0.f, 0.f};
+  float gc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  bool seen = false;  // visible in some view
+  for (int b = 0; b < nviews; ++b) {
+    const Camera c = load_camera(intr + 4 * b, extr + 12 * b);
+    float cam[16] = {0.f};
+    const int64_t o = int64_t(b) * n + i;
+    if (live && depth[o] != 0.f) {
+      seen = true;
+      float dp[3] = {0.f, 0.f, 0.f};
+      float de[3] = {0.f, 0.f, 0.f};
+      project_adjoint<kCam>(c, p.x, p.y, p.z, duv[2 * o], duv[2 * o + 1], ddepth ? ddepth[o] : 0.f, dp, cam);
+      if (radius[o] > 0) {
+        if (!have_cov) {
+          cov3d_core(s, q, cv);
+          have_cov = true;
+        }
+        float dc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (ewa_adjoint<kCam>(c, p.x, p.y, p.z, cv, ld3(dconic, o), dc, de, cam)) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) gc[k] += dc[k];
+        } else {
+          de[0] = de[1] = de[2] = 0.f;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gp[k] += dp[k] + de[k];
+    }
+    if constexpr (kCam) reduce_camera(cam, dintr ? dintr + 4 * b : nullptr, dextr ? dextr + 12 * b : nullptr);
+  }
+  if (live) {
+    float3 ds = make_float3(0.f, 0.f, 0.f);
+    float4 dq = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (seen) cov3d_adjoint(s, q, gc, ds, dq);
+    st3(dxyz, i, gp[0], gp[1], gp[2]);
+    st3(dscale, i, ds.x, ds.y, ds.z);
+    st4(drot, i, dq);
+  }
+}
+
+}  // namespace msplat

@@ -307,29 +307,78 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(const float *__restri
 // Gaussian's tile rectangle touches (0 for a negative depth, whose reference
 // key sorts into no tile, and for an empty rectangle), depth bits as the first
 // sort key, identity as the value, and the packed tile rectangle.
+// Entries are the per-view rows of the views in `v`; listed[i] = area > 0.
 __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(const float *__restrict__ uv,
                                                                const float *__restrict__ depth,
                                                                const int32_t *__restrict__ radius, int64_t n,
-                                                               int gx, int gy, int S, int32_t *__restrict__ area,
-                                                               uint32_t *__restrict__ keys,
-                                                               uint32_t *__restrict__ vals,
+                                                               Views v, int S,
+                                                               int32_t *__restrict__ area,
+                                                               uint32_t *__restrict__ listed,
                                                                uint2 *__restrict__ rect) {
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n) return;
+  const int b = view_of_row(v, i);
   const int r = radius[i];
   const uint32_t db = fbits(depth[i]);
   int a = 0;
   TileRect t{0, 0, 0, 0};
   if (r > 0 && !(db >> 31)) {
-    t = tile_rect(uv[2 * i], uv[2 * i + 1], r, gx, gy);
+    t = tile_rect(uv[2 * i], uv[2 * i + 1], r, v.gx[b], v.gy[b]);
     if (t.area() > 0) a = ((t.x1 + S - 1) / S - t.x0 / S) * ((t.y1 + S - 1) / S - t.y0 / S);
   }
   area[i] = a;
-  // Gaussians in no cell sort after every listed one (listed keys have the
-  // sign bit clear), so the emission never walks past them.
-  keys[i] = a > 0 ? db : 0xffffffffu;
-  vals[i] = uint32_t(i);
+  listed[i] = a > 0 ? 1u : 0u;
   rect[i] = pack_rect(t);
+}
+
+// Stable partition of the rows: listed rows (pos[i] = their rank, from an
+// exclusive scan of `listed`) to [0, nlisted) with their depth bits as sort
+// keys, the others after them, so the depth sort only has to order
+// [0, nlisted) and every row still appears in the order array.
+__global__ __launch_bounds__(kThreads) void partition_kernel(const float *__restrict__ depth,
+                                                             const int32_t *__restrict__ area,
+                                                             const uint32_t *__restrict__ pos, int64_t n,
+                                                             const int64_t *__restrict__ nlisted,
+                                                             uint32_t *__restrict__ keys,
+                                                             uint32_t *__restrict__ vals) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = pos[i];
+  const int64_t j = area[i] > 0 ? p : *nlisted + (i - p);
+  keys[j] = fbits(depth[i]);
+  vals[j] = uint32_t(i);
+}
+
+// Emission for cell lists (a few cells per Gaussian): one thread per
+// depth-ordered position i writes its Gaussian's cells, row-major, at slots
+// offs[i]... (slots past the capacity are not written).
+__global__ __launch_bounds__(kThreads) void emit_direct_kernel(
+    const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius,
+    const uint32_t *__restrict__ order, const int64_t *__restrict__ offs, const int32_t *__restrict__ area,
+    int64_t n, int64_t cap, const int64_t *__restrict__ dlen, Views v, int S, uint32_t *__restrict__ cls,
+    uint32_t *__restrict__ gid) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = order[i];
+  const int a = area[g];
+  if (a == 0) return;
+  const int64_t k = dev_len(dlen, cap);
+  const int64_t o = offs[i];
+  const int b = view_of_row(v, g);
+  const TileRect t = tile_rect(uv[2 * int64_t(g)], uv[2 * int64_t(g) + 1], radius[g], v.gx[b], v.gy[b]);
+  const int x0 = t.x0 / S, x1 = (t.x1 + S - 1) / S;
+  const int w = imax(1, x1 - x0), ncx = v.ncx[b];
+  // depth sign bit set: the reference key's upper word is all ones (no tile)
+  const uint32_t base = (fbits(depth[g]) >> 31) ? 0u : uint32_t(v.cell0[b] + (t.y0 / S) * ncx + x0) + 1u;
+  int q = 0, r = 0;
+  for (int e = 0; e < a && o + e < k; ++e) {
+    cls[o + e] = base ? base + uint32_t(q * ncx + r) : 0u;
+    gid[o + e] = g;
+    if (++r == w) {
+      r = 0;
+      ++q;
+    }
+  }
 }
 
 // tile_range[c-1] = [first, last+1) of class c in the sorted classes; class 0
@@ -404,13 +453,13 @@ __global__ __launch_bounds__(kThreads) void emit_owner_kernel(const int32_t *__r
 __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
     const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius,
     const uint32_t *__restrict__ order, const int64_t *__restrict__ offs, const int64_t *__restrict__ owner,
-    int64_t n, int64_t cap, const int64_t *__restrict__ dlen, int gx, int gy, int S, int ncx,
+    int64_t n, int64_t cap, const int64_t *__restrict__ dlen, Views v, int S,
     uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
   const int64_t k = dev_len(dlen, cap);  // slots past it (or past cap) are not written
   if (int64_t(blockIdx.x) * kEmitChunk >= k) return;
   __shared__ int32_t s_rel[kThreads + 1];
-  __shared__ int32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads];
-  __shared__ uint32_t s_g[kThreads], s_c[kThreads];
+  __shared__ int32_t s_w[kThreads], s_ncx[kThreads];
+  __shared__ uint32_t s_g[kThreads], s_c[kThreads];  // id, class of the box's first cell (0: no tile)
   const int tid = threadIdx.x;
   const int64_t c0 = int64_t(blockIdx.x) * kEmitChunk;
   const int span = int(k - c0 < kEmitChunk ? k - c0 : kEmitChunk);
@@ -429,13 +478,13 @@ __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
       rel = rel_of(offs[i]);
       s_g[tid] = g;
       if (r > 0) {
-        const TileRect t = tile_rect(uv[2 * g], uv[2 * g + 1], r, gx, gy);
+        const int b = view_of_row(v, g);
+        const TileRect t = tile_rect(uv[2 * g], uv[2 * g + 1], r, v.gx[b], v.gy[b]);
         const int x0 = t.x0 / S, x1 = (t.x1 + S - 1) / S;  // cell columns (tiles when S = 1)
-        s_x0[tid] = x0;
-        s_y0[tid] = t.y0 / S;
         s_w[tid] = imax(1, x1 - x0);
+        s_ncx[tid] = v.ncx[b];
         // depth sign bit set: the reference key's upper word is all ones
-        s_c[tid] = (fbits(depth[g]) >> 31) ? 0u : 1u;
+        s_c[tid] = (fbits(depth[g]) >> 31) ? 0u : uint32_t(v.cell0[b] + (t.y0 / S) * v.ncx[b] + x0) + 1u;
       }
     }
     s_rel[tid] = rel;
@@ -458,7 +507,7 @@ __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
         int q = int(float(t) * __builtin_amdgcn_rcpf(float(w)));
         int rr = t - q * w;
         if (rr < 0) { --q; rr += w; } else if (rr >= w) { ++q; rr -= w; }
-        cls[c0 + p] = s_c[lo] ? uint32_t((s_y0[lo] + q) * ncx + s_x0[lo] + rr) + 1u : 0u;
+        cls[c0 + p] = s_c[lo] ? s_c[lo] + uint32_t(q * s_ncx[lo] + rr) : 0u;
         gid[c0 + p] = s_g[lo];
       }
     }

@@ -120,3 +120,23 @@ def config_multiview(seed: int = 3, n: int = 2_000_000, views: int = 16) -> dict
 
 def fov_focal(W: int, fov_deg: float) -> float:
     return W / (2 * math.tan(math.radians(fov_deg) / 2))
+
+
+SERVING_SIZES = ((640, 480), (1280, 720), (1920, 1080))
+
+
+def config_serving(seed: int = 4, counts=(100_000, 500_000, 1_500_000, 3_000_000)) -> dict:
+    """Resident scenes for serving: distributions as the 1080p workload (xyz in [-3,3]^3), C = 3."""
+    gen = torch.Generator().manual_seed(seed)
+    out = {}
+    for i, n in enumerate(counts):
+        g = make_gaussians(gen, n, 3.0)
+        g["feature"] = torch.rand(n, 3, generator=gen)
+        out[f"scene{i}_{n}"] = g
+    return out
+
+
+def serving_camera(rng, W: int, H: int):
+    """A random look-at camera at distance 4..5, fx = fy = 1.171875 W (rng: random.Random)."""
+    gen = torch.Generator().manual_seed(rng.randrange(1 << 30))
+    return make_camera(gen, W, H, 4.0 + rng.random(), 1.171875 * W)

@@ -46,6 +46,25 @@ def config_for(W: int, H: int, C: int) -> BlendConfig:
         return _table.get(shape_key(W, H, C), DEFAULT)
 
 
+def views_key(sizes, C: int) -> tuple:
+    return ("views", tuple((int(w), int(h)) for w, h in sizes), int(C))
+
+
+def config_for_views(sizes, C: int) -> BlendConfig:
+    """Launch parameters for one many-view launch (a single view uses its (W, H, C) entry)."""
+    if len(sizes) == 1:
+        return config_for(sizes[0][0], sizes[0][1], C)
+    with _lock:
+        if _override is not None:
+            return _override
+        return _table.get(views_key(sizes, C), DEFAULT)
+
+
+def record_views(sizes, C: int, cfg: BlendConfig) -> None:
+    with _lock:
+        _table[views_key(sizes, C)] = cfg
+
+
 def record(W: int, H: int, C: int, cfg: BlendConfig) -> None:
     with _lock:
         _table[shape_key(W, H, C)] = cfg
