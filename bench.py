@@ -36,7 +36,7 @@ FP32_PEAK_TFLOPS = 157.3
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "rasterize_1080p_pmc_traffic.txt")
 
 # Kernels the library times (msplat_timing_read names).
-KERNELS = ("preprocess_fwd", "preprocess_bwd", "sh_fwd", "sh_bwd", "depth_sort", "emit", "class_sort_pass",
+KERNELS = ("preprocess_fwd", "preprocess_bwd", "sh_fwd", "sh_bwd", "depth_sort", "emit", "class_sort",
            "rasterize_fwd", "rasterize_bwd")
 
 
@@ -85,9 +85,10 @@ def algorithmic_work(name, w):
         "preprocess_bwd": ("hbm", 108 * n),         # + depth, radius, dL/duv, dL/dconic in; 3 grads out
         "sh_fwd": ("hbm", n * (4 * Csh * D + 12 + 4 * Csh)),
         "sh_bwd": ("hbm", n * (8 * Csh * D + 24 + 4 * Csh)),
-        "depth_sort": ("hbm", n * 4 * 20),          # 4 passes x (key read for the histogram, pair read, pair write)
+        # the binned Gaussians only: one key read for all digit histograms, 4 x (pair read + pair write)
+        "depth_sort": ("hbm", w["nvis"] * (4 + 4 * 16)),
         "emit": ("hbm", w["count"] * 8 + w["nvis"] * 28),
-        "class_sort_pass": ("hbm", w["count"] * 20),
+        "class_sort": ("hbm", w["count"] * (4 + 16)),   # histogram read + one pass (cells fit 8 bits)
         "rasterize_fwd": ("flop", w["E"] * (16 + 2 * C)),
         "rasterize_bwd": ("flop", w["E"] * (30 + 6 * C)),
     }

@@ -129,7 +129,7 @@ int msplat_compute_sh_backward(const float *shs, const float *dirs,
 /* ---- kernel timing (no reference counterpart; for benchmarks).  While
  * enabled, the main kernel of each entry point is bracketed by HIP events on
  * the caller's stream.  Kernel names: preprocess_fwd, preprocess_bwd, sh_fwd,
- * sh_bwd, depth_sort, emit, class_sort_pass, blend_fwd, blend_bwd,
+ * sh_bwd, depth_sort, emit, class_sort, blend_fwd, blend_bwd,
  * rasterize_fwd, rasterize_bwd.  msplat_timing_enable returns the previous
  * state; msplat_timing_read waits for the recorded events of one kernel,
  * returns their total milliseconds and launch count, and forgets them. */

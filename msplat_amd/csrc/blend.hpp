@@ -57,6 +57,7 @@ struct BlendIn {
 template <int PPL>
 struct PixBlock {
   int tile, view, tx, ty;  // global tile, its view, tile coordinates in the view
+  int sub;                 // which of the tile's 4 / PPL pixel blocks
   int W, H;
   int64_t pix0;            // first pixel of the view in the concatenated planes
   int64_t vbase;           // first per-view row of the view
@@ -73,6 +74,7 @@ MSPLAT_DEV PixBlock<PPL> pix_block(const BlendIn &in) {
   PixBlock<PPL> p;
   p.tile = w / R;
   const int sub = w % R;
+  p.sub = sub;
   p.active = p.tile < in.ntiles;
   p.view = p.active ? view_of_tile(in.views, p.tile) : 0;
   const int local = p.active ? p.tile - in.views.tile0[p.view] : 0;
@@ -112,6 +114,7 @@ struct BlendStage {
   float4 rec[kBlendLanes * kQuads];
   int32_t id[kBlendLanes];   // Gaussian of each slot
   int32_t pos[kBlendLanes];  // cell lists: list index of each slot
+  uint32_t strips[kBlendLanes];  // strips of the tile each slot may touch
 };
 
 // Feature row of per-view row vg (gshift: Gaussian = vg + gshift).
@@ -168,39 +171,71 @@ MSPLAT_DEV int mask_rank(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
-// Exact tile culling for cell lists: false only if no pixel centre of tile
-// (tx, ty) can reach alpha >= 1/255.  The Gaussian's largest power over the
-// tile's pixel-centre rectangle -- q = a dx^2 + 2 b dx dy + c dy^2 is convex,
-// so off-centre its minimum lies on an edge of the box -- must stay below
-// -ln(255 o) by a margin (0.05 in power, alpha x 0.95) far above the fp32
-// rounding of the blending's own power and exp2; degenerate or non-finite
-// parameters are never culled.  Culled Gaussians change neither colour nor
-// transmittance, so images, final_T and gradients are unchanged; only the
-// list positions counted in ncontrib differ.
-MSPLAT_DEV bool may_touch(const BlendIn &in, int vg, int64_t gshift, int tx, int ty) {
-  const float u = in.uv[2 * int64_t(vg)], v = in.uv[2 * int64_t(vg) + 1];
-  const float a = in.conic[3 * int64_t(vg)], b = in.conic[3 * int64_t(vg) + 1], c = in.conic[3 * int64_t(vg) + 2];
+// Exact culling for cell lists.  A Gaussian is skipped for a pixel box (a
+// tile, or one of its four 16x4 strips) only if no pixel centre in the box can
+// reach alpha >= 1/255: its largest power over the box's pixel-centre
+// rectangle -- q = a dx^2 + 2 b dx dy + c dy^2 is convex, so off-centre its
+// minimum lies on an edge of the box -- must stay below -ln(255 o) by a margin
+// (0.05 in power, alpha x 0.95) far above the fp32 rounding of the blending's
+// own power and exp2; degenerate or non-finite parameters are never culled.
+// Culled Gaussians change neither colour nor transmittance, so images,
+// final_T and gradients are unchanged; only the list positions counted in
+// ncontrib differ.
+struct CullGeom {
+  float u, v, a, b, c, lim;
+  bool regular;  // false: never cull
+};
+
+MSPLAT_DEV CullGeom cull_geom(const BlendIn &in, int vg, int64_t gshift) {
+  CullGeom g;
+  g.u = in.uv[2 * int64_t(vg)];
+  g.v = in.uv[2 * int64_t(vg) + 1];
+  g.a = in.conic[3 * int64_t(vg)];
+  g.b = in.conic[3 * int64_t(vg) + 1];
+  g.c = in.conic[3 * int64_t(vg) + 2];
   const float o = in.opacity[vg + gshift];
-  if (!(a > 0.f && c > 0.f && a * c - b * b > 0.f && o > 0.f && o < 1e30f && fabsf(u) < 1e30f &&
-        fabsf(v) < 1e30f))
-    return true;
-  // d = mean - pixel over the pixel centres p in [16 t, 16 t + 15]
-  const float x0 = u - float(kTile * tx + kTile - 1), x1 = u - float(kTile * tx);
-  const float y0 = v - float(kTile * ty + kTile - 1), y1 = v - float(kTile * ty);
+  g.regular = g.a > 0.f && g.c > 0.f && g.a * g.c - g.b * g.b > 0.f && o > 0.f && o < 1e30f &&
+              fabsf(g.u) < 1e30f && fabsf(g.v) < 1e30f;
+  // alpha < 1/255  <=  power = -q / 2 < -ln(255 o) - 0.05  <=>  q > lim
+  g.lim = g.regular ? 2.f * logf(255.f * o) + 0.1f : 0.f;
+  return g;
+}
+
+// May a pixel centre p in [px0, px1] x [py0, py1] get alpha >= 1/255?
+MSPLAT_DEV bool box_may_touch(const CullGeom &g, float px0, float px1, float py0, float py1) {
+  if (!g.regular) return true;
+  const float x0 = g.u - px1, x1 = g.u - px0;  // d = mean - pixel
+  const float y0 = g.v - py1, y1 = g.v - py0;
   if (x0 <= 0.f && x1 >= 0.f && y0 <= 0.f && y1 >= 0.f) return true;
+  const float a = g.a, b = g.b, c = g.c;
   auto q = [&](float dx, float dy) { return a * dx * dx + 2.f * b * dx * dy + c * dy * dy; };
   auto on_x = [&](float dx) { return q(dx, fminf(fmaxf(-b * dx / c, y0), y1)); };
   auto on_y = [&](float dy) { return q(fminf(fmaxf(-b * dy / a, x0), x1), dy); };
   const float qmin = fminf(fminf(on_x(x0), on_x(x1)), fminf(on_y(y0), on_y(y1)));
-  // alpha < 1/255  <=  power = -q / 2 < -ln(255 o) - 0.05
-  return !(qmin > 2.f * logf(255.f * o) + 0.1f);
+  return !(qmin > g.lim);
 }
 
-// Cell-list filter: the Gaussian's tile rectangle covers the tile (and, with
-// culling, it may touch one of the tile's pixels).
-MSPLAT_DEV bool walk_keeps(const BlendIn &in, int vg, int64_t gshift, int tx, int ty) {
-  if (!rect_covers(reinterpret_cast<const uint2 *>(in.rect)[vg], tx, ty)) return false;
-  return !in.cull || may_touch(in, vg, gshift, tx, ty);
+// Cell-list filter: 0 if the Gaussian is skipped for tile (tx, ty), else the
+// mask of the tile's 16x4 strips (bit s: rows 4s..4s+3) it may touch -- all
+// four without culling.
+MSPLAT_DEV uint32_t walk_mask(const BlendIn &in, int vg, int64_t gshift, int tx, int ty) {
+  if (!rect_covers(reinterpret_cast<const uint2 *>(in.rect)[vg], tx, ty)) return 0u;
+  if (!in.cull) return 0xfu;
+  const CullGeom g = cull_geom(in, vg, gshift);
+  const float px0 = float(kTile * tx), px1 = float(kTile * tx + kTile - 1), py0 = float(kTile * ty);
+  if (!box_may_touch(g, px0, px1, py0, py0 + float(kTile - 1))) return 0u;
+  uint32_t m = 0u;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    if (box_may_touch(g, px0, px1, py0 + float(4 * s), py0 + float(4 * s + 3))) m |= 1u << s;
+  return m;
+}
+
+// Strips (of the mask above) holding pixel slot i of a wave's lanes.
+template <int PPL>
+MSPLAT_DEV uint32_t slot_strips(int sub, int i) {
+  if constexpr (PPL == 1) return 3u << (2 * (sub >> 1));  // an 8x8 quadrant: rows of two strips
+  else return 1u << ((PPL == 2 ? 2 * sub : 0) + i);  // 16x4 strips: rows 4 i (+ 8 sub)
 }
 
 // The list a tile walks: [start, end) of its tile list or of its cell's list.
@@ -263,9 +298,11 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
       const int e = pos + lane;
       int g = 0;
       bool pass = false;
+      uint32_t strips = 0u;
       if (e < end) {
         g = in.list[e];
-        pass = walk_keeps(in, g, pb.gshift, pb.tx, pb.ty);
+        strips = walk_mask(in, g, pb.gshift, pb.tx, pb.ty);
+        pass = strips != 0u;
       }
       const uint64_t m = __ballot(pass);
       cnt = __popcll(m);
@@ -273,11 +310,15 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
         const int slot = mask_rank(m);
         stage_gaussian<CMAX>(s, slot, g, in, cc, pb.gshift);
         s.pos[slot] = e;
+        s.strips[slot] = strips;
       }
       pos += kBlendLanes;
     } else {
       cnt = end - pos < kBlendLanes ? end - pos : kBlendLanes;
-      if (lane < cnt) stage_gaussian<CMAX>(s, lane, in.list[pos + lane], in, cc, pb.gshift);
+      if (lane < cnt) {
+        stage_gaussian<CMAX>(s, lane, in.list[pos + lane], in, cc, pb.gshift);
+        s.strips[lane] = 0xfu;
+      }
       pos += cnt;
     }
     __syncthreads();
@@ -289,8 +330,10 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
       const int contributor = fpos + j + 1;
       int epos = 0;
       if constexpr (kWalk) epos = s.pos[j] + 1;
+      const uint32_t strips = __builtin_amdgcn_readfirstlane(s.strips[j]);
 #pragma unroll
       for (int i = 0; i < PPL; ++i) {
+        if (!(strips & slot_strips<PPL>(pb.sub, i))) continue;  // uniform: culled for these rows
         // Branch-free: every quantity is computed and the reference's
         // early-outs become selects, so each update happens exactly where the
         // reference's would (with the same operations) and nowhere else.
@@ -384,6 +427,9 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
     last_alpha[i] = 0.f;
   }
   wl = wave_max(wl);
+  int wl_slot[PPL];  // per pixel slot: the wave's largest list end
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) wl_slot[i] = wave_max(last[i]);
 
   const int2 lst = tile_list<kWalk>(in, pb);
   const int lo_bound = lst.x;
@@ -404,16 +450,25 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
       const int e = lo + lane;
       int gi = 0;
       bool pass = false;
+      uint32_t strips = 0u;
       if (e < hi) {
         gi = in.list[e];
-        pass = walk_keeps(in, gi, pb.gshift, pb.tx, pb.ty);
+        strips = walk_mask(in, gi, pb.gshift, pb.tx, pb.ty);
+        pass = strips != 0u;
       }
       const uint64_t m = __ballot(pass);
       cnt = __popcll(m);
-      if (pass) stage_gaussian<CMAX>(s, mask_rank(m), gi, in, cc, pb.gshift);
+      if (pass) {
+        const int slot = mask_rank(m);
+        stage_gaussian<CMAX>(s, slot, gi, in, cc, pb.gshift);
+        s.strips[slot] = strips;
+      }
     } else {
       cnt = hi - lo;
-      if (lane < cnt) stage_gaussian<CMAX>(s, lane, in.list[lo + lane], in, cc, pb.gshift);
+      if (lane < cnt) {
+        stage_gaussian<CMAX>(s, lane, in.list[lo + lane], in, cc, pb.gshift);
+        s.strips[lane] = 0xfu;
+      }
     }
     const int fbase = fcount - cnt;  // tile-list position of slot 0
     __syncthreads();
@@ -430,43 +485,46 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
 #pragma unroll
       for (int ch = 0; ch < CMAX; ++ch) vf[ch] = 0.f;
       bool any = false;
+      const uint32_t strips = __builtin_amdgcn_readfirstlane(s.strips[j]);
 #pragma unroll
       for (int i = 0; i < PPL; ++i) {
-        // Branch-free, as in the forward: the reference's skips become
-        // selects; a pixel's state and sums change only where it contributes.
+        if (!(strips & slot_strips<PPL>(pb.sub, i))) continue;  // uniform: culled for these rows
+        if (rel >= wl_slot[i]) continue;                         // uniform: every pixel of the slot ended
+        // The reference's early-outs as branches (measured faster here than
+        // selects: most pixels of a tile share the outcome).
+        if (!(rel < last[i])) continue;
         const float dx = g0.x - float(pb.px[i]), dy = g0.y - float(pb.py[i]);
         const float power = gauss_power(a, b, c, dx, dy);
+        if (power > 0.f) continue;
         const float G = fast_exp(power);
         const float alpha = fminf(0.99f, op * G);
-        const bool hit = rel < last[i] && !(power > 0.f) && !(alpha < 1.f / 255.f);
-        any = any || hit;
+        if (alpha < 1.f / 255.f) continue;
+        any = true;
         // 1 / (1 - alpha) on the hardware reciprocal (the reference divides
         // with fast-math); 1 - alpha >= 0.01, so it is well conditioned.
         const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-        const float Tn = T[i] * inv;
-        T[i] = hit ? Tn : T[i];
+        T[i] = T[i] * inv;
         const float dcol = alpha * T[i];
         float dL_dalpha = 0.f;
 #pragma unroll
         for (int ch = 0; ch < CMAX; ++ch) {  // channels >= cc carry zeros
-          const float acc = last_alpha[i] * last_f[i][ch] + (1.f - last_alpha[i]) * accum[i][ch];
-          accum[i][ch] = hit ? acc : accum[i][ch];
-          last_f[i][ch] = hit ? fe[ch] : last_f[i][ch];
+          accum[i][ch] = last_alpha[i] * last_f[i][ch] + (1.f - last_alpha[i]) * accum[i][ch];
+          last_f[i][ch] = fe[ch];
           dL_dalpha += (fe[ch] - accum[i][ch]) * g[i][ch];
-          vf[ch] += hit ? dcol * g[i][ch] : 0.f;
+          vf[ch] += dcol * g[i][ch];
         }
         dL_dalpha *= T[i];
-        last_alpha[i] = hit ? alpha : last_alpha[i];
+        last_alpha[i] = alpha;
         dL_dalpha += (-T_final[i] * inv) * bg_dot[i];
         const float dL_dG = op * dL_dalpha;
         const float gdx = ((-G) * dx) * a - (G * dy) * b;
         const float gdy = ((-G) * dy) * c - (G * dx) * b;
-        vuv0 += hit ? dL_dG * gdx : 0.f;
-        vuv1 += hit ? dL_dG * gdy : 0.f;
-        vc0 += hit ? (((-0.5f * G) * dx) * dx) * dL_dG : 0.f;
-        vc1 += hit ? (((-G) * dx) * dy) * dL_dG : 0.f;
-        vc2 += hit ? (((-0.5f * G) * dy) * dy) * dL_dG : 0.f;
-        vo += hit ? G * dL_dalpha : 0.f;
+        vuv0 += dL_dG * gdx;
+        vuv1 += dL_dG * gdy;
+        vc0 += (((-0.5f * G) * dx) * dx) * dL_dG;
+        vc1 += (((-G) * dx) * dy) * dL_dG;
+        vc2 += (((-0.5f * G) * dy) * dy) * dL_dG;
+        vo += G * dL_dalpha;
       }
       if (__ballot(any) == 0ull) continue;
       // Sum the 6 + cc per-lane contributions over the wave in one transposed

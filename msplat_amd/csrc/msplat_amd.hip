@@ -124,7 +124,7 @@ struct Carver {
 };
 
 struct PrepLayout {
-  size_t area, ka, va, kb, vb, offs, hist, sums, flag, nlisted, total;
+  size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, total;
 };
 
 // (W, H are part of the size query so that the layout may depend on the
@@ -144,6 +144,8 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
   int64_t ns = sortk::scan_tiles(256 * nt);
   if (sortk::scan_tiles(n) > ns) ns = sortk::scan_tiles(n);
   L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
+  L.ghist = c.take(size_t(sortk::ghist_words()) * 4);
+  L.status = c.take(size_t(sortk::status_words(n)) * 4);
   L.flag = c.take(4);
   L.nlisted = c.take(8);
   (void)W;
@@ -153,7 +155,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
 }
 
 struct EmitLayout {
-  size_t ca, ga, cb, gb, owner, hist, sums, total;
+  size_t ca, ga, cb, gb, owner, hist, sums, ghist, status, total;
 };
 
 EmitLayout emit_layout(int64_t k, int W, int H) {
@@ -171,6 +173,8 @@ EmitLayout emit_layout(int64_t k, int W, int H) {
   L.hist = c.take(size_t(256 * nt) * 4);
   const int64_t ns = sortk::scan_tiles(256 * nt);
   L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
+  L.ghist = c.take(size_t(sortk::ghist_words()) * 4);
+  L.status = c.take(size_t(sortk::status_words(k)) * 4);
   L.total = c.off;
   return L;
 }
@@ -283,13 +287,11 @@ void depth_order(void *ws, const PrepLayout &L, int64_t n, int64_t *d_count, hip
                  const int64_t *dlen = nullptr) {
   uint32_t *ka = at<uint32_t>(ws, L.ka), *va = at<uint32_t>(ws, L.va);
   uint32_t *kb = at<uint32_t>(ws, L.kb), *vb = at<uint32_t>(ws, L.vb);
-  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums)};
+  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
+                               at<uint32_t>(ws, L.status)};
   {
     Timed timed_("depth_sort", s);
-    sortk::radix_pass(ka, va, kb, vb, n, 0, 8, rb, s, dlen);  // a -> b -> a -> b -> a
-    sortk::radix_pass(kb, vb, ka, va, n, 8, 8, rb, s, dlen);
-    sortk::radix_pass(ka, va, kb, vb, n, 16, 8, rb, s, dlen);
-    sortk::radix_pass(kb, vb, ka, va, n, 24, 8, rb, s, dlen);
+    sortk::radix_sort(ka, va, kb, vb, n, 32, 4, rb, s, dlen);  // a -> b -> a -> b -> a
   }
   sortk::exclusive_scan<int32_t, int64_t>(at<int32_t>(ws, L.area), va, n, at<int64_t>(ws, L.sums),
                                           at<int64_t>(ws, L.offs), d_count, s);
@@ -320,24 +322,12 @@ const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t
     sortk::emit_chunk_kernel<<<dim3(unsigned(sortk::emit_chunks(k))), sortk::kThreads, 0, s>>>(
         uv, depth, radius, order, offs, owner, n, k, dlen, vw, S, ca, ga);
   }
-  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums)};
+  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
+                               at<uint32_t>(ws, L.status)};
   const int bits = bit_length(uint32_t(ncls));  // classes 0..ncls
   const int npass = (bits + 7) / 8;
-  const int db = (bits + npass - 1) / npass;
-  uint32_t *sk = ca, *sv = ga;
-  for (int p = 0; p < npass; ++p) {
-    const int shift = p * db;
-    const int pb = bits - shift < db ? bits - shift : db;
-    uint32_t *dk = (sk == ca) ? cb : ca;
-    uint32_t *dv = (p == npass - 1) ? ids : ((sv == ga) ? gb : ga);
-    {
-      Timed timed_("class_sort_pass", s);
-      sortk::radix_pass(sk, sv, dk, dv, k, shift, pb, rb, s, dlen);
-    }
-    sk = dk;
-    sv = dv;
-  }
-  return sk;
+  Timed timed_("class_sort", s);
+  return sortk::radix_sort(ca, ga, cb, gb, k, bits, npass, rb, s, dlen, ids) ? cb : ca;
 }
 
 // Cells of S x S tiles on the tile grid of a W x H image.

@@ -175,12 +175,37 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
 //     and the keys are reordered through LDS;
 //  3. the tile is written out digit run by digit run, so consecutive lanes
 //     store consecutive addresses (coalesced), at offsets from the global scan.
+//
+// Where each (digit, tile) run starts comes either from a scanned per-tile
+// histogram (kOnesweep = false: histogram, scan and scatter are separate
+// launches) or, in one launch per pass (kOnesweep = true), from the digit
+// totals of the whole pass plus a decoupled look-back over the earlier tiles:
+// workgroups take tiles in dispatch order from a ticket counter, publish
+// their per-digit counts, then add up the earlier tiles' published counts
+// until one of them carries its inclusive prefix.  A tile publishes its counts
+// before it looks back, so every wait ends; each wait is also bounded and sets
+// *error if it ever runs out.
+constexpr int kMaxPasses = 4;
+constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kValueMask = (1u << 30) - 1u;
+constexpr int64_t kLookbackSpins = int64_t(1) << 16;
+constexpr int kLookbackWidth = 8;
+
+template <bool kOnesweep>
 __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t cap,
     const int64_t *__restrict__ dlen, int shift, int bits, const uint32_t *__restrict__ offsets, int64_t ntiles,
-    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout) {
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t *__restrict__ ticket,
+    uint32_t *__restrict__ status, uint32_t *__restrict__ error) {
   const int64_t len = dev_len(dlen, cap);
-  if (int64_t(blockIdx.x) * kRadixTile >= len) return;  // whole workgroup: past the data
+  int64_t tile = blockIdx.x;
+  if constexpr (kOnesweep) {
+    __shared__ uint32_t s_ticket;
+    if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
+    __syncthreads();
+    tile = s_ticket;
+  }
+  // whole workgroup past the data (no later tile holds data, so none waits for it)
+  if (tile * kRadixTile >= len) return;
   constexpr int kWaves = kThreads / kWave;
   constexpr int kPerWave = kRadixTile / kWaves;  // 1024 keys, contiguous
   __shared__ uint32_t s_key[kRadixTile];
@@ -193,10 +218,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
   const int ndig = 1 << bits;
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) s_run[w][tid] = 0u;
-  if (tid < ndig) s_gstart[tid] = offsets[int64_t(tid) * ntiles + blockIdx.x];
+  if (!kOnesweep && tid < ndig) s_gstart[tid] = offsets[int64_t(tid) * ntiles + tile];
   __syncthreads();
 
-  const int64_t base = int64_t(blockIdx.x) * kRadixTile + int64_t(wave) * kPerWave;
+  const int64_t base = tile * kRadixTile + int64_t(wave) * kPerWave;
   uint32_t key[kItems], val[kItems], rnk[kItems];
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
@@ -229,6 +254,51 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     for (int w = 0; w < kWaves; ++w) tot += s_run[w][tid];
   }
   int64_t dummy;
+  if constexpr (kOnesweep) {
+    // this tile's first slot per digit = digits below it in the whole pass
+    // (offsets holds the pass's digit totals) + this digit in earlier tiles
+    const uint32_t doff = uint32_t(block_excl_scan(tid < ndig ? int64_t(offsets[tid]) : 0, &dummy));
+    if (tid < ndig) {
+      uint32_t *st = status + tile * 256;
+      uint32_t prefix = 0;
+      if (tile == 0) {
+        __hip_atomic_store(st + tid, kFlagIncl | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(st + tid, kFlagAgg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // Look back kLookbackWidth tiles per round: their words are loaded
+        // together (the loads' latencies overlap), then consumed in order.
+        bool found = false;
+        for (int64_t t = tile - 1; t >= 0 && !found; t -= kLookbackWidth) {
+          uint32_t w[kLookbackWidth];
+#pragma unroll
+          for (int q = 0; q < kLookbackWidth; ++q)
+            w[q] = t - q >= 0 ? __hip_atomic_load(status + (t - q) * 256 + tid, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : kFlagIncl;
+#pragma unroll
+          for (int q = 0; q < kLookbackWidth; ++q) {
+            if (found) break;
+            if (t - q < 0) {  // before the first tile: nothing more to add
+              found = true;
+              break;
+            }
+            int64_t spins = 0;
+            while ((w[q] & ~kValueMask) == 0u && ++spins < kLookbackSpins)
+              w[q] = __hip_atomic_load(status + (t - q) * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((w[q] & ~kValueMask) == 0u) {  // never published: give up, flag it
+              atomicOr(error, 1u);
+              found = true;
+              break;
+            }
+            prefix += w[q] & kValueMask;
+            if ((w[q] & ~kValueMask) == kFlagIncl) found = true;
+          }
+        }
+        __hip_atomic_store(st + tid, kFlagIncl | (prefix + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      s_gstart[tid] = doff + prefix;
+    }
+  }
   const uint32_t lstart = uint32_t(block_excl_scan(tid < ndig ? int64_t(tot) : 0, &dummy));
   if (tid < ndig) {
     s_lstart[tid] = lstart;
@@ -251,7 +321,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     }
   }
   __syncthreads();
-  const int64_t tbase = int64_t(blockIdx.x) * kRadixTile;
+  const int64_t tbase = tile * kRadixTile;
   const int cnt = len - tbase < kRadixTile ? int(len - tbase) : kRadixTile;
   for (int e = tid; e < cnt; e += kThreads) {
     const uint32_t k = s_key[e];
@@ -262,13 +332,50 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
   }
 }
 
+// Workspace of the radix sorts.  hist / sums: the multi-launch pass;
+// ghist (kMaxPasses x 256 digit totals, then kMaxPasses tickets and an error
+// word) and status (kMaxPasses x tiles x 256 look-back words): onesweep.
 struct RadixBuffers {
-  uint32_t *hist;  // 256 * radix_tiles(len)
-  int64_t *sums;   // scan_tiles(256 * radix_tiles(len))
+  uint32_t *hist;    // 256 * radix_tiles(len)
+  int64_t *sums;     // scan_tiles(256 * radix_tiles(len))
+  uint32_t *ghist;   // kMaxPasses * 256 + kMaxPasses + 1
+  uint32_t *status;  // kMaxPasses * 256 * radix_tiles(len)
 };
 
+MSPLAT_HD int64_t ghist_words() { return int64_t(kMaxPasses) * 256 + kMaxPasses + 1; }
+MSPLAT_HD int64_t status_words(int64_t len) { return int64_t(kMaxPasses) * 256 * radix_tiles(len); }
+
+// Digit totals of every pass in one read of the keys (pass p covers bits
+// [p db, min(p db + db, bits))), and the look-back words cleared.  ghist must
+// be zero on entry.
+__global__ __launch_bounds__(kThreads) void onesweep_hist_kernel(const uint32_t *__restrict__ keys, int64_t cap,
+                                                                 const int64_t *__restrict__ dlen, int npass,
+                                                                 int db, int bits, uint32_t *__restrict__ ghist,
+                                                                 uint32_t *__restrict__ status, int64_t nstatus) {
+  __shared__ uint32_t h[kMaxPasses][256];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int p = 0; p < kMaxPasses; ++p) h[p][tid] = 0u;
+  __syncthreads();
+  const int64_t len = dev_len(dlen, cap);
+  const int64_t stride = int64_t(gridDim.x) * kThreads;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + tid; i < len; i += stride) {
+    const uint32_t k = keys[i];
+    for (int p = 0; p < npass; ++p) {
+      const int sh = p * db;
+      const int pb = bits - sh < db ? bits - sh : db;
+      atomicAdd(&h[p][(k >> sh) & ((1u << pb) - 1u)], 1u);
+    }
+  }
+  __syncthreads();
+  for (int p = 0; p < npass; ++p)
+    if (h[p][tid]) atomicAdd(&ghist[p * 256 + tid], h[p][tid]);
+  for (int64_t j = int64_t(blockIdx.x) * kThreads + tid; j < nstatus; j += stride) status[j] = 0u;
+}
+
 // One stable pass over bits [shift, shift + bits) of the first `cap` pairs,
-// or of the first *dlen (<= cap) when dlen is given.
+// or of the first *dlen (<= cap) when dlen is given: histogram, scan and
+// scatter launches.
 inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t cap,
                        int shift, int bits, const RadixBuffers &rb, hipStream_t s,
                        const int64_t *dlen = nullptr) {
@@ -277,8 +384,49 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
   radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, cap, dlen, shift, bits, rb.hist, nt);
   const int64_t hlen = int64_t(1 << bits) * nt;
   exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
-  radix_scatter_tile_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, vin, cap, dlen, shift, bits,
-                                                                           rb.hist, nt, kout, vout);
+  radix_scatter_tile_kernel<false><<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(
+      kin, vin, cap, dlen, shift, bits, rb.hist, nt, kout, vout, nullptr, nullptr, nullptr);
+}
+
+// Stable LSD sort of `cap` (or *dlen) pairs over `bits` key bits in npass
+// passes of db bits (db <= 8, npass <= kMaxPasses), ping-ponging a -> b -> a
+// ...; returns which buffer holds the result (0: a, 1: b).  Onesweep needs
+// cap < 2^30 (the look-back words carry 30-bit counts); larger inputs use the
+// multi-launch passes.  vlast, when given, receives the values of the last
+// pass instead of the other buffer.
+inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, int64_t cap, int bits, int npass,
+                      const RadixBuffers &rb, hipStream_t s, const int64_t *dlen = nullptr,
+                      uint32_t *vlast = nullptr) {
+  const int db = (bits + npass - 1) / npass;
+  const int64_t nt = radix_tiles(cap);
+  const bool onesweep = cap < (int64_t(1) << 30) && npass <= kMaxPasses;
+  if (nt > 0 && onesweep) {
+    launch_fill_zero(rb.ghist, ghist_words(), s);
+    const int64_t nst = int64_t(npass) * 256 * nt;
+    const unsigned hb = unsigned(imin(int(nt), 1024));
+    onesweep_hist_kernel<<<dim3(hb), dim3(kThreads), 0, s>>>(ka, cap, dlen, npass, db, bits, rb.ghist, rb.status,
+                                                            nst);
+  }
+  uint32_t *sk = ka, *sv = va;
+  int in_b = 0;
+  for (int p = 0; p < npass; ++p) {
+    const int shift = p * db;
+    const int pb = bits - shift < db ? bits - shift : db;
+    uint32_t *dk = in_b ? ka : kb;
+    uint32_t *dv = (p == npass - 1 && vlast) ? vlast : (in_b ? va : vb);
+    if (nt > 0) {
+      if (onesweep)
+        radix_scatter_tile_kernel<true><<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(
+            sk, sv, cap, dlen, shift, pb, rb.ghist + p * 256, nt, dk, dv, rb.ghist + kMaxPasses * 256 + p,
+            rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses);
+      else
+        radix_pass(sk, sv, dk, dv, cap, shift, pb, rb, s, dlen);
+    }
+    sk = dk;
+    sv = dv;
+    in_b ^= 1;
+  }
+  return in_b;
 }
 
 // ---- sort-specific kernels ---------------------------------------------------
