@@ -231,6 +231,69 @@ def bench_multiview(dev, steps: int = 5, warmup: int = 2) -> dict:
                 speedup_batched=round(t_loop / t_batch, 3), graph_capture="disabled (see msplat_amd/graphs.py)")
 
 
+def bench_batched_1080p(dev, views: int = 8, steps: int = 3, warmup: int = 1) -> dict:
+    """N = 1M Gaussians (the 1080p scene), B cameras at 1920x1080 and distance 4, C = 3 U(0,1) features,
+    fwd + bwd (L2) with gradients for the Gaussians, three ways: (a) the plain per-view path, msplat's
+    stage chain project_point -> compute_cov3d -> ewa_project -> sort_gaussian -> alpha_blending
+    (msplat/__init__.py:71-91) once per view; (b) this library's single-view rasterization once per
+    view; (c) one many-view launch per stage.  Also the max-abs image difference of (c) against (a)."""
+    import torch
+
+    import msplat_amd as ms
+    from msplat_amd import scenes, views as mv
+
+    cfg = scenes.config_sh3_1080p()
+    g = cfg["gaussians"]
+    gen = torch.Generator().manual_seed(7)
+    cams = [scenes.make_camera(gen, 1920, 1080, 4.0, 1.171875 * 1920).to(dev) for _ in range(views)]
+    feat = torch.rand(g["xyz"].shape[0], 3, generator=gen)
+    P = {k: v.to(dev).contiguous().requires_grad_() for k, v in
+         dict(xyz=g["xyz"], scale=g["scale"], rotate=g["rotate"], opacity=g["opacity"], feature=feat).items()}
+    args = [P[k] for k in ("xyz", "scale", "rotate", "opacity", "feature")]
+    intrs = torch.stack([c.intr for c in cams])
+    extrs = torch.stack([c.extr for c in cams])
+    sizes = [(c.W, c.H) for c in cams]
+
+    def plain_view(c):
+        xyz, scale, rotate, opacity, feature = args
+        uv, depth = ms.project_point(xyz, c.intr, c.extr, c.W, c.H)
+        visible = depth != 0
+        cov3d = ms.compute_cov3d(scale, rotate, visible)
+        conic, radius, tiles = ms.ewa_project(xyz, cov3d, c.intr, c.extr, uv, c.W, c.H, visible)
+        ids, tile_range = ms.sort_gaussian(uv, depth, c.W, c.H, radius, tiles)
+        return ms.alpha_blending(uv, conic, opacity, feature, ids, tile_range, 0.0, c.W, c.H)
+
+    def plain_step():
+        loss = sum((im * im).sum() for im in (plain_view(c) for c in cams))
+        return torch.autograd.grad(loss, args)
+
+    def loop_step():
+        loss = sum((im * im).sum() for im in (ms.rasterization(*args, c.intr, c.extr, c.W, c.H, 0.0) for c in cams))
+        return torch.autograd.grad(loss, args)
+
+    def batched_step():
+        loss = sum((im * im).sum() for im in mv.rasterization_views(*args, intrs, extrs, sizes, 0.0))
+        return torch.autograd.grad(loss, args)
+
+    with torch.no_grad():
+        a = [plain_view(c) for c in cams]
+        b = mv.rasterization_views(*args, intrs, extrs, sizes, 0.0)
+        diff = max(float((x - y).abs().max()) for x, y in zip(a, b))
+        del a, b
+    pix = 1920 * 1080 * views
+    t_plain = _time_steps(plain_step, steps, warmup)
+    t_loop = _time_steps(loop_step, steps, warmup)
+    t_batch = _time_steps(batched_step, steps, warmup)
+    log(f"1080p x{views}: plain per-view {t_plain:.2f} ms, single-view loop {t_loop:.2f} ms, "
+        f"batched {t_batch:.2f} ms per step; max |batched - plain| = {diff:.3g}")
+    return dict(workload=f"N=1M, {views} views 1920x1080, C=3, fwd+bwd L2, grads to Gaussians",
+                pixels_per_step=pix, plain_per_view_ms=round(t_plain, 3), single_view_loop_ms=round(t_loop, 3),
+                batched_ms=round(t_batch, 3), batched_mpix_s=round(pix / t_batch / 1e3, 2),
+                plain_per_view_mpix_s=round(pix / t_plain / 1e3, 2),
+                speedup_vs_plain=round(t_plain / t_batch, 3), speedup_vs_loop=round(t_loop / t_batch, 3),
+                max_abs_diff_vs_plain=diff)
+
+
 def bench_serving(dev, rate: float, duration: float) -> dict:
     """4 resident scenes (100k, 500k, 1.5M, 3M Gaussians, C = 3), Poisson requests for a uniformly chosen
     scene with a random camera at 640x480, 1280x720 or 1920x1080, forward only."""
@@ -358,6 +421,7 @@ def main():
         roof_hbm = roofline(hd, kern[hd], w)
 
     multiview = None if args.no_multiview else bench_multiview(dev)
+    batched = None if args.no_multiview else bench_batched_1080p(dev)
     serving_stats = None if args.no_serving else bench_serving(dev, args.serve_rate, args.serve_seconds)
 
     cpu = None
@@ -374,7 +438,8 @@ def main():
                             cell_entries=w["count"], blend=dict(cell=cfg_used.cell, ppl_fwd=cfg_used.ppl_fwd,
                                                                 ppl_bwd=cfg_used.ppl_bwd),
                             parallelism="single GPU"),
-                roofline=roof, roofline_hbm=roof_hbm, cpu_baseline=cpu, multiview=multiview, serving=serving_stats,
+                roofline=roof, roofline_hbm=roof_hbm, cpu_baseline=cpu, multiview=multiview, batched_1080p=batched,
+                serving=serving_stats,
                 kernels_ms={k: round(v["avg_ms"], 4) for k, v in kern.items()})
     print(json.dumps(line), flush=True)
 

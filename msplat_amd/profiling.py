@@ -58,6 +58,9 @@ def active() -> "StageTimer | None":
 #
 #   stats DIR                 kernel-trace statistics of a `rocprofv3 --kernel-trace
 #                             --stats --output-format csv -d DIR` run
+#   steps DIR PATTERN         per step (a step ends with a kernel whose name matches
+#                             PATTERN): wall time on the device, busy time (sum of kernel
+#                             durations), idle share and launches
 #   pmc FETCH_DIR WRITE_DIR MIB
 #                             HBM bytes per launch of the library's kernels from two
 #                             `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
@@ -75,11 +78,15 @@ KERNEL_PATTERNS = {
     "rasterize_fwd": r"blend_fwd_kernel<\d+, \d+, true>",
     "blend_bwd": r"blend_bwd_kernel<\d+, \d+, false>",
     "blend_fwd": r"blend_fwd_kernel<\d+, \d+, false>",
-    "sh_fwd": r"sh_fwd_(lds_)?kernel",
-    "sh_bwd": r"sh_bwd_(lds_)?kernel",
-    "preprocess_fwd": r"preprocess_fwd_kernel",
-    "preprocess_bwd": r"preprocess_bwd_kernel",
-    "emit": r"emit_chunk_kernel",
+    "sh_fwd": r"sh_fwd_\w*kernel",
+    "sh_bwd": r"sh_bwd_\w*kernel",
+    "preprocess_fwd": r"preprocess_fwd_\w*kernel",
+    "preprocess_bwd": r"preprocess_bwd_\w*kernel",
+    "bin_prepare": r"bin_prepare_kernel",
+    "emit": r"emit_(chunk|direct)_kernel",
+    # both radix sorts (depth order and cell class) launch the same kernels
+    "radix_scatter": r"radix_scatter_tile_kernel",
+    "radix_hist": r"(onesweep|radix)_hist_kernel",
 }
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
@@ -99,6 +106,18 @@ def summarize_stats(d, top=25):
         name = _re.sub(r"\(.*", "", r["Name"])[:72]
         lines.append(f"{name:72s} {r['Calls']:>6s} {float(r['AverageNs']) / 1e3:10.1f} "
                      f"{float(r['TotalDurationNs']) / 1e6:10.3f} {100 * float(r['TotalDurationNs']) / total:5.1f}%")
+    return "\n".join(lines)
+
+
+def summarize_steps(d, pattern, last=5):
+    rows = sorted(_rows(d, "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"]))
+    ends = [i for i, r in enumerate(rows) if _re.search(pattern, r["Kernel_Name"])]
+    lines = [f"{'step':>4s} {'wall_us':>9s} {'busy_us':>9s} {'idle':>6s} {'launches':>8s}"]
+    for j, (a, b) in enumerate(list(zip(ends[:-1], ends[1:]))[-last:]):
+        seg = rows[a + 1:b + 1]
+        wall = int(rows[b]["End_Timestamp"]) - int(rows[a]["End_Timestamp"])
+        busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in seg)
+        lines.append(f"{j:4d} {wall / 1e3:9.1f} {busy / 1e3:9.1f} {100 * (1 - busy / wall):5.1f}% {len(seg):8d}")
     return "\n".join(lines)
 
 
@@ -144,6 +163,8 @@ def summarize_pmc(fetch_dir, write_dir, mib):
 if __name__ == "__main__":
     if len(_sys.argv) >= 3 and _sys.argv[1] == "stats":
         print(summarize_stats(_sys.argv[2]))
+    elif len(_sys.argv) >= 4 and _sys.argv[1] == "steps":
+        print(summarize_steps(_sys.argv[2], _sys.argv[3]))
     elif len(_sys.argv) >= 5 and _sys.argv[1] == "pmc":
         print(summarize_pmc(_sys.argv[2], _sys.argv[3], int(_sys.argv[4])))
     else:
