@@ -2,8 +2,10 @@
 
 Workload (the headline configuration): N = 1,000,000 Gaussians in
 [-3,3]^3, one 1920x1080 view (fx = fy = 1.171875 W, camera at distance 4),
-features = compute_sh(degree 3, [N,3,16] ~ N(0,0.3)) concatenated with the
-projected depth (C = 4), L2 loss against a U(0,1) target, backward to every
+features = SH degree 3 ([N,3,16] ~ N(0,0.3)) of the view directions
+(compute_sh_points: direction and SH in one kernel; --unfused-sh computes the
+directions with torch and calls compute_sh) concatenated with the projected
+depth (C = 4), L2 loss against a U(0,1) target, backward to every
 Gaussian parameter.  A step is one forward + backward of that view; inputs
 are resident in HBM before the timed region.  The blending launch parameters
 are chosen by msplat_amd.autotune before the warm-up (--no-autotune keeps
@@ -44,7 +46,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_step(cfg, dev):
+def build_step(cfg, dev, fused_sh: bool = True):
     import torch
 
     import msplat_amd as ms
@@ -59,7 +61,10 @@ def build_step(cfg, dev):
     def step():
         xyz = P["xyz"]
         _, depth = ms.project_point(xyz, cam.intr, cam.extr, W, H)
-        rgb = ms.compute_sh(P["shs"], scenes.view_dirs(xyz, cam.eye))
+        if fused_sh:  # view direction + SH in one kernel
+            rgb = ms.compute_sh_points(P["shs"], xyz, cam.eye)
+        else:  # the user-level composition: torch view directions, then compute_sh
+            rgb = ms.compute_sh(P["shs"], scenes.view_dirs(xyz, cam.eye))
         feats = torch.cat([rgb, depth], 1)
         out = ms.rasterization(xyz, P["scale"], P["rotate"], P["opacity"], feats, cam.intr, cam.extr, W, H, 0.0)
         loss = ((out - target) ** 2).sum()
@@ -329,6 +334,8 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a PMC calibration kernel)")
     ap.add_argument("--no-multiview", action="store_true")
+    ap.add_argument("--unfused-sh", action="store_true",
+                    help="view directions in torch + compute_sh instead of compute_sh_points")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -346,7 +353,7 @@ def main():
 
     dev = torch.device("cuda:0")
     cfg = scenes.config_sh3_1080p(n=args.n)
-    step, P, cam = build_step(cfg, dev)
+    step, P, cam = build_step(cfg, dev, fused_sh=not args.unfused_sh)
     W, H = cam.W, cam.H
     C = 4
     log(f"workload: N={args.n} {W}x{H} C={C} (SH3 RGB + depth); warmup {args.warmup}, steps {args.steps}")
@@ -437,6 +444,8 @@ def main():
                             gaussians=args.n, width=W, height=H, channels=C, intersections=K,
                             cell_entries=w["count"], blend=dict(cell=cfg_used.cell, ppl_fwd=cfg_used.ppl_fwd,
                                                                 ppl_bwd=cfg_used.ppl_bwd),
+                            sh="torch view directions + compute_sh" if args.unfused_sh else
+                            "compute_sh_points (view direction fused into the SH kernel)",
                             parallelism="single GPU"),
                 roofline=roof, roofline_hbm=roof_hbm, cpu_baseline=cpu, multiview=multiview, batched_1080p=batched,
                 serving=serving_stats,

@@ -126,6 +126,24 @@ int msplat_compute_sh_backward(const float *shs, const float *dirs,
                                int64_t n, int C, int D, float *dL_dshs,
                                float *dL_ddirs, void *stream);
 
+/* SH of the view direction from one camera centre, fused: value =
+ * compute_sh(shs, (xyz - campos) / |xyz - campos|), with |v| =
+ * sqrtf((vx vx + vy vy) + vz vz).  No counterpart in the reference, whose
+ * compute_sh takes the directions (compute_sh.py:8-33) and leaves their
+ * computation to user code; this removes that torch code and its adjoint.
+ * campos is [3] in device memory.  The adjoint writes dL/dshs and dL/dxyz
+ * (through the normalisation); dL/dcampos = -sum_i dL/dxyz_i. */
+int msplat_compute_sh_points_forward(const float *shs, const float *xyz,
+                                     const float *campos,
+                                     const uint8_t *visible, int64_t n,
+                                     int C, int D, float *value, void *stream);
+int msplat_compute_sh_points_backward(const float *shs, const float *xyz,
+                                      const float *campos,
+                                      const uint8_t *visible,
+                                      const float *dL_dvalue, int64_t n, int C,
+                                      int D, float *dL_dshs, float *dL_dxyz,
+                                      void *stream);
+
 /* ---- kernel timing (no reference counterpart; for benchmarks).  While
  * enabled, the main kernel of each entry point is bracketed by HIP events on
  * the caller's stream.  Kernel names: preprocess_fwd, preprocess_bwd, sh_fwd,

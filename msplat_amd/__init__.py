@@ -9,7 +9,7 @@ PyTorch fallback.
 
 from .alpha_blending import alpha_blending
 from .compute_cov3d import compute_cov3d
-from .compute_sh import compute_sh
+from .compute_sh import compute_sh, compute_sh_points
 from .ewa_project import ewa_project
 from .project_point import project_point
 from .rasterization import rasterization
@@ -23,6 +23,8 @@ __all__ = [
     "compute_sh",
     "alpha_blending",
     "rasterization",
+    # extension: view direction + SH fused (no reference counterpart)
+    "compute_sh_points",
 ]
 
 __version__ = "0.1.0"

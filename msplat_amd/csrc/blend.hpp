@@ -183,6 +183,7 @@ MSPLAT_DEV int mask_rank(uint64_t m) {
 // ncontrib differ.
 struct CullGeom {
   float u, v, a, b, c, lim;
+  float ba, bc;  // b / a, b / c: where q is least along a vertical / horizontal edge
   bool regular;  // false: never cull
 };
 
@@ -198,6 +199,8 @@ MSPLAT_DEV CullGeom cull_geom(const BlendIn &in, int vg, int64_t gshift) {
               fabsf(g.u) < 1e30f && fabsf(g.v) < 1e30f;
   // alpha < 1/255  <=  power = -q / 2 < -ln(255 o) - 0.05  <=>  q > lim
   g.lim = g.regular ? 2.f * logf(255.f * o) + 0.1f : 0.f;
+  g.ba = g.regular ? g.b / g.a : 0.f;
+  g.bc = g.regular ? g.b / g.c : 0.f;
   return g;
 }
 
@@ -209,8 +212,11 @@ MSPLAT_DEV bool box_may_touch(const CullGeom &g, float px0, float px1, float py0
   if (x0 <= 0.f && x1 >= 0.f && y0 <= 0.f && y1 >= 0.f) return true;
   const float a = g.a, b = g.b, c = g.c;
   auto q = [&](float dx, float dy) { return a * dx * dx + 2.f * b * dx * dy + c * dy * dy; };
-  auto on_x = [&](float dx) { return q(dx, fminf(fmaxf(-b * dx / c, y0), y1)); };
-  auto on_y = [&](float dy) { return q(fminf(fmaxf(-b * dy / a, x0), x1), dy); };
+  // the least q on an edge: at the clamped stationary point (the divisions are
+  // hoisted into cull_geom; a rounding difference in where q is evaluated
+  // changes it only to second order, far inside the margin)
+  auto on_x = [&](float dx) { return q(dx, fminf(fmaxf(-g.bc * dx, y0), y1)); };
+  auto on_y = [&](float dy) { return q(fminf(fmaxf(-g.ba * dy, x0), x1), dy); };
   const float qmin = fminf(fminf(on_x(x0), on_x(x1)), fminf(on_y(y0), on_y(y1)));
   return !(qmin > g.lim);
 }
@@ -405,26 +411,33 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   const int64_t hw = int64_t(pb.H) * pb.W;
   const float *vdout = dout + int64_t(in.C) * pb.pix0;  // this view's [C, H, W]
 
-  float T[PPL], T_final[PPL], bg_dot[PPL], last_alpha[PPL];
+  // Per pixel: T (recovered back to front), tb = -T_final (bg . dL/dpixel)
+  // for the background term, the channels' dL/dpixel and the colour
+  // accumulated behind the current Gaussian.  The reference updates that
+  // accumulation lazily at the next contributor from the previous one's
+  // alpha and colour (alpha_blending.cu:213-214); updating it right after
+  // each contributor performs the same operations on the same values and
+  // keeps no copy of the previous colour.
+  float T[PPL], tb[PPL];
   int last[PPL];
-  float g[PPL][CMAX], accum[PPL][CMAX], last_f[PPL][CMAX];
+  float g[PPL][CMAX], accum[PPL][CMAX];
   int wl = 0;
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
     const bool inside = pb.px[i] < pb.W && pb.py[i] < pb.H;
     const int64_t pix = int64_t(pb.py[i]) * pb.W + pb.px[i];
-    T_final[i] = inside ? final_T[pb.pix0 + pix] : 0.f;
+    const float T_final = inside ? final_T[pb.pix0 + pix] : 0.f;
     last[i] = inside ? ncontrib[pb.pix0 + pix] : 0;
     wl = imax(wl, last[i]);
-    bg_dot[i] = 0.f;
+    float bg_dot = 0.f;
 #pragma unroll
     for (int ch = 0; ch < CMAX; ++ch) {
       g[i][ch] = (inside && ch < cc) ? vdout[(in.c0 + ch) * hw + pix] : 0.f;
-      if (ch < cc) bg_dot[i] += in.bg * g[i][ch];
-      accum[i][ch] = last_f[i][ch] = 0.f;
+      if (ch < cc) bg_dot += in.bg * g[i][ch];
+      accum[i][ch] = 0.f;
     }
-    T[i] = T_final[i];
-    last_alpha[i] = 0.f;
+    T[i] = T_final;
+    tb[i] = -T_final * bg_dot;
   }
   wl = wave_max(wl);
   int wl_slot[PPL];  // per pixel slot: the wave's largest list end
@@ -508,14 +521,12 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         float dL_dalpha = 0.f;
 #pragma unroll
         for (int ch = 0; ch < CMAX; ++ch) {  // channels >= cc carry zeros
-          accum[i][ch] = last_alpha[i] * last_f[i][ch] + (1.f - last_alpha[i]) * accum[i][ch];
-          last_f[i][ch] = fe[ch];
           dL_dalpha += (fe[ch] - accum[i][ch]) * g[i][ch];
           vf[ch] += dcol * g[i][ch];
+          accum[i][ch] = alpha * fe[ch] + (1.f - alpha) * accum[i][ch];
         }
         dL_dalpha *= T[i];
-        last_alpha[i] = alpha;
-        dL_dalpha += (-T_final[i] * inv) * bg_dot[i];
+        dL_dalpha += tb[i] * inv;
         const float dL_dG = op * dL_dalpha;
         const float gdx = ((-G) * dx) * a - (G * dy) * b;
         const float gdy = ((-G) * dy) * c - (G * dx) * b;

@@ -229,39 +229,39 @@ ShGeom sh_geom(int C, int D) {
   return g;
 }
 
-template <int L>
-void sh_fwd_launch(int64_t n, int C, const float *shs, const float *dirs, const uint8_t *vis, float *val,
+template <int L, class Dir>
+void sh_fwd_launch(int64_t n, int C, const float *shs, const Dir &dir, const uint8_t *vis, float *val,
                    hipStream_t s) {
   const ShGeom g = sh_geom(C, (L + 1) * (L + 1));
   Timed timed_("sh_fwd", s);
   if constexpr (L <= kShGcMaxDeg) {
     if (C <= kWave && aligned16(shs)) {
-      sh_fwd_gc_kernel<L><<<grid_1d(n * C, 256), 256, 0, s>>>(n, C, shs, dirs, vis, val);
+      sh_fwd_gc_kernel<L, Dir><<<grid_1d(n * C, 256), 256, 0, s>>>(n, C, shs, dir, vis, val);
       return;
     }
   }
   if (g.G)
-    sh_fwd_lds_kernel<L><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dirs, vis, val, g);
+    sh_fwd_lds_kernel<L, Dir><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dir, vis, val, g);
   else
-    sh_fwd_kernel<L><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dirs, vis, val);
+    sh_fwd_kernel<L, Dir><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dir, vis, val);
 }
-template <int L>
-void sh_bwd_launch(int64_t n, int C, const float *shs, const float *dirs, const uint8_t *vis, const float *dval,
-                   float *dshs, float *ddirs, hipStream_t s) {
+template <int L, class Dir>
+void sh_bwd_launch(int64_t n, int C, const float *shs, const Dir &dir, const uint8_t *vis, const float *dval,
+                   float *dshs, hipStream_t s) {
   const ShGeom g = sh_geom(C, (L + 1) * (L + 1));
   Timed timed_("sh_bwd", s);
   if constexpr (L <= kShGcMaxDeg) {
     if (C <= kWave && aligned16(shs) && aligned16(dshs)) {
       const int64_t waves = (n + kWave / C - 1) / (kWave / C);
-      sh_bwd_gc_kernel<L><<<grid_1d(waves * kWave, 256), 256, 0, s>>>(n, C, shs, dirs, vis, dval, dshs, ddirs);
+      sh_bwd_gc_kernel<L, Dir><<<grid_1d(waves * kWave, 256), 256, 0, s>>>(n, C, shs, dir, vis, dval, dshs);
       return;
     }
   }
   if (g.G)
-    sh_bwd_lds_kernel<L><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dirs, vis, dval, dshs,
-                                                                              ddirs, g);
+    sh_bwd_lds_kernel<L, Dir><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dir, vis, dval,
+                                                                                   dshs, g);
   else
-    sh_bwd_kernel<L><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dirs, vis, dval, dshs, ddirs);
+    sh_bwd_kernel<L, Dir><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dir, vis, dval, dshs);
 }
 
 #define MSPLAT_SH_DISPATCH(L, CALL) \
@@ -471,11 +471,8 @@ template <bool kWalk>
 void blend_backward(BlendIn in, int ppl, int64_t m, int64_t n, int64_t nf, const float *final_T,
                     const int32_t *ncontrib, const int32_t *walk, const float *dout, float *duv, float *dconic,
                     float *dopac, float *dfeat, hipStream_t s) {
-  launch_fill_zero(duv, 2 * m, s);
-  launch_fill_zero(dconic, 3 * m, s);
-  launch_fill_zero(dopac, n, s);
+  launch_fill_zero_words({{duv, 2 * m}, {dconic, 3 * m}, {dopac, n}, {in.C ? dfeat : nullptr, nf * in.C}}, s);
   if (in.C == 0) return;  // no channel, no gradient
-  launch_fill_zero(dfeat, nf * in.C, s);
   for (int c0 = 0; c0 < in.C; c0 += 64) {
     in.c0 = c0;
     const int rem = in.C - c0 < 64 ? in.C - c0 : 64;
@@ -563,8 +560,7 @@ int msplat_project_point_backward(const float *xyz, int64_t n, const float *intr
   REQUIRE(n >= 0, "project_point_backward: n = %lld", (long long)n);
   hipStream_t s = as_stream(stream);
   const bool cam = dL_dintr || dL_dextr;
-  if (dL_dintr) launch_fill_zero(dL_dintr, 4, s);
-  if (dL_dextr) launch_fill_zero(dL_dextr, 12, s);
+  launch_fill_zero_words({{dL_dintr, 4}, {dL_dextr, 12}}, s);
   if (n == 0) return launched("project_point_backward");
   REQUIRE(xyz && intr && extr && depth && dL_duv && dL_dxyz, "project_point_backward: null pointer");
   if (cam)
@@ -619,8 +615,7 @@ int msplat_ewa_project_backward(const float *xyz, const float *cov3d, const floa
   REQUIRE(n >= 0, "ewa_project_backward: n = %lld", (long long)n);
   hipStream_t s = as_stream(stream);
   const bool cam = dL_dintr || dL_dextr;
-  if (dL_dintr) launch_fill_zero(dL_dintr, 4, s);
-  if (dL_dextr) launch_fill_zero(dL_dextr, 12, s);
+  launch_fill_zero_words({{dL_dintr, 4}, {dL_dextr, 12}}, s);
   if (n == 0) return launched("ewa_project_backward");
   REQUIRE(xyz && cov3d && intr && extr && radius && dL_dconic && dL_dxyz && dL_dcov3d,
           "ewa_project_backward: null pointer");
@@ -659,8 +654,7 @@ int msplat_preprocess_backward(const float *xyz, const float *scale, const float
   REQUIRE(n >= 0, "preprocess_backward: n = %lld", (long long)n);
   hipStream_t s = as_stream(stream);
   const bool cam = dL_dintr || dL_dextr;
-  if (dL_dintr) launch_fill_zero(dL_dintr, 4, s);
-  if (dL_dextr) launch_fill_zero(dL_dextr, 12, s);
+  launch_fill_zero_words({{dL_dintr, 4}, {dL_dextr, 12}}, s);
   if (n == 0) return launched("preprocess_backward");
   REQUIRE(xyz && scale && rotate && intr && extr && depth && radius && dL_duv && dL_dconic && dL_dxyz &&
               dL_dscale && dL_drotate,
@@ -687,7 +681,7 @@ int msplat_compute_sh_forward(const float *shs, const float *dirs, const uint8_t
   if (n == 0 || C == 0) return MSPLAT_OK;
   REQUIRE(shs && dirs && value, "compute_sh_forward: null pointer");
   hipStream_t s = as_stream(stream);
-#define CALL(LL) sh_fwd_launch<LL>(n, C, shs, dirs, visible, value, s)
+#define CALL(LL) sh_fwd_launch<LL>(n, C, shs, ShDirArray{dirs, nullptr}, visible, value, s)
   MSPLAT_SH_DISPATCH(L, CALL)
 #undef CALL
   return launched("compute_sh_forward");
@@ -706,10 +700,45 @@ int msplat_compute_sh_backward(const float *shs, const float *dirs, const uint8_
     return launched("compute_sh_backward");
   }
   REQUIRE(shs && dirs && dL_dvalue && dL_dshs && dL_ddirs, "compute_sh_backward: null pointer");
-#define CALL(LL) sh_bwd_launch<LL>(n, C, shs, dirs, visible, dL_dvalue, dL_dshs, dL_ddirs, s)
+#define CALL(LL) sh_bwd_launch<LL>(n, C, shs, ShDirArray{dirs, dL_ddirs}, visible, dL_dvalue, dL_dshs, s)
   MSPLAT_SH_DISPATCH(L, CALL)
 #undef CALL
   return launched("compute_sh_backward");
+}
+
+int msplat_compute_sh_points_forward(const float *shs, const float *xyz, const float *campos, const uint8_t *visible,
+                                     int64_t n, int C, int D, float *value, void *stream) {
+  REQUIRE(n >= 0 && C >= 0, "compute_sh_points_forward: n = %lld, C = %d", (long long)n, C);
+  const int L = sh_degree(D);
+  REQUIRE(L >= 0, "compute_sh_points_forward: D = %d is not (deg+1)^2 with deg <= 10", D);
+  if (n == 0 || C == 0) return MSPLAT_OK;
+  REQUIRE(shs && xyz && campos && value, "compute_sh_points_forward: null pointer");
+  hipStream_t s = as_stream(stream);
+  const ShDirFromPoints dir{xyz, campos, nullptr};
+#define CALL(LL) sh_fwd_launch<LL>(n, C, shs, dir, visible, value, s)
+  MSPLAT_SH_DISPATCH(L, CALL)
+#undef CALL
+  return launched("compute_sh_points_forward");
+}
+
+int msplat_compute_sh_points_backward(const float *shs, const float *xyz, const float *campos,
+                                      const uint8_t *visible, const float *dL_dvalue, int64_t n, int C, int D,
+                                      float *dL_dshs, float *dL_dxyz, void *stream) {
+  REQUIRE(n >= 0 && C >= 0, "compute_sh_points_backward: n = %lld, C = %d", (long long)n, C);
+  const int L = sh_degree(D);
+  REQUIRE(L >= 0, "compute_sh_points_backward: D = %d is not (deg+1)^2 with deg <= 10", D);
+  if (n == 0) return MSPLAT_OK;
+  hipStream_t s = as_stream(stream);
+  if (C == 0) {
+    launch_fill_zero(dL_dxyz, 3 * n, s);
+    return launched("compute_sh_points_backward");
+  }
+  REQUIRE(shs && xyz && campos && dL_dvalue && dL_dshs && dL_dxyz, "compute_sh_points_backward: null pointer");
+  const ShDirFromPoints dir{xyz, campos, dL_dxyz};
+#define CALL(LL) sh_bwd_launch<LL>(n, C, shs, dir, visible, dL_dvalue, dL_dshs, s)
+  MSPLAT_SH_DISPATCH(L, CALL)
+#undef CALL
+  return launched("compute_sh_points_backward");
 }
 
 // ---------------------------------------------------------------- sort
@@ -880,8 +909,7 @@ int msplat_gaussian_keys(const float *uv, const float *depth, const int32_t *rad
   hipStream_t s = as_stream(stream);
   if (k > 0) {
     REQUIRE(keys && idx, "gaussian_keys: null output");
-    launch_fill_zero(keys, k, s);
-    launch_fill_zero(idx, k, s);
+    launch_fill_zero_words({{keys, 2 * k}, {idx, k}}, s);
   }
   if (n == 0 || k == 0) return launched("gaussian_keys");
   REQUIRE(uv && depth && radius && offsets, "gaussian_keys: null pointer");
@@ -1039,8 +1067,7 @@ int msplat_preprocess_views_backward(const float *xyz, const float *scale, const
           "preprocess_backward: n = %lld, views = %d", (long long)n, B);
   hipStream_t s = as_stream(stream);
   const bool cam = dL_dintr || dL_dextr;
-  if (dL_dintr) launch_fill_zero(dL_dintr, 4 * int64_t(B), s);
-  if (dL_dextr) launch_fill_zero(dL_dextr, 12 * int64_t(B), s);
+  launch_fill_zero_words({{dL_dintr, 4 * int64_t(B)}, {dL_dextr, 12 * int64_t(B)}}, s);
   if (n == 0) return launched("preprocess_backward");
   REQUIRE(xyz && scale && rotate && intr && extr && depth && radius && dL_duv && dL_dconic && dL_dxyz &&
               dL_dscale && dL_drotate,

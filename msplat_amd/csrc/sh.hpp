@@ -162,6 +162,65 @@ MSPLAT_DEV void sh_dir_grad(float x, float y, float z, const float *row, const f
   }
 }
 
+// ---- direction sources --------------------------------------------------------
+// The kernels read Gaussian i's direction through `load` and hand back the
+// direction gradient through `store_grad` (`store_zero` for an invisible
+// Gaussian).
+
+// Directions given as an [n,3] array (msplat/compute_sh.py:8-33); the
+// gradient is dL/ddir [n,3].
+struct ShDirArray {
+  const float *dirs;
+  float *grad;
+  MSPLAT_DEV void load(int64_t i, float &x, float &y, float &z) const {
+    x = dirs[3 * i];
+    y = dirs[3 * i + 1];
+    z = dirs[3 * i + 2];
+  }
+  MSPLAT_DEV void store_grad(int64_t i, float gx, float gy, float gz) const {
+    grad[3 * i] = gx;
+    grad[3 * i + 1] = gy;
+    grad[3 * i + 2] = gz;
+  }
+  MSPLAT_DEV void store_zero(int64_t i) const { store_grad(i, 0.f, 0.f, 0.f); }
+};
+
+// View directions from the Gaussian centres and one camera centre,
+// d = (p - c) / |p - c|, |v| = sqrt((vx vx + vy vy) + vz vz); the gradient is
+// dL/dp [n,3] through the normalisation, (g - d (d . g)) / |p - c|.
+struct ShDirFromPoints {
+  const float *xyz;
+  const float *campos;  // [3] in device memory
+  float *grad;
+  MSPLAT_DEV float offset(int64_t i, float &vx, float &vy, float &vz) const {
+    vx = xyz[3 * i] - campos[0];
+    vy = xyz[3 * i + 1] - campos[1];
+    vz = xyz[3 * i + 2] - campos[2];
+    return sqrtf(vx * vx + vy * vy + vz * vz);
+  }
+  MSPLAT_DEV void load(int64_t i, float &x, float &y, float &z) const {
+    float vx, vy, vz;
+    const float r = offset(i, vx, vy, vz);
+    x = vx / r;
+    y = vy / r;
+    z = vz / r;
+  }
+  MSPLAT_DEV void store_grad(int64_t i, float gx, float gy, float gz) const {
+    float vx, vy, vz;
+    const float r = offset(i, vx, vy, vz);
+    const float x = vx / r, y = vy / r, z = vz / r;
+    const float dot = x * gx + y * gy + z * gz;
+    grad[3 * i] = (gx - x * dot) / r;
+    grad[3 * i + 1] = (gy - y * dot) / r;
+    grad[3 * i + 2] = (gz - z * dot) / r;
+  }
+  MSPLAT_DEV void store_zero(int64_t i) const {
+    grad[3 * i] = 0.f;
+    grad[3 * i + 1] = 0.f;
+    grad[3 * i + 2] = 0.f;
+  }
+};
+
 // ---- LDS-staged kernels ------------------------------------------------------
 // A workgroup of G threads owns G consecutive Gaussians, whose coefficient
 // rows (R = C*D floats each) form one contiguous block of HBM.  The block is
@@ -200,9 +259,9 @@ MSPLAT_DEV void sh_block_copy(float *__restrict__ lds, float *__restrict__ glob,
   }
 }
 
-template <int L>
+template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_fwd_lds_kernel(int64_t n, int C, const float *__restrict__ shs,
-                                                         const float *__restrict__ dirs,
+                                                         const Dir dir,
                                                          const uint8_t *__restrict__ vis,
                                                          float *__restrict__ value, ShGeom sg) {
   constexpr int D = (L + 1) * (L + 1);
@@ -219,9 +278,10 @@ __global__ __launch_bounds__(256) void sh_fwd_lds_kernel(int64_t n, int C, const
     for (int c = 0; c < C; ++c) out[c] = 0.f;
     return;
   }
+  float dx, dy, dz;
+  dir.load(i, dx, dy, dz);
   float Y[D];
-  sh_basis<L>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2],
-              [&](int k, float v, float, float, float) { Y[k] = v; });
+  sh_basis<L>(dx, dy, dz, [&](int k, float v, float, float, float) { Y[k] = v; });
   const float *row = lds + t * sg.S;
   for (int c = 0; c < C; ++c) {
     float acc = 0.f;
@@ -231,13 +291,12 @@ __global__ __launch_bounds__(256) void sh_fwd_lds_kernel(int64_t n, int C, const
   }
 }
 
-template <int L>
+template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_bwd_lds_kernel(int64_t n, int C, const float *__restrict__ shs,
-                                                         const float *__restrict__ dirs,
+                                                         const Dir dir,
                                                          const uint8_t *__restrict__ vis,
                                                          const float *__restrict__ dval,
-                                                         float *__restrict__ dshs, float *__restrict__ ddirs,
-                                                         ShGeom sg) {
+                                                         float *__restrict__ dshs, ShGeom sg) {
   constexpr int D = (L + 1) * (L + 1);
   extern __shared__ float lds[];
   const int64_t g0 = int64_t(blockIdx.x) * sg.G;
@@ -251,7 +310,8 @@ __global__ __launch_bounds__(256) void sh_bwd_lds_kernel(int64_t n, int C, const
     const bool live = !vis || vis[i];
     float gx = 0.f, gy = 0.f, gz = 0.f;
     if (live) {
-      const float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
+      float x, y, z;
+      dir.load(i, x, y, z);
       const float *g = dval + i * C;
       sh_dir_grad<L>(x, y, z, row, g, C, gx, gy, gz);
       // the row is consumed: overwrite it with dL/dshs = Y_k g_c
@@ -265,9 +325,10 @@ __global__ __launch_bounds__(256) void sh_bwd_lds_kernel(int64_t n, int C, const
     } else {
       for (int r = 0; r < sg.R; ++r) row[r] = 0.f;
     }
-    ddirs[3 * i] = gx;
-    ddirs[3 * i + 1] = gy;
-    ddirs[3 * i + 2] = gz;
+    if (live)
+      dir.store_grad(i, gx, gy, gz);
+    else
+      dir.store_zero(i);
   }
   __syncthreads();
   sh_block_copy<false>(lds, dshs, g0, rows, sg);
@@ -316,9 +377,9 @@ MSPLAT_DEV void sh_store_row(float *__restrict__ row, const float (&r)[D]) {
 }
 
 // value[i, c] = sum_k Y_k(dir_i) shs[i, c, k], k ascending.
-template <int L>
+template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_fwd_gc_kernel(int64_t n, int C, const float *__restrict__ shs,
-                                                        const float *__restrict__ dirs,
+                                                        const Dir dir,
                                                         const uint8_t *__restrict__ vis,
                                                         float *__restrict__ value) {
   constexpr int D = (L + 1) * (L + 1);
@@ -329,9 +390,10 @@ __global__ __launch_bounds__(256) void sh_fwd_gc_kernel(int64_t n, int C, const 
     value[t] = 0.f;
     return;
   }
+  float dx, dy, dz;
+  dir.load(i, dx, dy, dz);
   float Y[D];
-  sh_basis<L>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2],
-              [&](int k, float v, float, float, float) { Y[k] = v; });
+  sh_basis<L>(dx, dy, dz, [&](int k, float v, float, float, float) { Y[k] = v; });
   float r[D];
   sh_load_row<D>(shs + t * D, r);
   float acc = 0.f;
@@ -342,12 +404,12 @@ __global__ __launch_bounds__(256) void sh_fwd_gc_kernel(int64_t n, int C, const 
 
 // dL/dshs[i,c,k] = Y_k g_c;  dL/ddir = sum_c sum_k (g_c shs[i,c,k]) grad Y_k,
 // the inner sum in basis order.
-template <int L>
+template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const float *__restrict__ shs,
-                                                        const float *__restrict__ dirs,
+                                                        const Dir dir,
                                                         const uint8_t *__restrict__ vis,
                                                         const float *__restrict__ dval,
-                                                        float *__restrict__ dshs, float *__restrict__ ddirs) {
+                                                        float *__restrict__ dshs) {
   constexpr int D = (L + 1) * (L + 1);
   const int G = kWave / C;  // Gaussians per wave
   const int lane = threadIdx.x % kWave;
@@ -364,7 +426,8 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
       for (int k = 0; k < D; ++k) z[k] = 0.f;
       sh_store_row<D>(dshs + t * D, z);
     } else {
-      const float x = dirs[3 * i], y = dirs[3 * i + 1], zz = dirs[3 * i + 2];
+      float x, y, zz;
+      dir.load(i, x, y, zz);
       const float gc = dval[t];
       float r[D];
       sh_load_row<D>(shs + t * D, r);
@@ -389,16 +452,17 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
     sz += oz;
   }
   if (active && c == 0) {
-    ddirs[3 * i] = sx;
-    ddirs[3 * i + 1] = sy;
-    ddirs[3 * i + 2] = sz;
+    if (vis && !vis[i])
+      dir.store_zero(i);
+    else
+      dir.store_grad(i, sx, sy, sz);
   }
 }
 
 // value[i, c] = sum_k Y_k(dir_i) * shs[i, c, k]  (compute_sh.cu:1600-1637)
-template <int L>
+template <int L, class Dir>
 __global__ __launch_bounds__(kShThreads) void sh_fwd_kernel(int64_t n, int C, const float *__restrict__ shs,
-                                                            const float *__restrict__ dirs,
+                                                            const Dir dir,
                                                             const uint8_t *__restrict__ vis,
                                                             float *__restrict__ value) {
   constexpr int D = (L + 1) * (L + 1);
@@ -409,9 +473,10 @@ __global__ __launch_bounds__(kShThreads) void sh_fwd_kernel(int64_t n, int C, co
     for (int c = 0; c < C; ++c) out[c] = 0.f;
     return;
   }
+  float dx, dy, dz;
+  dir.load(i, dx, dy, dz);
   float Y[D];
-  sh_basis<L>(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2],
-              [&](int k, float v, float, float, float) { Y[k] = v; });
+  sh_basis<L>(dx, dy, dz, [&](int k, float v, float, float, float) { Y[k] = v; });
   const float *row = shs + i * int64_t(C) * D;
   for (int c = 0; c < C; ++c) {
     float acc = 0.f;
@@ -423,13 +488,12 @@ __global__ __launch_bounds__(kShThreads) void sh_fwd_kernel(int64_t n, int C, co
 
 // dL/dshs[i,c,k] = Y_k g_c;  dL/ddir = sum_k (sum_c g_c shs[i,c,k]) grad Y_k
 // (compute_sh.cu:1639-1694).
-template <int L>
+template <int L, class Dir>
 __global__ __launch_bounds__(kShThreads) void sh_bwd_kernel(int64_t n, int C, const float *__restrict__ shs,
-                                                            const float *__restrict__ dirs,
+                                                            const Dir dir,
                                                             const uint8_t *__restrict__ vis,
                                                             const float *__restrict__ dval,
-                                                            float *__restrict__ dshs,
-                                                            float *__restrict__ ddirs) {
+                                                            float *__restrict__ dshs) {
   constexpr int D = (L + 1) * (L + 1);
   const int64_t i = int64_t(blockIdx.x) * kShThreads + threadIdx.x;
   if (i >= n) return;
@@ -438,10 +502,11 @@ __global__ __launch_bounds__(kShThreads) void sh_bwd_kernel(int64_t n, int C, co
     for (int c = 0; c < C; ++c)
 #pragma unroll
       for (int k = 0; k < D; ++k) drow[c * D + k] = 0.f;
-    ddirs[3 * i] = ddirs[3 * i + 1] = ddirs[3 * i + 2] = 0.f;
+    dir.store_zero(i);
     return;
   }
-  const float x = dirs[3 * i], y = dirs[3 * i + 1], z = dirs[3 * i + 2];
+  float x, y, z;
+  dir.load(i, x, y, z);
   const float *row = shs + i * int64_t(C) * D;
   const float *g = dval + i * C;
   {
@@ -455,9 +520,7 @@ __global__ __launch_bounds__(kShThreads) void sh_bwd_kernel(int64_t n, int C, co
   }
   float gx = 0.f, gy = 0.f, gz = 0.f;
   sh_dir_grad<L>(x, y, z, row, g, C, gx, gy, gz);
-  ddirs[3 * i] = gx;
-  ddirs[3 * i + 1] = gy;
-  ddirs[3 * i + 2] = gz;
+  dir.store_grad(i, gx, gy, gz);
 }
 
 }  // namespace msplat

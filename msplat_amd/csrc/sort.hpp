@@ -223,12 +223,18 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
 
   const int64_t base = tile * kRadixTile + int64_t(wave) * kPerWave;
   uint32_t key[kItems], val[kItems], rnk[kItems];
+  // All loads first: the ranking rounds below are separated by LDS fences,
+  // which would otherwise hold each round's load until the previous round's
+  // LDS traffic completes (kItems memory latencies in a row).
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const int64_t i = base + it * kWave + lane;
-    const bool valid = i < len;
-    key[it] = valid ? kin[i] : 0u;
-    val[it] = valid ? vin[i] : 0u;
+    key[it] = i < len ? kin[i] : 0u;
+    val[it] = i < len ? vin[i] : 0u;
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const bool valid = base + it * kWave + lane < len;
     const uint32_t d = (key[it] >> shift) & mask;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; ++b) {

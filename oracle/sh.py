@@ -145,3 +145,32 @@ def sh_backward(shs, dirs, dvalue, visible=None, dtype=np.float32):
         dshs = np.where(vis[:, None, None], dshs, dt(0))
         dd = np.where(vis[:, None], dd, dt(0))
     return dshs, dd
+
+
+def view_dirs_from_points(xyz, campos, dtype=np.float32):
+    """Fused view directions: v = xyz - campos, r = sqrt((vx vx + vy vy) + vz vz),
+    d = v / r, in the kernel's operation order.  No reference counterpart (the
+    reference's compute_sh takes the directions: compute_sh.py:8-33)."""
+    dt = np.dtype(dtype).type
+    v = np.asarray(xyz, dtype=dt) - np.asarray(campos, dtype=dt).reshape(1, 3)
+    r = np.sqrt((v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
+    return v / r[:, None], r
+
+
+def sh_points_forward(shs, xyz, campos, visible=None, dtype=np.float32):
+    """compute_sh of the fused view directions: value [N, C]."""
+    d, _ = view_dirs_from_points(xyz, campos, dtype)
+    return sh_forward(shs, d, visible, dtype)
+
+
+def sh_points_backward(shs, xyz, campos, dvalue, visible=None, dtype=np.float32):
+    """dL/dshs [N,C,D] and dL/dxyz [N,3]: the SH adjoint's dL/ddir g taken through
+    the normalisation, (g - d (d . g)) / r, dot product summed x, y, z."""
+    dt = np.dtype(dtype).type
+    d, r = view_dirs_from_points(xyz, campos, dtype)
+    dshs, g = sh_backward(shs, d, dvalue, visible, dtype)
+    dot = (d[:, 0] * g[:, 0] + d[:, 1] * g[:, 1]) + d[:, 2] * g[:, 2]
+    dxyz = (g - d * dot[:, None]) / r[:, None]
+    if visible is not None:
+        dxyz = np.where(np.asarray(visible).reshape(-1)[:, None], dxyz, dt(0))
+    return dshs, dxyz.astype(dt)

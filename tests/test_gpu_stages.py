@@ -146,6 +146,55 @@ def test_sh_bitexact(deg):
     assert_bitexact(np_(d.grad), odd, "ddirs")
 
 
+@pytest.mark.parametrize("deg,C", [(0, 3), (1, 3), (3, 3), (3, 4), (6, 35), (7, 3), (10, 3), (3, 70)])
+def test_sh_points_bitexact(deg, C):
+    """Fused view direction + SH against the oracle's restatement, bit for bit
+    (gc kernels for degree <= 6 and C <= 64, LDS / per-Gaussian kernels otherwise)."""
+    D = (deg + 1) ** 2
+    gen = torch.Generator().manual_seed(300 + deg + C)
+    n = 555
+    xyz = torch.randn(n, 3, generator=gen) * 2.0
+    campos = torch.tensor([0.5, -3.5, 2.0])
+    shs = torch.randn(n, C, D, generator=gen)
+    vis = torch.rand(n, generator=gen) > 0.1
+    s = shs.to(DEV).requires_grad_()
+    x = xyz.to(DEV).requires_grad_()
+    c = campos.to(DEV).requires_grad_()
+    val = ms.compute_sh_points(s, x, c, vis.to(DEV))
+    assert_bitexact(np_(val), osh.sh_points_forward(np_(shs), np_(xyz), np_(campos), np_(vis)), "value")
+    gv = torch.randn(n, C, generator=gen)
+    (val * gv.to(DEV)).sum().backward()
+    ods, odx = osh.sh_points_backward(np_(shs), np_(xyz), np_(campos), np_(gv), np_(vis))
+    assert_bitexact(np_(s.grad), ods, "dshs")
+    assert_bitexact(np_(x.grad), odx, "dxyz")
+    assert rel_err(np_(c.grad), -odx.astype(np.float64).sum(0)) < 1e-5
+
+
+def test_sh_points_equals_unfused_composition():
+    """compute_sh_points == compute_sh(shs, normalize(xyz - campos)) with torch's
+    normalisation, values and gradients (xyz, shs, campos) to fp32 rounding."""
+    gen = torch.Generator().manual_seed(9)
+    n, C, D = 4000, 3, 16
+    xyz = (torch.rand(n, 3, generator=gen) * 2 - 1) * 3.0
+    campos = torch.tensor([1.0, -4.0, 0.5])
+    shs = torch.randn(n, C, D, generator=gen) * 0.3
+    gv = torch.randn(n, C, generator=gen).to(DEV)
+    outs = []
+    for fused in (True, False):
+        s = shs.to(DEV).requires_grad_()
+        x = xyz.to(DEV).requires_grad_()
+        c = campos.to(DEV).requires_grad_()
+        if fused:
+            val = ms.compute_sh_points(s, x, c)
+        else:
+            d = x - c[None]
+            val = ms.compute_sh(s, d / d.norm(dim=1, keepdim=True))
+        (val * gv).sum().backward()
+        outs.append((val.detach(), s.grad, x.grad, c.grad))
+    for a, b, name in zip(outs[0], outs[1], ("value", "dshs", "dxyz", "dcampos")):
+        assert rel_err(np_(a), np_(b)) < 1e-5, name
+
+
 def test_sort_kat():
     import json
     import os

@@ -84,3 +84,52 @@ def test_visibility_and_fp32_accuracy(rng):
     assert np.all(v32[~vis] == 0)
     # the recurrence in fp32 stays far inside the reference test's 5e-4 tolerance at degree 10
     assert np.abs(v32 - v64).max() < 2e-5
+
+
+@pytest.mark.parametrize("deg", [0, 1, 3, 6, 10])
+def test_sh_points_backward_fd(deg, rng):
+    """The fused view-direction adjoint against central differences of the fp64 forward."""
+    D = (deg + 1) ** 2
+    n, C = 5, 3
+    shs = rng.normal(size=(n, C, D))
+    xyz = rng.normal(size=(n, 3)) * 2.0
+    campos = np.array([0.3, -4.0, 1.5])
+    g = rng.normal(size=(n, C))
+    loss = lambda p: (osh.sh_points_forward(shs, p, campos, dtype=np.float64) * g).sum()
+    _, dxyz = osh.sh_points_backward(shs, xyz, campos, g, dtype=np.float64)
+    h = 1e-6
+    for idx in np.ndindex(xyz.shape):
+        xp, xm = xyz.copy(), xyz.copy()
+        xp[idx] += h
+        xm[idx] -= h
+        np.testing.assert_allclose(dxyz[idx], (loss(xp) - loss(xm)) / (2 * h), rtol=1e-6, atol=1e-7)
+
+
+def test_sh_points_normalisation_adjoint_matches_torch(rng):
+    """(g - d (d.g)) / r equals torch's fp64 autograd of v / |v| (an independent adjoint)."""
+    import torch
+
+    xyz = rng.normal(size=(50, 3)) * 3.0
+    campos = rng.normal(size=3) * 4.0
+    g = rng.normal(size=(50, 3))
+    d, r = osh.view_dirs_from_points(xyz, campos, np.float64)
+    v = torch.tensor(xyz - campos[None], dtype=torch.float64, requires_grad=True)
+    dt = v / v.norm(dim=1, keepdim=True)
+    np.testing.assert_allclose(d, dt.detach().numpy(), rtol=1e-15, atol=1e-15)
+    dt.backward(torch.tensor(g))
+    dot = (d * g).sum(1, keepdims=True)
+    np.testing.assert_allclose((g - d * dot) / r[:, None], v.grad.numpy(), rtol=1e-12, atol=1e-14)
+
+
+def test_sh_points_forward_is_sh_of_normalised_offsets(rng):
+    n, C, D = 40, 3, 16
+    shs = rng.normal(size=(n, C, D))
+    xyz = rng.normal(size=(n, 3))
+    campos = np.array([1.0, 2.0, -3.0])
+    v = xyz - campos
+    ref = osh.sh_forward(shs, v / np.linalg.norm(v, axis=1, keepdims=True), dtype=np.float64)
+    np.testing.assert_allclose(osh.sh_points_forward(shs, xyz, campos, dtype=np.float64), ref, rtol=1e-13, atol=1e-14)
+    vis = np.arange(n) % 3 != 0
+    _, dxyz = osh.sh_points_backward(shs.astype(np.float32), xyz.astype(np.float32), campos, rng.normal(size=(n, C)),
+                                     vis)
+    assert np.all(dxyz[~vis] == 0) and dxyz.dtype == np.float32
