@@ -103,6 +103,13 @@ MSPLAT_DEV PixBlock<PPL> pix_block(const BlendIn &in) {
 // Workgroups per launch for a given pixels-per-lane (8 XCD bands of per_xcd).
 inline int blend_per_xcd(int ntiles, int ppl) { return (ntiles * (4 / ppl) + 7) / 8; }
 
+// 0.0f in a fresh VGPR that the compiler cannot treat as a constant.
+MSPLAT_DEV float opaque_zero() {
+  float z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
 // exp(x) as 2^(x log2 e) on the hardware exp2 (v_exp_f32).
 MSPLAT_DEV float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 
@@ -253,9 +260,14 @@ MSPLAT_DEV int2 tile_list(const BlendIn &in, const PixBlock<PPL> &pb) {
 }
 
 // power = -0.5 (a dx^2 + c dy^2) - b dx dy with d = mean - pixel
-// (alpha_blending.cu:74-82).
+// (alpha_blending.cu:74-82), with two explicit fused multiply-adds (the
+// blending kernels are bound by VALU issue).  Written out rather than left to
+// contraction, so that every kernel instantiation -- forward and adjoint, any
+// pixels-per-lane, tile or cell lists -- rounds it identically and they all
+// agree on every alpha and every skip.
 MSPLAT_DEV float gauss_power(float a, float b, float c, float dx, float dy) {
-  return -0.5f * ((a * dx) * dx + (c * dy) * dy) - (b * dx) * dy;
+  const float q = __builtin_fmaf(a * dx, dx, (c * dy) * dy);
+  return __builtin_fmaf(-0.5f, q, -((b * dx) * dy));
 }
 
 // Forward.  Outputs rendered [C, H, W] (+ bg * T), final_T and ncontrib
@@ -279,11 +291,13 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
   const int end = lst.y;
 
   bool done[PPL];
-  float T[PPL];
+  float T[PPL], fx[PPL], fy[PPL];
   int last[PPL], last_e[PPL];
   float F[PPL][CMAX];
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
+    fx[i] = float(pb.px[i]);
+    fy[i] = float(pb.py[i]);
     done[i] = !(pb.px[i] < pb.W && pb.py[i] < pb.H);
     T[i] = 1.f;
     last[i] = 0;
@@ -329,6 +343,12 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
     }
     __syncthreads();
     for (int j = 0; j < cnt; ++j) {
+      if ((j & 7) == 7) {  // every 8 Gaussians: stop once every pixel of the wave is saturated
+        bool live = false;
+#pragma unroll
+        for (int i = 0; i < PPL; ++i) live |= !done[i];
+        if (__ballot(live) == 0ull) break;
+      }
       float4 g0, g1;
       stage_geom(s, j, g0, g1);
       float f[CMAX];
@@ -343,7 +363,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
         // Branch-free: every quantity is computed and the reference's
         // early-outs become selects, so each update happens exactly where the
         // reference's would (with the same operations) and nowhere else.
-        const float dx = g0.x - float(pb.px[i]), dy = g0.y - float(pb.py[i]);
+        const float dx = g0.x - fx[i], dy = g0.y - fy[i];
         const float power = gauss_power(g0.z, g0.w, g1.x, dx, dy);
         const float alpha = fminf(0.99f, g1.y * fast_exp(power));
         // a NaN power is not skipped, as in the reference
@@ -351,11 +371,10 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
         const float nT = T[i] * (1.f - alpha);
         const bool sat = hit && nT < 0.0001f;
         const bool add = hit && !sat;
+        const float w = alpha * T[i];
 #pragma unroll
-        for (int ch = 0; ch < CMAX; ++ch) {  // channels >= cc are staged as 0 and never stored
-          const float nf = F[i][ch] + f[ch] * alpha * T[i];
-          F[i][ch] = add ? nf : F[i][ch];
-        }
+        for (int ch = 0; ch < CMAX; ++ch)  // channels >= cc are staged as 0 and never stored
+          F[i][ch] = add ? __builtin_fmaf(f[ch], w, F[i][ch]) : F[i][ch];
         T[i] = add ? nT : T[i];
         last[i] = add ? contributor : last[i];
         if constexpr (kWalk) last_e[i] = add ? epos : last_e[i];
@@ -403,6 +422,10 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
     BlendIn in, const float *__restrict__ final_T, const int32_t *__restrict__ ncontrib,
     const int32_t *__restrict__ walk, const float *__restrict__ dout, float *__restrict__ duv,
     float *__restrict__ dconic, float *__restrict__ dopac, float *__restrict__ dfeat) {
+  // The gradient arithmetic may contract into fused multiply-adds (gradients
+  // are compared to a tolerance); every value the forward also computes
+  // (power, alpha, T) goes through explicitly rounded code.
+#pragma clang fp contract(fast)
   __shared__ BlendStage<CMAX> s;
   const PixBlock<PPL> pb = pix_block<PPL>(in);
   if (!pb.active) return;
@@ -418,7 +441,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   // alpha and colour (alpha_blending.cu:213-214); updating it right after
   // each contributor performs the same operations on the same values and
   // keeps no copy of the previous colour.
-  float T[PPL], tb[PPL];
+  float T[PPL], tb[PPL], fx[PPL], fy[PPL];
   int last[PPL];
   float g[PPL][CMAX], accum[PPL][CMAX];
   int wl = 0;
@@ -427,6 +450,8 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
     const bool inside = pb.px[i] < pb.W && pb.py[i] < pb.H;
     const int64_t pix = int64_t(pb.py[i]) * pb.W + pb.px[i];
     const float T_final = inside ? final_T[pb.pix0 + pix] : 0.f;
+    fx[i] = float(pb.px[i]);
+    fy[i] = float(pb.py[i]);
     last[i] = inside ? ncontrib[pb.pix0 + pix] : 0;
     wl = imax(wl, last[i]);
     float bg_dot = 0.f;
@@ -493,25 +518,28 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
       const float a = g0.z, b = g0.w, c = g1.x, op = g1.y;
       float fe[CMAX];
       stage_features(s, j, g1, fe);
-      float vuv0 = 0.f, vuv1 = 0.f, vc0 = 0.f, vc1 = 0.f, vc2 = 0.f, vo = 0.f;
+      // Accumulators start from an opaque zero: with a literal 0 the compiler
+      // rematerialises the constant on every skip path of every pixel slot.
+      float vuv0 = opaque_zero(), vuv1 = opaque_zero(), vc0 = opaque_zero(), vc1 = opaque_zero(),
+            vc2 = opaque_zero(), vo = opaque_zero();
       float vf[CMAX];
 #pragma unroll
-      for (int ch = 0; ch < CMAX; ++ch) vf[ch] = 0.f;
+      for (int ch = 0; ch < CMAX; ++ch) vf[ch] = opaque_zero();
       bool any = false;
       const uint32_t strips = __builtin_amdgcn_readfirstlane(s.strips[j]);
 #pragma unroll
       for (int i = 0; i < PPL; ++i) {
         if (!(strips & slot_strips<PPL>(pb.sub, i))) continue;  // uniform: culled for these rows
         if (rel >= wl_slot[i]) continue;                         // uniform: every pixel of the slot ended
-        // The reference's early-outs as branches (measured faster here than
-        // selects: most pixels of a tile share the outcome).
-        if (!(rel < last[i])) continue;
-        const float dx = g0.x - float(pb.px[i]), dy = g0.y - float(pb.py[i]);
+        // The reference's three early-outs (position, power > 0, alpha <
+        // 1/255) folded into one branch: alpha is cheap next to the gradient
+        // terms, and one level of divergence keeps the accumulators out of
+        // per-level zero copies.
+        const float dx = g0.x - fx[i], dy = g0.y - fy[i];
         const float power = gauss_power(a, b, c, dx, dy);
-        if (power > 0.f) continue;
         const float G = fast_exp(power);
         const float alpha = fminf(0.99f, op * G);
-        if (alpha < 1.f / 255.f) continue;
+        if (!(rel < last[i]) || power > 0.f || alpha < 1.f / 255.f) continue;
         any = true;
         // 1 / (1 - alpha) on the hardware reciprocal (the reference divides
         // with fast-math); 1 - alpha >= 0.01, so it is well conditioned.
