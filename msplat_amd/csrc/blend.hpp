@@ -480,6 +480,36 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
     fcount = hi - lst.x;
   }
 
+  // After the transposed reduction, lane 16 r + k holds gradient value
+  // v = 4 (k + 16 q) + kRowValue[r] in round q.  Its target array, offset and
+  // row stride depend on the lane only: the address of a Gaussian's value is
+  // then base + gidx * stride.
+  constexpr int NV = ((6 + CMAX) + 3) / 4 * 4;
+  constexpr int NQ = (NV / 4 + 15) / 16;  // atomic rounds
+  float *tbase[NQ];
+  int tstride[NQ];
+  bool tvalid[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int vec = (lane & 15) + 16 * q;
+    const int v = 4 * vec + row_value(lane >> 4);
+    tvalid[q] = vec < NV / 4 && v < 6 + cc;
+    const int64_t gsh = pb.gshift;
+    if (v < 2) {
+      tbase[q] = duv + v;
+      tstride[q] = 2;
+    } else if (v < 5) {
+      tbase[q] = dconic + (v - 2);
+      tstride[q] = 3;
+    } else if (v == 5) {
+      tbase[q] = dopac + gsh;
+      tstride[q] = 1;
+    } else {
+      tbase[q] = dfeat + (in.feat_per_view ? 0 : gsh * in.C) + in.c0 + (v - 6);
+      tstride[q] = in.C;
+    }
+  }
+
   while (fcount > 0 && hi > lo_bound) {
     const int lo = hi - kBlendLanes > lo_bound ? hi - kBlendLanes : lo_bound;
     int cnt;
@@ -567,10 +597,8 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
       }
       if (__ballot(any) == 0ull) continue;
       // Sum the 6 + cc per-lane contributions over the wave in one transposed
-      // reduction; lane 16 r + k then owns value 4 k' + kRowValue[r] of
-      // vector k' = k + 16 q in atomic round q and issues its atomic.
+      // reduction, then each lane adds its value to its target.
       const int gidx = s.id[j];
-      constexpr int NV = ((6 + CMAX) + 3) / 4 * 4;
       float vals[NV];
       vals[0] = vuv0;
       vals[1] = vuv1;
@@ -582,23 +610,14 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
       for (int ch = 0; ch < NV - 6; ++ch) vals[6 + ch] = ch < CMAX ? vf[ch < CMAX ? ch : 0] : 0.f;
       float tot[NV / 4];
       wave_sum_transposed<NV>(vals, tot);
-      const int row = lane >> 4, k = lane & 15;
 #pragma unroll
-      for (int q = 0; q * 16 < NV / 4; ++q) {
-        const int vec = k + 16 * q;
+      for (int q = 0; q < NQ; ++q) {
+        const int vec = (lane & 15) + 16 * q;
         float mine = 0.f;
 #pragma unroll
         for (int jj = 0; jj < NV / 4; ++jj)
           if (jj == vec) mine = tot[jj];
-        const int v = 4 * vec + kRowValue[row];
-        float *dst = nullptr;
-        if (vec < NV / 4 && v < 6 + cc) {
-          if (v < 2) dst = duv + 2 * int64_t(gidx) + v;
-          else if (v < 5) dst = dconic + 3 * int64_t(gidx) + (v - 2);
-          else if (v == 5) dst = dopac + (gidx + pb.gshift);
-          else dst = dfeat + feature_row(in, gidx, pb.gshift) * in.C + in.c0 + (v - 6);
-        }
-        if (dst) unsafeAtomicAdd(dst, mine);
+        if (tvalid[q]) unsafeAtomicAdd(tbase[q] + int64_t(gidx) * tstride[q], mine);
       }
     }
     fcount = fbase;

@@ -140,6 +140,10 @@ MSPLAT_DEV bool wave_any(bool p) { return __ballot(p) != 0ull; }
 // per value instead of 12 for a separate butterfly per value.
 constexpr int kRowValue[4] = {0, 2, 1, 3};
 
+// kRowValue[r] as arithmetic (a table indexed by a lane-dependent row would
+// be a memory load).
+MSPLAT_HD int row_value(int r) { return ((r & 1) << 1) | (r >> 1); }
+
 MSPLAT_DEV float dpp_row_ror_add(float v, int ctrl) {
   // ctrl: 0x121..0x12F = row_ror:1..15 (rotate within each row of 16 lanes)
   int r;
