@@ -548,10 +548,13 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
       const float a = g0.z, b = g0.w, c = g1.x, op = g1.y;
       float fe[CMAX];
       stage_features(s, j, g1, fe);
-      // Accumulators start from an opaque zero: with a literal 0 the compiler
-      // rematerialises the constant on every skip path of every pixel slot.
-      float vuv0 = opaque_zero(), vuv1 = opaque_zero(), vc0 = opaque_zero(), vc1 = opaque_zero(),
-            vc2 = opaque_zero(), vo = opaque_zero();
+      // The lane's pixels share one column (PPL 2 and 4 stack rows), so dx is
+      // common to them and the conic and mean gradients reduce to three
+      // per-lane moments of w = G dL/dG over its pixels: sum w, sum w dy and
+      // sum w dy^2.  Accumulators start from an opaque zero: with a literal 0
+      // the compiler rematerialises the constant on every skip path.
+      const float dx = g0.x - fx[0];
+      float sw = opaque_zero(), swy = opaque_zero(), swyy = opaque_zero(), vo = opaque_zero();
       float vf[CMAX];
 #pragma unroll
       for (int ch = 0; ch < CMAX; ++ch) vf[ch] = opaque_zero();
@@ -565,7 +568,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         // 1/255) folded into one branch: alpha is cheap next to the gradient
         // terms, and one level of divergence keeps the accumulators out of
         // per-level zero copies.
-        const float dx = g0.x - fx[i], dy = g0.y - fy[i];
+        const float dy = g0.y - fy[i];
         const float power = gauss_power(a, b, c, dx, dy);
         const float G = fast_exp(power);
         const float alpha = fminf(0.99f, op * G);
@@ -579,22 +582,27 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         float dL_dalpha = 0.f;
 #pragma unroll
         for (int ch = 0; ch < CMAX; ++ch) {  // channels >= cc carry zeros
-          dL_dalpha += (fe[ch] - accum[i][ch]) * g[i][ch];
+          const float d = fe[ch] - accum[i][ch];
+          dL_dalpha += d * g[i][ch];
           vf[ch] += dcol * g[i][ch];
-          accum[i][ch] = alpha * fe[ch] + (1.f - alpha) * accum[i][ch];
+          accum[i][ch] += alpha * d;  // = alpha fe + (1 - alpha) accum
         }
         dL_dalpha *= T[i];
         dL_dalpha += tb[i] * inv;
-        const float dL_dG = op * dL_dalpha;
-        const float gdx = ((-G) * dx) * a - (G * dy) * b;
-        const float gdy = ((-G) * dy) * c - (G * dx) * b;
-        vuv0 += dL_dG * gdx;
-        vuv1 += dL_dG * gdy;
-        vc0 += (((-0.5f * G) * dx) * dx) * dL_dG;
-        vc1 += (((-G) * dx) * dy) * dL_dG;
-        vc2 += (((-0.5f * G) * dy) * dy) * dL_dG;
+        const float w = G * (op * dL_dalpha);  // G dL/dG
+        const float wy = w * dy;
+        sw += w;
+        swy += wy;
+        swyy += wy * dy;
         vo += G * dL_dalpha;
       }
+      // dL/duv = -(sum w (a dx + b dy), sum w (b dx + c dy)),
+      // dL/dconic = -(sum w dx^2 / 2, sum w dx dy, sum w dy^2 / 2)
+      const float vuv0 = -(a * dx * sw + b * swy);
+      const float vuv1 = -(b * dx * sw + c * swy);
+      const float vc0 = -0.5f * dx * dx * sw;
+      const float vc1 = -dx * swy;
+      const float vc2 = -0.5f * swyy;
       if (__ballot(any) == 0ull) continue;
       // Sum the 6 + cc per-lane contributions over the wave in one transposed
       // reduction, then each lane adds its value to its target.
