@@ -14,6 +14,12 @@
   Finished batches are found by polling their completion events; their
   memory returns to the budget and each request's latency (completion -
   arrival) is recorded.
+- A launch never waits for the device: cameras go up through pinned staging
+  buffers (asynchronous copies on the batch's stream) and the binning runs
+  in capacity mode (the entry count stays on the device).  The capacity is
+  a running estimate of cell entries per (view, Gaussian) row; a batch whose
+  lists overflowed it is detected when it retires, the estimate is raised
+  and the batch is rendered again with the exact count.
 - ``run_poisson`` drives the server with Poisson arrivals and reports p50 /
   p99 latency beside throughput.
 
@@ -105,10 +111,14 @@ class _Batch:
     out: torch.Tensor
     nbytes: int
     stream: torch.cuda.Stream
+    rows: int = 0
+    capacity: int | None = None
+    count: torch.Tensor | None = None  # [1] int64 on the device: the true cell-entry count
 
 
 class RenderServer:
-    def __init__(self, store: SceneStore, hbm_budget: int = 16 << 30, max_views: int = 16, streams: int = 3):
+    def __init__(self, store: SceneStore, hbm_budget: int = 16 << 30, max_views: int = 16, streams: int = 3,
+                 entries_per_row: float = 2.0, min_capacity: int = 4096):
         if not 1 <= max_views <= views.MAX_VIEWS:
             raise ValueError(f"max_views must be in 1..{views.MAX_VIEWS}")
         self.store = store
@@ -121,6 +131,18 @@ class RenderServer:
         self.next_stream = 0
         self.next_batch = 0
         self.done: list[Request] = []
+        # capacity estimate (cell entries per row) and how often it was too small
+        self.entries_per_row = float(entries_per_row)
+        self.min_capacity = int(min_capacity)
+        self.rerenders = 0
+        # per stream: pinned host staging and device buffers for the cameras,
+        # and the event after the last copy out of the staging buffers
+        dev = store.device
+        self._cam_host = [(torch.empty((max_views, 4), pin_memory=True), torch.empty((max_views, 12), pin_memory=True))
+                          for _ in self.streams]
+        self._cam_dev = [(torch.empty((max_views, 4), device=dev), torch.empty((max_views, 12), device=dev))
+                         for _ in self.streams]
+        self._cam_copied: list[torch.cuda.Event | None] = [None] * len(self.streams)
 
     def submit(self, req: Request) -> None:
         if req.scene not in self.store.scenes:
@@ -142,23 +164,50 @@ class RenderServer:
             nbytes += cost
         return batch
 
-    def _launch(self, reqs: list[Request]) -> None:
+    def _cameras(self, slot: int, reqs: list[Request], stream) -> tuple[torch.Tensor, torch.Tensor]:
+        """The batch's [B, 4] intrinsics and [B, 12] extrinsics on the device, without a host wait."""
+        B = len(reqs)
+        di, de = self._cam_dev[slot]
+        if any(r.intr.is_cuda or r.extr.is_cuda for r in reqs):  # already on the device
+            with torch.cuda.stream(stream):
+                for i, r in enumerate(reqs):
+                    di[i].copy_(r.intr.reshape(4))
+                    de[i].copy_(r.extr.reshape(-1)[:12])
+            return di[:B], de[:B]
+        hi, he = self._cam_host[slot]
+        if self._cam_copied[slot] is not None:
+            self._cam_copied[slot].synchronize()  # the previous copy out of this staging has run
+        for i, r in enumerate(reqs):
+            hi[i].copy_(r.intr.reshape(4))
+            he[i].copy_(r.extr.reshape(-1)[:12])
+        with torch.cuda.stream(stream):
+            di[:B].copy_(hi[:B], non_blocking=True)
+            de[:B].copy_(he[:B], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        self._cam_copied[slot] = ev
+        return di[:B], de[:B]
+
+    def _launch(self, reqs: list[Request], exact: bool = False) -> None:
         st = self.store
         dev = st.device
         nbytes = sum(self._cost(r) for r in reqs)
-        stream = self.streams[self.next_stream % len(self.streams)]
+        slot = self.next_stream % len(self.streams)
+        stream = self.streams[slot]
         self.next_stream += 1
         stream.wait_stream(torch.cuda.current_stream(dev))
         sizes = [(r.W, r.H) for r in reqs]
         first = [st.scenes[r.scene].first for r in reqs]
         count = [st.scenes[r.scene].count for r in reqs]
+        rows = sum(count)
+        capacity = None if exact else int(rows * self.entries_per_row) + self.min_capacity
         cfg = tuning.config_for_views(sizes, st.C)
+        intrs, extrs = self._cameras(slot, reqs, stream)
         with torch.cuda.stream(stream), torch.no_grad():
-            intrs = torch.stack([r.intr.to(dev, torch.float32).reshape(4) for r in reqs])
-            extrs = torch.stack([r.extr.to(dev, torch.float32).reshape(-1)[:12] for r in reqs])
             uv, depth, conic, radius, _ = views.preprocess_views_forward(st.xyz, st.scale, st.rotate, intrs, extrs,
                                                                          sizes, first=first, count=count)
-            bins = views.bin_views(uv, depth, radius, st.xyz.shape[0], sizes, cfg.cell, first=first, count=count)
+            bins = views.bin_views(uv, depth, radius, st.xyz.shape[0], sizes, cfg.cell, capacity=capacity,
+                                   first=first, count=count)
             bg = reqs[0].bg
             out, _, _, _ = views.rasterize_views_forward(uv, conic, st.opacity, st.feature, st.xyz.shape[0], sizes,
                                                          bins, bg, cfg.ppl_fwd, cfg.cull, first=first, count=count)
@@ -166,7 +215,7 @@ class RenderServer:
             ev.record(stream)
         for r in reqs:
             r.batch = self.next_batch
-        self.in_flight.append(_Batch(self.next_batch, reqs, ev, out, nbytes, stream))
+        self.in_flight.append(_Batch(self.next_batch, reqs, ev, out, nbytes, stream, rows, capacity, bins.count))
         self.next_batch += 1
         self.used += nbytes
 
@@ -179,20 +228,29 @@ class RenderServer:
             self._launch(group)
 
     def _retire(self) -> int:
-        keep, finished = [], 0
+        keep, again, finished = [], [], 0
         for b in self.in_flight:
-            if b.event.query():
-                now = time.perf_counter()
-                imgs = views.split_views(b.out, self.store.C, [(r.W, r.H) for r in b.reqs])
-                for r, im in zip(b.reqs, imgs):
-                    r.image = im
-                    r.t_done = now
-                    self.done.append(r)
-                self.used -= b.nbytes
-                finished += 1
-            else:
+            if not b.event.query():
                 keep.append(b)
+                continue
+            self.used -= b.nbytes
+            if b.capacity is not None:
+                k = int(b.count.item())  # the batch has finished: no wait
+                if k > b.capacity:  # truncated lists: raise the estimate, render again exactly
+                    self.entries_per_row = max(self.entries_per_row, 1.25 * k / max(b.rows, 1))
+                    self.rerenders += 1
+                    again.append(b.reqs)
+                    continue
+            now = time.perf_counter()
+            imgs = views.split_views(b.out, self.store.C, [(r.W, r.H) for r in b.reqs])
+            for r, im in zip(b.reqs, imgs):
+                r.image = im
+                r.t_done = now
+                self.done.append(r)
+            finished += 1
         self.in_flight = keep
+        for reqs in again:
+            self._launch(reqs, exact=True)
         return finished
 
     def step(self) -> int:
@@ -234,19 +292,23 @@ def run_poisson(server: RenderServer, make_request, rate: float, duration: float
         server.submit(make_request(rng))
     server.drain()
     server.done.clear()
+    # the client's arrival times and requests (cameras) are drawn before the
+    # clock starts, so that making them does not delay the server loop
+    arrivals, t = [], rng.expovariate(rate)
+    while t < duration:
+        arrivals.append((t, make_request(rng)))
+        t += rng.expovariate(rate)
     t0 = time.perf_counter()
-    t_next = t0 + rng.expovariate(rate)
-    submitted = 0
+    nxt = 0
     while True:
         now = time.perf_counter()
-        while t_next <= now and t_next < t0 + duration:
-            req = make_request(rng)
-            req.t_arrival = t_next
+        while nxt < len(arrivals) and t0 + arrivals[nxt][0] <= now:
+            req = arrivals[nxt][1]
+            req.t_arrival = t0 + arrivals[nxt][0]
             server.submit(req)
-            submitted += 1
-            t_next += rng.expovariate(rate)
+            nxt += 1
         if not server.step():
-            if t_next >= t0 + duration and not server.busy():
+            if nxt >= len(arrivals) and not server.busy():
                 break
             time.sleep(20e-6)
     t1 = time.perf_counter()
@@ -256,4 +318,5 @@ def run_poisson(server: RenderServer, make_request, rate: float, duration: float
                 throughput_req_s=round(len(server.done) / (t1 - t0), 2),
                 throughput_mpix_s=round(pix / (t1 - t0) / 1e6, 2),
                 p50_ms=round(percentile(lat, 0.5), 3), p99_ms=round(percentile(lat, 0.99), 3),
-                max_ms=round(max(lat), 3) if lat else None, batches=server.next_batch)
+                max_ms=round(max(lat), 3) if lat else None, batches=server.next_batch,
+                rerendered_batches=server.rerenders)

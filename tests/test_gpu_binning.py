@@ -57,6 +57,23 @@ def test_bins_ties_and_negative_depth():
         _check_bins(uv, depth, radius, W, H, cell)
 
 
+@pytest.mark.parametrize("cell", [1, 8])
+def test_bins_capacity_mode(cell):
+    """With a capacity the count stays on the device: a capacity above the count
+    gives the exact lists (entries past the count unused); one below it writes
+    only `capacity` entries and overflowed() reports it."""
+    _, uv, depth, _, radius, _ = _projected(4, 4000, 192, 128)
+    exact = binning.bin_gaussians(uv, depth, 192, 128, radius, cell)
+    k = exact.cell_ids.numel()
+    assert k > 100 and not exact.overflowed()
+    roomy = binning.bin_gaussians(uv, depth, 192, 128, radius, cell, capacity=k + 777)
+    assert not roomy.overflowed() and int(roomy.count.item()) == k
+    assert torch.equal(roomy.cell_ids[:k], exact.cell_ids) and torch.equal(roomy.cell_range, exact.cell_range)
+    short = binning.bin_gaussians(uv, depth, 192, 128, radius, cell, capacity=k // 3)
+    assert short.overflowed() and int(short.count.item()) == k
+    assert int(short.cell_range.max()) <= k // 3
+
+
 def test_bins_empty():
     uv = torch.zeros(0, 2, device=DEV)
     b = binning.bin_gaussians(uv, torch.zeros(0, 1, device=DEV), 64, 64, torch.zeros(0, dtype=torch.int32,

@@ -65,3 +65,21 @@ def test_poisson_stats():
     stats = serving.run_poisson(server, lambda rng: _request(rng, st), rate=300.0, duration=0.5, seed=1, warmup=4)
     assert stats["requests"] > 0
     assert 0 < stats["p50_ms"] <= stats["p99_ms"] <= stats["max_ms"]
+
+
+def test_capacity_overflow_renders_again():
+    """A capacity estimate far too small truncates the first batches' lists; the
+    server sees the overflow when they retire, raises the estimate and renders
+    them again, so every image still equals the single-view rendering."""
+    st = _store()
+    rng = random.Random(7)
+    server = serving.RenderServer(st, hbm_budget=4096 << 20, max_views=6, streams=2, entries_per_row=1e-4,
+                                  min_capacity=16)
+    reqs = [_request(rng, st) for _ in range(9)]
+    for r in reqs:
+        server.submit(r)
+    server.drain()
+    assert server.rerenders >= 1 and server.entries_per_row > 1e-4
+    assert len(server.done) == len(reqs) and server.used == 0
+    for r in reqs:
+        assert torch.equal(r.image, _alone(st, r)), f"request on scene {r.scene} {r.W}x{r.H} (batch {r.batch})"
