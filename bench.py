@@ -5,7 +5,8 @@ Workload (the headline configuration): N = 1,000,000 Gaussians in
 features = SH degree 3 ([N,3,16] ~ N(0,0.3)) of the view directions
 (compute_sh_points: direction and SH in one kernel; --unfused-sh computes the
 directions with torch and calls compute_sh) concatenated with the projected
-depth (C = 4), L2 loss against a U(0,1) target, backward to every
+depth (C = 4), L2 loss against a U(0,1) target (l2_loss: one fused launch
+each way; --torch-loss for the torch expression), backward to every
 Gaussian parameter.  A step is one forward + backward of that view; inputs
 are resident in HBM before the timed region.  The blending launch parameters
 are chosen by msplat_amd.autotune before the warm-up (--no-autotune keeps
@@ -46,7 +47,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_step(cfg, dev, fused_sh: bool = True):
+def build_step(cfg, dev, fused_sh: bool = True, fused_loss: bool = True):
     import torch
 
     import msplat_amd as ms
@@ -67,7 +68,7 @@ def build_step(cfg, dev, fused_sh: bool = True):
             rgb = ms.compute_sh(P["shs"], scenes.view_dirs(xyz, cam.eye))
         feats = torch.cat([rgb, depth], 1)
         out = ms.rasterization(xyz, P["scale"], P["rotate"], P["opacity"], feats, cam.intr, cam.extr, W, H, 0.0)
-        loss = ((out - target) ** 2).sum()
+        loss = ms.l2_loss(out, target) if fused_loss else ((out - target) ** 2).sum()
         loss.backward()
         for p in P.values():
             p.grad = None
@@ -334,6 +335,8 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a PMC calibration kernel)")
     ap.add_argument("--no-multiview", action="store_true")
+    ap.add_argument("--torch-loss", action="store_true",
+                    help="the L2 loss as torch ops instead of the fused l2_loss")
     ap.add_argument("--unfused-sh", action="store_true",
                     help="view directions in torch + compute_sh instead of compute_sh_points")
     ap.add_argument("--no-serving", action="store_true")
@@ -353,7 +356,7 @@ def main():
 
     dev = torch.device("cuda:0")
     cfg = scenes.config_sh3_1080p(n=args.n)
-    step, P, cam = build_step(cfg, dev, fused_sh=not args.unfused_sh)
+    step, P, cam = build_step(cfg, dev, fused_sh=not args.unfused_sh, fused_loss=not args.torch_loss)
     W, H = cam.W, cam.H
     C = 4
     log(f"workload: N={args.n} {W}x{H} C={C} (SH3 RGB + depth); warmup {args.warmup}, steps {args.steps}")
@@ -446,6 +449,8 @@ def main():
                                                                 ppl_bwd=cfg_used.ppl_bwd),
                             sh="torch view directions + compute_sh" if args.unfused_sh else
                             "compute_sh_points (view direction fused into the SH kernel)",
+                            loss="((out - target) ** 2).sum() in torch" if args.torch_loss else
+                            "l2_loss (one fused launch each way)",
                             parallelism="single GPU"),
                 roofline=roof, roofline_hbm=roof_hbm, cpu_baseline=cpu, multiview=multiview, batched_1080p=batched,
                 serving=serving_stats,

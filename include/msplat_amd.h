@@ -344,6 +344,20 @@ int msplat_rasterize_views_backward(
     const float *dL_drendered, float *dL_duv, float *dL_dconic,
     float *dL_dopacity, float *dL_dfeature, void *stream);
 
+/* ---- squared-error loss next to the rasterizer (no reference counterpart:
+ * the reference's training loops, tutorials/gs_2d.py:73-90, compute it with
+ * separate torch kernels).  msplat_sq_diff_sum writes sum_i (a_i - t_i)^2
+ * to out[0] in one launch (plus a one-word clear), independent of
+ * scheduling: fixed per-workgroup trees, the partials added in order in
+ * fp64.  Calls that share a workspace must not overlap.
+ * msplat_sq_diff_grad writes grad_i = 2 (a_i - t_i) dL[0]. */
+size_t msplat_sq_diff_sum_workspace_bytes(void);
+int msplat_sq_diff_sum(const float *a, const float *t, int64_t n,
+                       void *workspace, size_t workspace_bytes, float *out,
+                       void *stream);
+int msplat_sq_diff_grad(const float *a, const float *t, int64_t n,
+                        const float *dL, float *grad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

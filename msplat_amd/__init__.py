@@ -11,6 +11,7 @@ from .alpha_blending import alpha_blending
 from .compute_cov3d import compute_cov3d
 from .compute_sh import compute_sh, compute_sh_points
 from .ewa_project import ewa_project
+from .loss import l2_loss
 from .project_point import project_point
 from .rasterization import rasterization
 from .sort_gaussian import sort_gaussian
@@ -23,8 +24,10 @@ __all__ = [
     "compute_sh",
     "alpha_blending",
     "rasterization",
-    # extension: view direction + SH fused (no reference counterpart)
+    # extensions with no reference counterpart: view direction + SH fused,
+    # the training step's squared-error loss in one launch each way
     "compute_sh_points",
+    "l2_loss",
 ]
 
 __version__ = "0.1.0"

@@ -57,6 +57,9 @@ SIGNATURES = {
     "msplat_bin_emit": (I32, [P, P, P, I64, I32, I32, I32, P, I64, P, P, SZ, P, P, P]),
     "msplat_preprocess_views_forward": (I32, [P, P, P, I64, I32, P, P, P, P, P, P, F32, F32, P, P, P, P, P, P]),
     "msplat_preprocess_views_backward": (I32, [P, P, P, I64, I32, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "msplat_sq_diff_sum_workspace_bytes": (SZ, []),
+    "msplat_sq_diff_sum": (I32, [P, P, I64, P, SZ, P, P]),
+    "msplat_sq_diff_grad": (I32, [P, P, I64, P, P, P]),
     "msplat_bin_views_prepare_workspace_bytes": (SZ, [I64]),
     "msplat_bin_views_prepare": (I32, [P, P, P, I64, I32, P, P, P, P, I32, P, P, SZ, P, P]),
     "msplat_bin_views_emit_workspace_bytes": (SZ, [I64]),

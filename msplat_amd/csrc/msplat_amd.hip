@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "blend.hpp"
+#include "loss.hpp"
 #include "common.hpp"
 #include "preprocess.hpp"
 #include "sh.hpp"
@@ -509,6 +510,39 @@ BlendIn blend_inputs(const float *uv, const float *conic, const float *opacity, 
 }  // namespace
 
 extern "C" {
+
+// ---------------------------------------------------------------- loss
+
+size_t msplat_sq_diff_sum_workspace_bytes(void) { return align256(sizeof(float) * kLossBlocks) + 256; }
+
+int msplat_sq_diff_sum(const float *a, const float *t, int64_t n, void *workspace, size_t workspace_bytes,
+                       float *out, void *stream) {
+  REQUIRE(n >= 0, "sq_diff_sum: n = %lld", (long long)n);
+  REQUIRE(out, "sq_diff_sum: null output");
+  const size_t need = msplat_sq_diff_sum_workspace_bytes();
+  if (workspace_bytes < need)
+    return fail(MSPLAT_ERR_WORKSPACE, "sq_diff_sum: workspace %zu < %zu bytes", workspace_bytes, need);
+  REQUIRE(workspace && (n == 0 || (a && t)), "sq_diff_sum: null pointer");
+  hipStream_t s = as_stream(stream);
+  float *partial = static_cast<float *>(workspace);
+  uint32_t *ticket = reinterpret_cast<uint32_t *>(static_cast<char *>(workspace) + align256(sizeof(float) * kLossBlocks));
+  launch_fill_zero_words({{ticket, 1}, {out, 1}}, s);
+  if (n == 0) return launched("sq_diff_sum");
+  const int64_t want = (n + 4 * kLossThreads - 1) / (4 * kLossThreads);
+  const unsigned blocks = unsigned(want < kLossBlocks ? want : kLossBlocks);
+  sq_diff_sum_kernel<<<blocks, kLossThreads, 0, s>>>(a, t, n, partial, ticket, out);
+  return launched("sq_diff_sum");
+}
+
+int msplat_sq_diff_grad(const float *a, const float *t, int64_t n, const float *dL, float *grad, void *stream) {
+  REQUIRE(n >= 0, "sq_diff_grad: n = %lld", (long long)n);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(a && t && dL && grad, "sq_diff_grad: null pointer");
+  const int64_t want = (n + kLossThreads - 1) / kLossThreads;
+  const unsigned blocks = unsigned(want < 8192 ? want : 8192);
+  sq_diff_grad_kernel<<<blocks, kLossThreads, 0, as_stream(stream)>>>(a, t, n, dL, grad);
+  return launched("sq_diff_grad");
+}
 
 int msplat_version(void) { return 100; }
 
