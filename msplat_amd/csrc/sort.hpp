@@ -198,14 +198,17 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     uint32_t *__restrict__ status, uint32_t *__restrict__ error) {
   const int64_t len = dev_len(dlen, cap);
   int64_t tile = blockIdx.x;
+  // Workgroups past the data leave before taking a ticket: the grid is sized
+  // for the capacity, and every ticket is an atomic on one shared word.
+  // Exactly ceil(len / tile) workgroups take tickets, so every ticket names a
+  // tile with data, and no later tile waits on a workgroup that left.
+  if (tile * kRadixTile >= len) return;
   if constexpr (kOnesweep) {
     __shared__ uint32_t s_ticket;
     if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
     __syncthreads();
     tile = s_ticket;
   }
-  // whole workgroup past the data (no later tile holds data, so none waits for it)
-  if (tile * kRadixTile >= len) return;
   constexpr int kWaves = kThreads / kWave;
   constexpr int kPerWave = kRadixTile / kWaves;  // 1024 keys, contiguous
   __shared__ uint32_t s_key[kRadixTile];
@@ -232,25 +235,43 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     key[it] = i < len ? kin[i] : 0u;
     val[it] = i < len ? vin[i] : 0u;
   }
+  // Rank in two sweeps.  First the lanes holding an equal digit in each
+  // round of 64 keys, by bit-wise ballots (the rounds are independent, so
+  // their ballot chains overlap).  Then, round by round, the lowest lane of
+  // each digit group adds the group's size to the wave's running count of
+  // that digit with an LDS atomic whose return value -- the count of that
+  // digit in earlier rounds -- it hands to its group.  A wave's LDS
+  // operations execute in program order and each digit has one leader per
+  // round, so ranks follow key order (stable), with no fence or wait between
+  // rounds.
+  uint32_t dig[kItems];
+  uint64_t grp[kItems];
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const bool valid = base + it * kWave + lane < len;
     const uint32_t d = (key[it] >> shift) & mask;
     uint64_t peers = __ballot(valid);
-    for (int b = 0; b < bits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(valid && bit);
-      peers &= bit ? bb : ~bb;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      if (b < bits) {  // uniform
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
     }
+    dig[it] = d;
+    grp[it] = valid ? peers : 0ull;
+  }
+#pragma unroll
+  for (int it = 0; it < kItems; ++it) {
+    const uint64_t peers = grp[it];
     const uint32_t below =
         __builtin_amdgcn_mbcnt_hi(uint32_t(peers >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(peers), 0u));
-    // wave-synchronous LDS: all lanes read the running count, then the one
-    // leader per digit advances it; the wavefront-scope fences keep the
-    // compiler from reordering or reusing these accesses across rounds.
-    rnk[it] = valid ? s_run[wave][d] + below : 0xffffffffu;
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (valid && below == 0) s_run[wave][d] += uint32_t(__popcll(peers));
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    uint32_t before = 0u;
+    if (peers != 0ull && below == 0u) before = atomicAdd(&s_run[wave][dig[it]], uint32_t(__popcll(peers)));
+    const int leader = peers != 0ull ? __ffsll((unsigned long long)peers) - 1 : lane;
+    before = __shfl(before, leader, kWave);
+    rnk[it] = peers != 0ull ? before + below : 0xffffffffu;
   }
   __syncthreads();
   // digit totals -> tile-local starts (exclusive scan over digits) and wave offsets
