@@ -125,7 +125,7 @@ struct Carver {
 };
 
 struct PrepLayout {
-  size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, total;
+  size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, area_s, rect_s, total;
 };
 
 // (W, H are part of the size query so that the layout may depend on the
@@ -149,6 +149,8 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
   L.status = c.take(size_t(sortk::status_words(n)) * 4);
   L.flag = c.take(4);
   L.nlisted = c.take(8);
+  L.area_s = c.take(un * 4);  // cell counts and tile rectangles in depth order (cell binning)
+  L.rect_s = c.take(un * 8);
   (void)W;
   (void)H;
   L.total = c.off;
@@ -285,17 +287,28 @@ void sh_bwd_launch(int64_t n, int C, const float *shs, const Dir &dir, const uin
 // -- and the exclusive scan of the per-Gaussian box counts in that order
 // (total -> *d_count).
 void depth_order(void *ws, const PrepLayout &L, int64_t n, int64_t *d_count, hipStream_t s,
-                 const int64_t *dlen = nullptr) {
+                 const int64_t *dlen = nullptr, const uint2 *rect = nullptr, int S = 1) {
   uint32_t *ka = at<uint32_t>(ws, L.ka), *va = at<uint32_t>(ws, L.va);
   uint32_t *kb = at<uint32_t>(ws, L.kb), *vb = at<uint32_t>(ws, L.vb);
   const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
                                at<uint32_t>(ws, L.status)};
+  int32_t *area = at<int32_t>(ws, L.area);
+  if (rect) {  // cell binning: carry cell counts and rectangles into depth order, scan them in place
+    const sortk::RadixGather gather{rect, S, at<int32_t>(ws, L.area_s), at<uint2>(ws, L.rect_s)};
+    {
+      Timed timed_("depth_sort", s);
+      sortk::radix_sort(ka, va, kb, vb, n, 32, 4, rb, s, dlen, nullptr, gather);  // a -> b -> a -> b -> a
+    }
+    sortk::exclusive_scan<int32_t, int64_t>(gather.oarea, nullptr, n, at<int64_t>(ws, L.sums),
+                                            at<int64_t>(ws, L.offs), d_count, s, dlen);
+    return;
+  }
   {
     Timed timed_("depth_sort", s);
     sortk::radix_sort(ka, va, kb, vb, n, 32, 4, rb, s, dlen);  // a -> b -> a -> b -> a
   }
-  sortk::exclusive_scan<int32_t, int64_t>(at<int32_t>(ws, L.area), va, n, at<int64_t>(ws, L.sums),
-                                          at<int64_t>(ws, L.offs), d_count, s);
+  sortk::exclusive_scan<int32_t, int64_t>(area, va, n, at<int64_t>(ws, L.sums), at<int64_t>(ws, L.offs), d_count,
+                                          s, dlen);
 }
 
 // Emits the (class, id) pairs of every entry in depth order -- classes are
@@ -315,7 +328,8 @@ const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t
   if (direct) {  // a few entries per Gaussian: one thread each
     Timed timed_("emit", s);
     sortk::emit_direct_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
-        uv, depth, radius, order, offs, at<int32_t>(prepared, P.area), n, k, dlen, vw, S, ca, ga);
+        order, offs, at<int32_t>(prepared, P.area_s), at<uint2>(prepared, P.rect_s), n,
+        at<int64_t>(prepared, P.nlisted), k, dlen, vw, S, ca, ga);
   } else {  // equal output chunks per workgroup (tile rectangles of thousands of tiles)
     sortk::emit_owner_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
         at<int32_t>(prepared, P.area), order, offs, n, sortk::emit_chunks(k), owner);
@@ -899,7 +913,7 @@ int msplat_bin_views_prepare(const float *uv, const float *depth, const int32_t 
     sortk::exclusive_scan<uint32_t, uint32_t>(listed, nullptr, m, at<int64_t>(workspace, L.sums), pos, nlisted, s);
     sortk::partition_kernel<<<grid_1d(m, sortk::kThreads), sortk::kThreads, 0, s>>>(
         depth, area, pos, m, nlisted, at<uint32_t>(workspace, L.ka), at<uint32_t>(workspace, L.va));
-    depth_order(workspace, L, m, d_count, s, nlisted);
+    depth_order(workspace, L, m, d_count, s, nlisted, reinterpret_cast<const uint2 *>(rect), cell);
   }
   sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, d_count, m);
   return launched("bin_prepare");

@@ -316,11 +316,14 @@ MSPLAT_DEV void reduce_camera(float (&cam)[16], float *dintr, float *dextr) {
   __shared__ float red[(kGeomThreads / kWave) * 16];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
+  // one transposed wave reduction of the 16 partials: lane 16 r + j (j < 4)
+  // then holds the wave total of value 4 j + kRowValue[r]
+  float tot[4];
+  wave_sum_transposed<16>(cam, tot);
+  const int row = lane >> 4, j = lane & 15;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const float s = wave_sum(cam[i]);
-    if (lane == 0) red[wave * 16 + i] = s;
-  }
+  for (int jj = 0; jj < 4; ++jj)
+    if (j == jj) red[wave * 16 + 4 * jj + row_value(row)] = tot[jj];
   __syncthreads();
   if (threadIdx.x < 16) {
     float s = 0.f;
@@ -332,6 +335,7 @@ MSPLAT_DEV void reduce_camera(float (&cam)[16], float *dintr, float *dextr) {
       unsafeAtomicAdd(dextr + (threadIdx.x - 4), s);
     }
   }
+  __syncthreads();  // red is reused by the next call (the many-view adjoint reduces once per view)
 }
 
 MSPLAT_DEV float3 ld3(const float *p, int64_t i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }

@@ -81,10 +81,17 @@ MSPLAT_DEV int64_t block_excl_scan(int64_t v, int64_t *total) {
   return before + incl - v;
 }
 
+// Elements at or past *dlen (when given) count as 0 and are not loaded.
 template <typename In>
 __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const In *__restrict__ in,
-                                                               const uint32_t *__restrict__ perm, int64_t len,
+                                                               const uint32_t *__restrict__ perm, int64_t cap,
+                                                               const int64_t *__restrict__ dlen,
                                                                int64_t *__restrict__ sums) {
+  const int64_t len = dev_len(dlen, cap);
+  if (int64_t(blockIdx.x) * kScanTile >= len) {  // uniform: nothing to add
+    if (threadIdx.x == 0) sums[blockIdx.x] = 0;
+    return;
+  }
   const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
   int64_t s = 0;
 #pragma unroll
@@ -110,11 +117,15 @@ __global__ __launch_bounds__(kThreads) void scan_partials_kernel(int64_t *__rest
   if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
+// Writes the first *dlen (or cap) outputs.
 template <typename In, typename Out>
 __global__ __launch_bounds__(kThreads) void scan_apply_kernel(const In *__restrict__ in,
-                                                              const uint32_t *__restrict__ perm, int64_t len,
+                                                              const uint32_t *__restrict__ perm, int64_t cap,
+                                                              const int64_t *__restrict__ dlen,
                                                               const int64_t *__restrict__ sums,
                                                               Out *__restrict__ out) {
+  const int64_t len = dev_len(dlen, cap);
+  if (int64_t(blockIdx.x) * kScanTile >= len) return;  // uniform
   const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
   int64_t vals[kScanItems];
   int64_t s = 0;
@@ -132,15 +143,16 @@ __global__ __launch_bounds__(kThreads) void scan_apply_kernel(const In *__restri
   }
 }
 
-// sums must hold scan_tiles(len) entries.
+// sums must hold scan_tiles(len) entries.  With dlen, only the first *dlen
+// (<= len) elements are scanned (the rest count as 0) and written.
 template <typename In, typename Out>
 inline void exclusive_scan(const In *in, const uint32_t *perm, int64_t len, int64_t *sums, Out *out,
-                           int64_t *total, hipStream_t s) {
+                           int64_t *total, hipStream_t s, const int64_t *dlen = nullptr) {
   const int64_t nb = scan_tiles(len);
   if (nb == 0) return;
-  scan_reduce_kernel<In><<<dim3(unsigned(nb)), dim3(kThreads), 0, s>>>(in, perm, len, sums);
+  scan_reduce_kernel<In><<<dim3(unsigned(nb)), dim3(kThreads), 0, s>>>(in, perm, len, dlen, sums);
   scan_partials_kernel<<<dim3(1), dim3(kThreads), 0, s>>>(sums, nb, total);
-  scan_apply_kernel<In, Out><<<dim3(unsigned(nb)), dim3(kThreads), 0, s>>>(in, perm, len, sums, out);
+  scan_apply_kernel<In, Out><<<dim3(unsigned(nb)), dim3(kThreads), 0, s>>>(in, perm, len, dlen, sums, out);
 }
 
 // ---- stable LSD radix sort of (uint32 key, uint32 value) pairs --------------
@@ -185,6 +197,22 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint32_t *__
 // until one of them carries its inclusive prefix.  A tile publishes its counts
 // before it looks back, so every wait ends; each wait is also bounded and sets
 // *error if it ever runs out.
+// Optional records carried into the sorted order by a pass (cell binning):
+// orect[p] = rect[value at p], the packed tile rectangle of that row, and
+// oarea[p] = the number of cells of S x S tiles it touches (null: none).
+struct RadixGather {
+  const uint2 *rect = nullptr;
+  int S = 1;
+  int32_t *oarea = nullptr;
+  uint2 *orect = nullptr;
+};
+
+// Cells of S x S tiles touched by a non-empty packed tile rectangle.
+MSPLAT_HD int rect_cells(uint2 r, int S) {
+  const int x0 = int(r.x & 0xffffu), y0 = int(r.x >> 16), x1 = int(r.y & 0xffffu), y1 = int(r.y >> 16);
+  return ((x1 + S - 1) / S - x0 / S) * ((y1 + S - 1) / S - y0 / S);
+}
+
 constexpr int kMaxPasses = 4;
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kValueMask = (1u << 30) - 1u;
 constexpr int64_t kLookbackSpins = int64_t(1) << 16;
@@ -195,7 +223,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t cap,
     const int64_t *__restrict__ dlen, int shift, int bits, const uint32_t *__restrict__ offsets, int64_t ntiles,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t *__restrict__ ticket,
-    uint32_t *__restrict__ status, uint32_t *__restrict__ error) {
+    uint32_t *__restrict__ status, uint32_t *__restrict__ error, RadixGather gather) {
   const int64_t len = dev_len(dlen, cap);
   int64_t tile = blockIdx.x;
   // Workgroups past the data leave before taking a ticket: the grid is sized
@@ -354,8 +382,14 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     const uint32_t k = s_key[e];
     const uint32_t d = (k >> shift) & mask;
     const uint32_t pos = s_gstart[d] + (uint32_t(e) - s_lstart[d]);
+    const uint32_t v = s_val[e];
     kout[pos] = k;
-    vout[pos] = s_val[e];
+    vout[pos] = v;
+    if (gather.oarea) {  // uniform: per-value records, moved into the sorted order
+      const uint2 r = gather.rect[v];
+      gather.orect[pos] = r;
+      gather.oarea[pos] = rect_cells(r, gather.S);
+    }
   }
 }
 
@@ -405,14 +439,14 @@ __global__ __launch_bounds__(kThreads) void onesweep_hist_kernel(const uint32_t 
 // scatter launches.
 inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t cap,
                        int shift, int bits, const RadixBuffers &rb, hipStream_t s,
-                       const int64_t *dlen = nullptr) {
+                       const int64_t *dlen = nullptr, RadixGather gather = {}) {
   const int64_t nt = radix_tiles(cap);
   if (nt == 0) return;
   radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, cap, dlen, shift, bits, rb.hist, nt);
   const int64_t hlen = int64_t(1 << bits) * nt;
   exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
   radix_scatter_tile_kernel<false><<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(
-      kin, vin, cap, dlen, shift, bits, rb.hist, nt, kout, vout, nullptr, nullptr, nullptr);
+      kin, vin, cap, dlen, shift, bits, rb.hist, nt, kout, vout, nullptr, nullptr, nullptr, gather);
 }
 
 // Stable LSD sort of `cap` (or *dlen) pairs over `bits` key bits in npass
@@ -423,7 +457,7 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
 // pass instead of the other buffer.
 inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, int64_t cap, int bits, int npass,
                       const RadixBuffers &rb, hipStream_t s, const int64_t *dlen = nullptr,
-                      uint32_t *vlast = nullptr) {
+                      uint32_t *vlast = nullptr, RadixGather gather = {}) {
   const int db = (bits + npass - 1) / npass;
   const int64_t nt = radix_tiles(cap);
   const bool onesweep = cap < (int64_t(1) << 30) && npass <= kMaxPasses;
@@ -441,13 +475,14 @@ inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, in
     const int pb = bits - shift < db ? bits - shift : db;
     uint32_t *dk = in_b ? ka : kb;
     uint32_t *dv = (p == npass - 1 && vlast) ? vlast : (in_b ? va : vb);
+    const RadixGather g = p == npass - 1 ? gather : RadixGather{};
     if (nt > 0) {
       if (onesweep)
         radix_scatter_tile_kernel<true><<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(
             sk, sv, cap, dlen, shift, pb, rb.ghist + p * 256, nt, dk, dv, rb.ghist + kMaxPasses * 256 + p,
-            rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses);
+            rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses, g);
       else
-        radix_pass(sk, sv, dk, dv, cap, shift, pb, rb, s, dlen);
+        radix_pass(sk, sv, dk, dv, cap, shift, pb, rb, s, dlen, g);
     }
     sk = dk;
     sv = dv;
@@ -499,7 +534,7 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(const float *__re
   TileRect t{0, 0, 0, 0};
   if (r > 0 && !(db >> 31)) {
     t = tile_rect(uv[2 * i], uv[2 * i + 1], r, v.gx[b], v.gy[b]);
-    if (t.area() > 0) a = ((t.x1 + S - 1) / S - t.x0 / S) * ((t.y1 + S - 1) / S - t.y0 / S);
+    if (t.area() > 0) a = rect_cells(pack_rect(t), S);
   }
   area[i] = a;
   listed[i] = a > 0 ? 1u : 0u;
@@ -528,30 +563,62 @@ __global__ __launch_bounds__(kThreads) void partition_kernel(const float *__rest
 // depth-ordered position i writes its Gaussian's cells, row-major, at slots
 // offs[i]... (slots past the capacity are not written).
 __global__ __launch_bounds__(kThreads) void emit_direct_kernel(
-    const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius,
-    const uint32_t *__restrict__ order, const int64_t *__restrict__ offs, const int32_t *__restrict__ area,
-    int64_t n, int64_t cap, const int64_t *__restrict__ dlen, Views v, int S, uint32_t *__restrict__ cls,
-    uint32_t *__restrict__ gid) {
+    const uint32_t *__restrict__ order, const int64_t *__restrict__ offs, const int32_t *__restrict__ area_s,
+    const uint2 *__restrict__ rect_s, int64_t n, const int64_t *__restrict__ nrows, int64_t cap,
+    const int64_t *__restrict__ dlen, Views v, int S, uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
+  // Row i of the depth order: its Gaussian row order[i], cell count area_s[i]
+  // and tile rectangle rect_s[i] (both carried into depth order by the depth
+  // sort's last pass, so every read here is coalesced).  Listed rows have
+  // positive depth, so every entry has a cell class.
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t g = order[i];
-  const int a = area[g];
-  if (a == 0) return;
+  const int64_t rows = dev_len(nrows, n);  // rows past the listed ones have nothing to emit
+  if (int64_t(blockIdx.x) * kThreads >= rows) return;  // uniform
+  const int lane = threadIdx.x % kWave;
+  uint32_t g = 0u, base = 0u;
+  int a = 0, w = 1, ncx = 1;
+  int64_t o = 0;
   const int64_t k = dev_len(dlen, cap);
-  const int64_t o = offs[i];
-  const int b = view_of_row(v, g);
-  const TileRect t = tile_rect(uv[2 * int64_t(g)], uv[2 * int64_t(g) + 1], radius[g], v.gx[b], v.gy[b]);
-  const int x0 = t.x0 / S, x1 = (t.x1 + S - 1) / S;
-  const int w = imax(1, x1 - x0), ncx = v.ncx[b];
-  // depth sign bit set: the reference key's upper word is all ones (no tile)
-  const uint32_t base = (fbits(depth[g]) >> 31) ? 0u : uint32_t(v.cell0[b] + (t.y0 / S) * ncx + x0) + 1u;
-  int q = 0, r = 0;
-  for (int e = 0; e < a && o + e < k; ++e) {
-    cls[o + e] = base ? base + uint32_t(q * ncx + r) : 0u;
-    gid[o + e] = g;
-    if (++r == w) {
-      r = 0;
-      ++q;
+  if (i < rows) {
+    a = area_s[i];
+    if (a > 0) {
+      g = order[i];
+      o = offs[i];
+      const int b = view_of_row(v, g);
+      const uint2 rp = rect_s[i];
+      const int x0 = int(rp.x & 0xffffu), y0 = int(rp.x >> 16), x1 = int(rp.y & 0xffffu);
+      const int cx0 = x0 / S;
+      w = imax(1, (x1 + S - 1) / S - cx0);
+      ncx = v.ncx[b];
+      base = uint32_t(v.cell0[b] + (y0 / S) * ncx + cx0) + 1u;
+    }
+  }
+  // A Gaussian's entries: the cells of its box, row by row, from offset o.
+  // Small boxes are written by their own thread; large ones by the whole
+  // wave, one at a time (a thread alone would hold up its wave and the grid).
+  constexpr int kSmall = 16;
+  if (a > 0 && a <= kSmall) {
+    int q = 0, r = 0;
+    for (int e = 0; e < a && o + e < k; ++e) {
+      cls[o + e] = base ? base + uint32_t(q * ncx + r) : 0u;
+      gid[o + e] = g;
+      if (++r == w) {
+        r = 0;
+        ++q;
+      }
+    }
+  }
+  uint64_t big = __ballot(a > kSmall);
+  while (big) {
+    const int src = __ffsll((unsigned long long)big) - 1;
+    big &= big - 1;
+    const int sa = __shfl(a, src, kWave), sw = __shfl(w, src, kWave), sn = __shfl(ncx, src, kWave);
+    const uint32_t sg = uint32_t(__shfl(int(g), src, kWave)), sb = uint32_t(__shfl(int(base), src, kWave));
+    const int64_t so = (int64_t(uint32_t(__shfl(int(uint32_t(o >> 32)), src, kWave))) << 32) |
+                       int64_t(uint32_t(__shfl(int(uint32_t(o)), src, kWave)));
+    for (int e = lane; e < sa && so + e < k; e += kWave) {
+      const int q = e / sw, r = e - q * sw;
+      cls[so + e] = sb ? sb + uint32_t(q * sn + r) : 0u;
+      gid[so + e] = sg;
     }
   }
 }
