@@ -2,10 +2,9 @@
 
 Workload (the headline configuration): N = 1,000,000 Gaussians in
 [-3,3]^3, one 1920x1080 view (fx = fy = 1.171875 W, camera at distance 4),
-features = SH degree 3 ([N,3,16] ~ N(0,0.3)) of the view directions
-(compute_sh_points: direction and SH in one kernel; --unfused-sh computes the
-directions with torch and calls compute_sh) concatenated with the projected
-depth (C = 4), L2 loss against a U(0,1) target (l2_loss: one fused launch
+features = SH degree 3 ([N,3,16] ~ N(0,0.3)) of the view directions plus
+the camera-space depth (C = 4), rendered by rasterization_sh (one op; the
+composition of separate ops is --features ops / torch), L2 loss against a U(0,1) target (l2_loss: one fused launch
 each way; --torch-loss for the torch expression), backward to every
 Gaussian parameter.  A step is one forward + backward of that view; inputs
 are resident in HBM before the timed region.  The blending launch parameters
@@ -47,7 +46,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_step(cfg, dev, fused_sh: bool = True, fused_loss: bool = True):
+def build_step(cfg, dev, features: str = "fused", fused_loss: bool = True):
     import torch
 
     import msplat_amd as ms
@@ -61,13 +60,18 @@ def build_step(cfg, dev, fused_sh: bool = True, fused_loss: bool = True):
 
     def step():
         xyz = P["xyz"]
-        _, depth = ms.project_point(xyz, cam.intr, cam.extr, W, H)
-        if fused_sh:  # view direction + SH in one kernel
-            rgb = ms.compute_sh_points(P["shs"], xyz, cam.eye)
-        else:  # the user-level composition: torch view directions, then compute_sh
-            rgb = ms.compute_sh(P["shs"], scenes.view_dirs(xyz, cam.eye))
-        feats = torch.cat([rgb, depth], 1)
-        out = ms.rasterization(xyz, P["scale"], P["rotate"], P["opacity"], feats, cam.intr, cam.extr, W, H, 0.0)
+        if features == "fused":  # SH colours + depth rendered by one op (SH on a side stream)
+            out = ms.rasterization_sh(xyz, P["scale"], P["rotate"], P["opacity"], P["shs"], cam.intr, cam.extr,
+                                      cam.eye, W, H, 0.0, depth_channel=True)
+        else:
+            _, depth = ms.project_point(xyz, cam.intr, cam.extr, W, H)
+            if features == "ops":  # view direction + SH in one kernel, then cat + rasterization
+                rgb = ms.compute_sh_points(P["shs"], xyz, cam.eye)
+            else:  # the user-level composition: torch view directions, then compute_sh
+                rgb = ms.compute_sh(P["shs"], scenes.view_dirs(xyz, cam.eye))
+            feats = torch.cat([rgb, depth], 1)
+            out = ms.rasterization(xyz, P["scale"], P["rotate"], P["opacity"], feats, cam.intr, cam.extr, W, H,
+                                   0.0)
         loss = ms.l2_loss(out, target) if fused_loss else ((out - target) ** 2).sum()
         loss.backward()
         for p in P.values():
@@ -337,8 +341,9 @@ def main():
     ap.add_argument("--no-multiview", action="store_true")
     ap.add_argument("--torch-loss", action="store_true",
                     help="the L2 loss as torch ops instead of the fused l2_loss")
-    ap.add_argument("--unfused-sh", action="store_true",
-                    help="view directions in torch + compute_sh instead of compute_sh_points")
+    ap.add_argument("--features", choices=("fused", "ops", "torch"), default="fused",
+                    help="fused: rasterization_sh renders SH colours + depth in one op; ops: compute_sh_points + "
+                         "project_point + cat + rasterization; torch: view directions in torch + compute_sh + ...")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -356,7 +361,7 @@ def main():
 
     dev = torch.device("cuda:0")
     cfg = scenes.config_sh3_1080p(n=args.n)
-    step, P, cam = build_step(cfg, dev, fused_sh=not args.unfused_sh, fused_loss=not args.torch_loss)
+    step, P, cam = build_step(cfg, dev, features=args.features, fused_loss=not args.torch_loss)
     W, H = cam.W, cam.H
     C = 4
     log(f"workload: N={args.n} {W}x{H} C={C} (SH3 RGB + depth); warmup {args.warmup}, steps {args.steps}")
@@ -447,8 +452,10 @@ def main():
                             gaussians=args.n, width=W, height=H, channels=C, intersections=K,
                             cell_entries=w["count"], blend=dict(cell=cfg_used.cell, ppl_fwd=cfg_used.ppl_fwd,
                                                                 ppl_bwd=cfg_used.ppl_bwd),
-                            sh="torch view directions + compute_sh" if args.unfused_sh else
-                            "compute_sh_points (view direction fused into the SH kernel)",
+                            features={"fused": "rasterization_sh (SH colours + depth in one op, SH on a side stream)",
+                                      "ops": "compute_sh_points + project_point + cat + rasterization",
+                                      "torch": "torch view directions + compute_sh + project_point + cat + "
+                                               "rasterization"}[args.features],
                             loss="((out - target) ** 2).sum() in torch" if args.torch_loss else
                             "l2_loss (one fused launch each way)",
                             parallelism="single GPU"),

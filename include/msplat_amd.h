@@ -131,18 +131,21 @@ int msplat_compute_sh_backward(const float *shs, const float *dirs,
  * sqrtf((vx vx + vy vy) + vz vz).  No counterpart in the reference, whose
  * compute_sh takes the directions (compute_sh.py:8-33) and leaves their
  * computation to user code; this removes that torch code and its adjoint.
- * campos is [3] in device memory.  The adjoint writes dL/dshs and dL/dxyz
- * (through the normalisation); dL/dcampos = -sum_i dL/dxyz_i. */
+ * campos is [3] in device memory.  value row i starts at value + i * ld_value
+ * (ld_value >= C: the SH channels can be written straight into a wider
+ * feature buffer), and likewise dL_dvalue.  The adjoint writes dL/dshs and
+ * dL/dxyz (through the normalisation); dL/dcampos = -sum_i dL/dxyz_i. */
 int msplat_compute_sh_points_forward(const float *shs, const float *xyz,
                                      const float *campos,
                                      const uint8_t *visible, int64_t n,
-                                     int C, int D, float *value, void *stream);
+                                     int C, int D, float *value, int ld_value,
+                                     void *stream);
 int msplat_compute_sh_points_backward(const float *shs, const float *xyz,
                                       const float *campos,
                                       const uint8_t *visible,
-                                      const float *dL_dvalue, int64_t n, int C,
-                                      int D, float *dL_dshs, float *dL_dxyz,
-                                      void *stream);
+                                      const float *dL_dvalue, int ld_dvalue,
+                                      int64_t n, int C, int D, float *dL_dshs,
+                                      float *dL_dxyz, void *stream);
 
 /* ---- kernel timing (no reference counterpart; for benchmarks).  While
  * enabled, the main kernel of each entry point is bracketed by HIP events on

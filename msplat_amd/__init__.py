@@ -14,6 +14,7 @@ from .ewa_project import ewa_project
 from .loss import l2_loss
 from .project_point import project_point
 from .rasterization import rasterization
+from .rasterization_sh import rasterization_sh
 from .sort_gaussian import sort_gaussian
 
 __all__ = [
@@ -28,6 +29,7 @@ __all__ = [
     # the training step's squared-error loss in one launch each way
     "compute_sh_points",
     "l2_loss",
+    "rasterization_sh",
 ]
 
 __version__ = "0.1.0"

@@ -69,7 +69,7 @@ class _ComputeSHPoints(torch.autograd.Function):
         value = torch.empty((n, C), dtype=torch.float32, device=s.device)
         with torch.cuda.device(s.device):
             _lib.call("msplat_compute_sh_points_forward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(c),
-                      _lib.ptr(_lib.u8("visible", visible)), n, C, D, _lib.ptr(value), _lib.stream_ptr())
+                      _lib.ptr(_lib.u8("visible", visible)), n, C, D, _lib.ptr(value), C, _lib.stream_ptr())
         ctx.save_for_backward(shs, xyz, campos, visible)
         return value
 
@@ -83,7 +83,7 @@ class _ComputeSHPoints(torch.autograd.Function):
         with torch.cuda.device(s.device):
             _lib.call("msplat_compute_sh_points_backward", _lib.ptr(s), _lib.ptr(_lib.f32("xyz", xyz)),
                       _lib.ptr(_lib.f32("campos", campos).reshape(3)), _lib.ptr(_lib.u8("visible", visible)),
-                      _lib.ptr(_lib.f32("dL_dvalue", dL_dvalue)), n, C, D, _lib.ptr(dshs), _lib.ptr(dxyz),
+                      _lib.ptr(_lib.f32("dL_dvalue", dL_dvalue)), C, n, C, D, _lib.ptr(dshs), _lib.ptr(dxyz),
                       _lib.stream_ptr())
         dcampos = None
         if ctx.needs_input_grad[2]:

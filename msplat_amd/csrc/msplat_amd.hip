@@ -233,38 +233,39 @@ ShGeom sh_geom(int C, int D) {
 }
 
 template <int L, class Dir>
-void sh_fwd_launch(int64_t n, int C, const float *shs, const Dir &dir, const uint8_t *vis, float *val,
+void sh_fwd_launch(int64_t n, int C, const float *shs, const Dir &dir, const uint8_t *vis, float *val, int ld,
                    hipStream_t s) {
   const ShGeom g = sh_geom(C, (L + 1) * (L + 1));
   Timed timed_("sh_fwd", s);
   if constexpr (L <= kShGcMaxDeg) {
     if (C <= kWave && aligned16(shs)) {
-      sh_fwd_gc_kernel<L, Dir><<<grid_1d(n * C, 256), 256, 0, s>>>(n, C, shs, dir, vis, val);
+      sh_fwd_gc_kernel<L, Dir><<<grid_1d(n * C, 256), 256, 0, s>>>(n, C, shs, dir, vis, val, ld);
       return;
     }
   }
   if (g.G)
-    sh_fwd_lds_kernel<L, Dir><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dir, vis, val, g);
+    sh_fwd_lds_kernel<L, Dir><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dir, vis, val, ld,
+                                                                                   g);
   else
-    sh_fwd_kernel<L, Dir><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dir, vis, val);
+    sh_fwd_kernel<L, Dir><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dir, vis, val, ld);
 }
 template <int L, class Dir>
 void sh_bwd_launch(int64_t n, int C, const float *shs, const Dir &dir, const uint8_t *vis, const float *dval,
-                   float *dshs, hipStream_t s) {
+                   int ld, float *dshs, hipStream_t s) {
   const ShGeom g = sh_geom(C, (L + 1) * (L + 1));
   Timed timed_("sh_bwd", s);
   if constexpr (L <= kShGcMaxDeg) {
     if (C <= kWave && aligned16(shs) && aligned16(dshs)) {
       const int64_t waves = (n + kWave / C - 1) / (kWave / C);
-      sh_bwd_gc_kernel<L, Dir><<<grid_1d(waves * kWave, 256), 256, 0, s>>>(n, C, shs, dir, vis, dval, dshs);
+      sh_bwd_gc_kernel<L, Dir><<<grid_1d(waves * kWave, 256), 256, 0, s>>>(n, C, shs, dir, vis, dval, ld, dshs);
       return;
     }
   }
   if (g.G)
     sh_bwd_lds_kernel<L, Dir><<<grid_1d(n, g.G), g.G, size_t(g.G) * g.S * 4, s>>>(n, C, shs, dir, vis, dval,
-                                                                                   dshs, g);
+                                                                                   ld, dshs, g);
   else
-    sh_bwd_kernel<L, Dir><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dir, vis, dval, dshs);
+    sh_bwd_kernel<L, Dir><<<grid_1d(n, kShThreads), kShThreads, 0, s>>>(n, C, shs, dir, vis, dval, ld, dshs);
 }
 
 #define MSPLAT_SH_DISPATCH(L, CALL) \
@@ -729,7 +730,7 @@ int msplat_compute_sh_forward(const float *shs, const float *dirs, const uint8_t
   if (n == 0 || C == 0) return MSPLAT_OK;
   REQUIRE(shs && dirs && value, "compute_sh_forward: null pointer");
   hipStream_t s = as_stream(stream);
-#define CALL(LL) sh_fwd_launch<LL>(n, C, shs, ShDirArray{dirs, nullptr}, visible, value, s)
+#define CALL(LL) sh_fwd_launch<LL>(n, C, shs, ShDirArray{dirs, nullptr}, visible, value, C, s)
   MSPLAT_SH_DISPATCH(L, CALL)
 #undef CALL
   return launched("compute_sh_forward");
@@ -748,30 +749,31 @@ int msplat_compute_sh_backward(const float *shs, const float *dirs, const uint8_
     return launched("compute_sh_backward");
   }
   REQUIRE(shs && dirs && dL_dvalue && dL_dshs && dL_ddirs, "compute_sh_backward: null pointer");
-#define CALL(LL) sh_bwd_launch<LL>(n, C, shs, ShDirArray{dirs, dL_ddirs}, visible, dL_dvalue, dL_dshs, s)
+#define CALL(LL) sh_bwd_launch<LL>(n, C, shs, ShDirArray{dirs, dL_ddirs}, visible, dL_dvalue, C, dL_dshs, s)
   MSPLAT_SH_DISPATCH(L, CALL)
 #undef CALL
   return launched("compute_sh_backward");
 }
 
 int msplat_compute_sh_points_forward(const float *shs, const float *xyz, const float *campos, const uint8_t *visible,
-                                     int64_t n, int C, int D, float *value, void *stream) {
+                                     int64_t n, int C, int D, float *value, int ld_value, void *stream) {
   REQUIRE(n >= 0 && C >= 0, "compute_sh_points_forward: n = %lld, C = %d", (long long)n, C);
   const int L = sh_degree(D);
   REQUIRE(L >= 0, "compute_sh_points_forward: D = %d is not (deg+1)^2 with deg <= 10", D);
   if (n == 0 || C == 0) return MSPLAT_OK;
   REQUIRE(shs && xyz && campos && value, "compute_sh_points_forward: null pointer");
+  REQUIRE(ld_value >= C, "compute_sh_points_forward: ld_value = %d < C = %d", ld_value, C);
   hipStream_t s = as_stream(stream);
   const ShDirFromPoints dir{xyz, campos, nullptr};
-#define CALL(LL) sh_fwd_launch<LL>(n, C, shs, dir, visible, value, s)
+#define CALL(LL) sh_fwd_launch<LL>(n, C, shs, dir, visible, value, ld_value, s)
   MSPLAT_SH_DISPATCH(L, CALL)
 #undef CALL
   return launched("compute_sh_points_forward");
 }
 
 int msplat_compute_sh_points_backward(const float *shs, const float *xyz, const float *campos,
-                                      const uint8_t *visible, const float *dL_dvalue, int64_t n, int C, int D,
-                                      float *dL_dshs, float *dL_dxyz, void *stream) {
+                                      const uint8_t *visible, const float *dL_dvalue, int ld_dvalue, int64_t n,
+                                      int C, int D, float *dL_dshs, float *dL_dxyz, void *stream) {
   REQUIRE(n >= 0 && C >= 0, "compute_sh_points_backward: n = %lld, C = %d", (long long)n, C);
   const int L = sh_degree(D);
   REQUIRE(L >= 0, "compute_sh_points_backward: D = %d is not (deg+1)^2 with deg <= 10", D);
@@ -782,8 +784,9 @@ int msplat_compute_sh_points_backward(const float *shs, const float *xyz, const 
     return launched("compute_sh_points_backward");
   }
   REQUIRE(shs && xyz && campos && dL_dvalue && dL_dshs && dL_dxyz, "compute_sh_points_backward: null pointer");
+  REQUIRE(ld_dvalue >= C, "compute_sh_points_backward: ld_dvalue = %d < C = %d", ld_dvalue, C);
   const ShDirFromPoints dir{xyz, campos, dL_dxyz};
-#define CALL(LL) sh_bwd_launch<LL>(n, C, shs, dir, visible, dL_dvalue, dL_dshs, s)
+#define CALL(LL) sh_bwd_launch<LL>(n, C, shs, dir, visible, dL_dvalue, ld_dvalue, dL_dshs, s)
   MSPLAT_SH_DISPATCH(L, CALL)
 #undef CALL
   return launched("compute_sh_points_backward");

@@ -263,7 +263,7 @@ template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_fwd_lds_kernel(int64_t n, int C, const float *__restrict__ shs,
                                                          const Dir dir,
                                                          const uint8_t *__restrict__ vis,
-                                                         float *__restrict__ value, ShGeom sg) {
+                                                         float *__restrict__ value, int ld, ShGeom sg) {
   constexpr int D = (L + 1) * (L + 1);
   extern __shared__ float lds[];
   const int64_t g0 = int64_t(blockIdx.x) * sg.G;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void sh_fwd_lds_kernel(int64_t n, int C, const
   const int t = threadIdx.x;
   if (t >= rows) return;
   const int64_t i = g0 + t;
-  float *out = value + i * C;
+  float *out = value + i * ld;
   if (vis && !vis[i]) {
     for (int c = 0; c < C; ++c) out[c] = 0.f;
     return;
@@ -295,7 +295,7 @@ template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_bwd_lds_kernel(int64_t n, int C, const float *__restrict__ shs,
                                                          const Dir dir,
                                                          const uint8_t *__restrict__ vis,
-                                                         const float *__restrict__ dval,
+                                                         const float *__restrict__ dval, int ld,
                                                          float *__restrict__ dshs, ShGeom sg) {
   constexpr int D = (L + 1) * (L + 1);
   extern __shared__ float lds[];
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void sh_bwd_lds_kernel(int64_t n, int C, const
     if (live) {
       float x, y, z;
       dir.load(i, x, y, z);
-      const float *g = dval + i * C;
+      const float *g = dval + i * ld;
       sh_dir_grad<L>(x, y, z, row, g, C, gx, gy, gz);
       // the row is consumed: overwrite it with dL/dshs = Y_k g_c
       float Y[D];
@@ -381,13 +381,14 @@ template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_fwd_gc_kernel(int64_t n, int C, const float *__restrict__ shs,
                                                         const Dir dir,
                                                         const uint8_t *__restrict__ vis,
-                                                        float *__restrict__ value) {
+                                                        float *__restrict__ value, int ld) {
   constexpr int D = (L + 1) * (L + 1);
   const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;  // row (i, c) = i * C + c
   if (t >= n * C) return;
   const int64_t i = t / C;
+  float *out = value + i * ld + (t - i * C);
   if (vis && !vis[i]) {
-    value[t] = 0.f;
+    *out = 0.f;
     return;
   }
   float dx, dy, dz;
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(256) void sh_fwd_gc_kernel(int64_t n, int C, const 
   float acc = 0.f;
 #pragma unroll
   for (int k = 0; k < D; ++k) acc += Y[k] * r[k];
-  value[t] = acc;
+  *out = acc;
 }
 
 // dL/dshs[i,c,k] = Y_k g_c;  dL/ddir = sum_c sum_k (g_c shs[i,c,k]) grad Y_k,
@@ -408,7 +409,7 @@ template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const float *__restrict__ shs,
                                                         const Dir dir,
                                                         const uint8_t *__restrict__ vis,
-                                                        const float *__restrict__ dval,
+                                                        const float *__restrict__ dval, int ld,
                                                         float *__restrict__ dshs) {
   constexpr int D = (L + 1) * (L + 1);
   const int G = kWave / C;  // Gaussians per wave
@@ -428,7 +429,7 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
     } else {
       float x, y, zz;
       dir.load(i, x, y, zz);
-      const float gc = dval[t];
+      const float gc = dval[i * ld + c];
       float r[D];
       sh_load_row<D>(shs + t * D, r);
       float Y[D];
@@ -464,11 +465,11 @@ template <int L, class Dir>
 __global__ __launch_bounds__(kShThreads) void sh_fwd_kernel(int64_t n, int C, const float *__restrict__ shs,
                                                             const Dir dir,
                                                             const uint8_t *__restrict__ vis,
-                                                            float *__restrict__ value) {
+                                                            float *__restrict__ value, int ld) {
   constexpr int D = (L + 1) * (L + 1);
   const int64_t i = int64_t(blockIdx.x) * kShThreads + threadIdx.x;
   if (i >= n) return;
-  float *out = value + i * C;
+  float *out = value + i * ld;
   if (vis && !vis[i]) {
     for (int c = 0; c < C; ++c) out[c] = 0.f;
     return;
@@ -492,7 +493,7 @@ template <int L, class Dir>
 __global__ __launch_bounds__(kShThreads) void sh_bwd_kernel(int64_t n, int C, const float *__restrict__ shs,
                                                             const Dir dir,
                                                             const uint8_t *__restrict__ vis,
-                                                            const float *__restrict__ dval,
+                                                            const float *__restrict__ dval, int ld,
                                                             float *__restrict__ dshs) {
   constexpr int D = (L + 1) * (L + 1);
   const int64_t i = int64_t(blockIdx.x) * kShThreads + threadIdx.x;
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(kShThreads) void sh_bwd_kernel(int64_t n, int C, co
   float x, y, z;
   dir.load(i, x, y, z);
   const float *row = shs + i * int64_t(C) * D;
-  const float *g = dval + i * C;
+  const float *g = dval + i * ld;
   {
     float Y[D];
     sh_basis<L>(x, y, z, [&](int k, float v, float, float, float) { Y[k] = v; });

@@ -1,0 +1,137 @@
+"""Rasterization of SH colours, fused: the view-dependent colour of every
+Gaussian (``compute_sh_points``: degree-L SH of the direction from the camera
+centre), optional extra channels and optionally the camera-space depth, all
+rendered in one pass.
+
+Equals ``rasterization(xyz, ..., cat([compute_sh_points(shs, xyz, campos),
+extra, project_point(xyz, ...)[1]], 1), ...)``, the composition a training
+step of the reference builds from separate ops (msplat/compute_sh.py,
+msplat/project_point.py, msplat/__init__.py:22-93), with three differences
+in how it runs:
+
+* the SH kernel writes its channels straight into the feature rows (no
+  ``cat``) and runs on a side stream while the main stream projects and
+  bins the Gaussians, whose sort kernels leave most of the GPU idle; the
+  adjoint likewise overlaps the SH adjoint with the geometry adjoint;
+* the depth channel is the fused preprocessing's own depth (no second
+  projection), and its gradient enters the geometry adjoint directly;
+* one autograd node instead of four.
+
+The reference has no such op; the composition's results are reproduced:
+images bit for bit, gradients to float summation order.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib, tuning
+from .binning import CellLists, bin_gaussians, rasterize_backward, rasterize_forward
+from .project_point import camera_grad_shape
+from .rasterization import preprocess_backward, preprocess_forward
+
+_side_streams: dict[int, torch.cuda.Stream] = {}
+
+
+def _side_stream(dev: torch.device) -> torch.cuda.Stream:
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    if idx not in _side_streams:
+        _side_streams[idx] = torch.cuda.Stream(device=idx)  # default priority
+    return _side_streams[idx]
+
+
+def rasterization_sh(xyz, scale, rotate, opacity, shs, intr, extr, campos, W: int, H: int, bg: float, extra=None,
+                     depth_channel: bool = False, ndc=None):
+    """Render ``[Csh + E + depth_channel, H, W]``: SH colours of shs [P, Csh, D]
+    seen from campos [3], then extra [P, E] (optional), then the depth.
+    Differentiable with respect to xyz, scale, rotate, opacity, shs, extra,
+    intr, extr and campos."""
+    return _RasterizeSH.apply(xyz, scale, rotate, opacity, shs, intr, extr, campos, int(W), int(H), float(bg),
+                              extra, bool(depth_channel), ndc)
+
+
+class _RasterizeSH(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, scale, rotate, opacity, shs, intr, extr, campos, W, H, bg, extra, depth_channel, ndc):
+        s = _lib.f32("shs", shs)
+        n, Csh, D = s.shape
+        x = _lib.f32("xyz", xyz)
+        cp = _lib.f32("campos", campos).reshape(3)
+        E = 0 if extra is None else int(extra.shape[1])
+        C = Csh + E + (1 if depth_channel else 0)
+        dev = x.device
+        main = torch.cuda.current_stream(dev)
+        side = _side_stream(dev)
+        feat = torch.empty((n, C), dtype=torch.float32, device=dev)
+        cfg = tuning.config_for(W, H, C)
+        with torch.cuda.device(dev):
+            # SH colours into feature columns [0, Csh) on the side stream
+            ready = torch.cuda.Event()
+            ready.record(main)
+            side.wait_event(ready)
+            with torch.cuda.stream(side):
+                _lib.call("msplat_compute_sh_points_forward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None, n, Csh,
+                          D, _lib.ptr(feat), C, side.cuda_stream)
+                sh_done = torch.cuda.Event()
+                sh_done.record(side)
+            for t in (s, x, cp, feat):
+                t.record_stream(side)
+            # meanwhile: geometry, the other columns, binning
+            uv, depth, conic, radius, _ = preprocess_forward(xyz, scale, rotate, intr, extr, W, H)
+            if E:
+                feat[:, Csh:Csh + E].copy_(extra)
+            if depth_channel:
+                feat[:, C - 1].copy_(depth[:, 0])
+            bins = bin_gaussians(uv, depth, W, H, radius, cfg.cell)
+            main.wait_event(sh_done)
+            out, final_T, ncontrib, walk = rasterize_forward(uv, conic, opacity, feat, bins, bg, W, H, cfg.ppl_fwd,
+                                                             cfg.cull)
+        ctx.W, ctx.H, ctx.bg, ctx.cell, ctx.ppl_bwd, ctx.cull = W, H, bg, cfg.cell, cfg.ppl_bwd, cfg.cull
+        ctx.dims = (Csh, D, E, C, depth_channel)
+        ctx.campos_shape = campos.shape
+        ctx.has_ndc = ndc is not None
+        ctx.save_for_backward(xyz, scale, rotate, opacity, s, intr, extr, cp, feat, uv, depth, conic, radius,
+                              bins.rect, bins.cell_ids, bins.cell_range, final_T, ncontrib, walk)
+        return out
+
+    @staticmethod
+    def backward(ctx, dL_dout):
+        (xyz, scale, rotate, opacity, s, intr, extr, cp, feat, uv, depth, conic, radius, rect, cell_ids, cell_range,
+         final_T, ncontrib, walk) = ctx.saved_tensors
+        Csh, D, E, C, depth_channel = ctx.dims
+        n = xyz.shape[0]
+        dev = xyz.device
+        main = torch.cuda.current_stream(dev)
+        side = _side_stream(dev)
+        bins = CellLists(rect, cell_ids, cell_range, ctx.cell)
+        with torch.cuda.device(dev):
+            duv, dconic, dopac, dfeat = rasterize_backward(uv, conic, opacity, feat, bins, ctx.bg, ctx.W, ctx.H,
+                                                           final_T, ncontrib, walk, dL_dout, ctx.ppl_bwd, ctx.cull)
+            # SH adjoint (feature columns [0, Csh)) on the side stream, next to the geometry adjoint
+            x = _lib.f32("xyz", xyz)
+            dshs = torch.empty((n, Csh, D), dtype=torch.float32, device=dev)
+            dxyz_sh = torch.empty((n, 3), dtype=torch.float32, device=dev)
+            ready = torch.cuda.Event()
+            ready.record(main)
+            side.wait_event(ready)
+            with torch.cuda.stream(side):
+                _lib.call("msplat_compute_sh_points_backward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None,
+                          _lib.ptr(dfeat), C, n, Csh, D, _lib.ptr(dshs), _lib.ptr(dxyz_sh), side.cuda_stream)
+                sh_done = torch.cuda.Event()
+                sh_done.record(side)
+            for t in (s, x, cp, dfeat, dshs, dxyz_sh):
+                t.record_stream(side)
+            dxyz, dscale, drot, dintr, dextr = preprocess_backward(
+                xyz, scale, rotate, intr, extr, depth, radius, duv, dconic,
+                dL_ddepth=dfeat[:, C - 1] if depth_channel else None,
+                want_intr=intr.requires_grad, want_extr=extr.requires_grad)
+            main.wait_event(sh_done)
+            dxyz = dxyz + dxyz_sh
+            dcampos = (-dxyz_sh.sum(0)).view(ctx.campos_shape) if ctx.needs_input_grad[7] else None
+        dextra = dfeat[:, Csh:Csh + E] if (E and ctx.needs_input_grad[11]) else None
+        dndc = None
+        if ctx.has_ndc:
+            dndc = duv * torch.tensor([0.5 * ctx.W, 0.5 * ctx.H], dtype=duv.dtype, device=duv.device)[None]
+        return (dxyz, dscale, drot, dopac.view_as(opacity), dshs, dintr,
+                camera_grad_shape(extr, dextr) if dextr is not None else None, dcampos, None, None, None,
+                dextra, None, dndc)
