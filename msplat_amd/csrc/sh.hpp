@@ -430,8 +430,16 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
       float x, y, zz;
       dir.load(i, x, y, zz);
       const float gc = dval[i * ld + c];
+      // a zero upstream gradient (most rows: Gaussians nobody sees) adds
+      // nothing to dL/ddir, so its coefficients are not read (for finite
+      // coefficients the sum is unchanged; dL/dshs is still Y_k * 0)
       float r[D];
-      sh_load_row<D>(shs + t * D, r);
+      if (gc != 0.f) {
+        sh_load_row<D>(shs + t * D, r);
+      } else {
+#pragma unroll
+        for (int k = 0; k < D; ++k) r[k] = 0.f;
+      }
       float Y[D];
       sh_basis<L>(x, y, zz, [&](int k, float v, float dx, float dy, float dz) {
         Y[k] = v * gc;
