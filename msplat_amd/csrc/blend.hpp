@@ -435,15 +435,15 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   const float *vdout = dout + int64_t(in.C) * pb.pix0;  // this view's [C, H, W]
 
   // Per pixel: T (recovered back to front), tb = -T_final (bg . dL/dpixel)
-  // for the background term, the channels' dL/dpixel and the colour
-  // accumulated behind the current Gaussian.  The reference updates that
-  // accumulation lazily at the next contributor from the previous one's
-  // alpha and colour (alpha_blending.cu:213-214); updating it right after
-  // each contributor performs the same operations on the same values and
-  // keeps no copy of the previous colour.
-  float T[PPL], tb[PPL], fx[PPL], fy[PPL];
+  // for the background term, the channels' dL/dpixel g and
+  // A = accum . g, the colour accumulated behind the current Gaussian
+  // (alpha_blending.cu:213-214) dotted with g.  The adjoint needs accum only
+  // through dL/dalpha = sum_c (f_c - accum_c) g_c = f . g - A, and the
+  // recurrence accum' = accum + alpha (f - accum) carries over to A' = A +
+  // alpha (f . g - A): one scalar per pixel instead of C colour sums.
+  float T[PPL], tb[PPL], fx[PPL], fy[PPL], A[PPL];
   int last[PPL];
-  float g[PPL][CMAX], accum[PPL][CMAX];
+  float g[PPL][CMAX];
   int wl = 0;
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
@@ -459,8 +459,8 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
     for (int ch = 0; ch < CMAX; ++ch) {
       g[i][ch] = (inside && ch < cc) ? vdout[(in.c0 + ch) * hw + pix] : 0.f;
       if (ch < cc) bg_dot += in.bg * g[i][ch];
-      accum[i][ch] = 0.f;
     }
+    A[i] = 0.f;
     T[i] = T_final;
     tb[i] = -T_final * bg_dot;
   }
@@ -579,15 +579,15 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
         T[i] = T[i] * inv;
         const float dcol = alpha * T[i];
-        float dL_dalpha = 0.f;
+        float fg = 0.f;  // f . g
 #pragma unroll
         for (int ch = 0; ch < CMAX; ++ch) {  // channels >= cc carry zeros
-          const float d = fe[ch] - accum[i][ch];
-          dL_dalpha += d * g[i][ch];
+          fg += fe[ch] * g[i][ch];
           vf[ch] += dcol * g[i][ch];
-          accum[i][ch] += alpha * d;  // = alpha fe + (1 - alpha) accum
         }
-        dL_dalpha *= T[i];
+        const float d = fg - A[i];
+        A[i] += alpha * d;
+        float dL_dalpha = d * T[i];
         dL_dalpha += tb[i] * inv;
         const float w = G * (op * dL_dalpha);  // G dL/dG
         const float wy = w * dy;
