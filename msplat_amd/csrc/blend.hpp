@@ -51,6 +51,7 @@ struct BlendIn {
   float bg;
   int feat_per_view;
   int ntiles, per_xcd;     // global tiles of all views; workgroups per XCD band
+  int ngroups;             // global 2 x 2 tile groups (grouped adjoint)
   Views views;
 };
 
@@ -98,6 +99,49 @@ MSPLAT_DEV PixBlock<PPL> pix_block(const BlendIn &in) {
     }
   }
   return p;
+}
+
+// Pixel block of wave `wave` (0..3) in a workgroup covering a 2 x 2 tile
+// group: the whole tile (PPL 4) at (2 gx + wave % 2, 2 gy + wave / 2).
+MSPLAT_DEV PixBlock<4> pix_block_group(const BlendIn &in, int wave) {
+  const int b = blockIdx.x;
+  const int q = (b & 7) * in.per_xcd + (b >> 3);  // XCD bands, as for single tiles
+  PixBlock<4> p;
+  const bool live = q < in.ngroups;
+  p.view = live ? view_of_group(in.views, q) : 0;
+  const int gx = in.views.gx[p.view], gy = in.views.gy[p.view];
+  const int local = live ? q - in.views.group0[p.view] : 0;
+  const int ngx = (gx + 1) / 2;
+  p.tx = 2 * (local % ngx) + (wave & 1);
+  p.ty = 2 * (local / ngx) + (wave >> 1);
+  p.active = live && p.tx < gx && p.ty < gy;
+  p.tile = in.views.tile0[p.view] + p.ty * gx + p.tx;
+  p.sub = 0;
+  p.W = in.views.W[p.view];
+  p.H = in.views.H[p.view];
+  p.pix0 = in.views.pix0[p.view];
+  p.vbase = in.views.vbase[p.view];
+  p.gshift = in.views.gbase[p.view] - p.vbase;
+  const int lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    p.px[i] = p.tx * kTile + (lane & 15);
+    p.py[i] = p.ty * kTile + (lane >> 4) + 4 * i;
+  }
+  return p;
+}
+
+// Synchronisation of one wave's staging: a workgroup barrier for one-wave
+// workgroups (free), a wave-level one when several waves work independently.
+template <int TG>
+MSPLAT_DEV void stage_sync() {
+  if constexpr (TG == 1) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 }
 
 // Workgroups per launch for a given pixels-per-lane (8 XCD bands of per_xcd).
@@ -415,10 +459,22 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
 
 // Adjoint: walk each pixel's contributors back to front, recovering T by
 // division (alpha_blending.cu:155-245).  Per Gaussian, every lane first sums
-// its PPL pixels' contributions, then the wave sums its 64 lanes and issues
-// one atomic per gradient value.
-template <int CMAX, int PPL, bool kWalk>
-__global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
+// its PPL pixels' contributions, then the wave sums its 64 lanes.
+//
+// TG = 1: one wave per workgroup; each wave adds its sums to the global
+// gradients, one atomic per value per Gaussian.  Those atomics -- the same
+// Gaussians from every tile it covers -- cost about a third of the kernel.
+// TG = 4 (cell lists, PPL 4, even cells): a workgroup's 4 waves cover a 2 x 2
+// group of tiles of one cell, so they walk the same cell list; each wave
+// adds its sums into an LDS accumulator indexed by cell-list position (the
+// first kGroupAcc positions of the cell) and the last wave to finish adds
+// the group's totals to the global gradients: up to 4 times fewer global
+// atomics.  The waves stage and walk independently; only the accumulator is
+// shared.
+constexpr int kGroupAcc = 256;
+
+template <int CMAX, int PPL, bool kWalk, int TG = 1>
+__global__ __launch_bounds__(kBlendLanes * TG) void blend_bwd_kernel(
     BlendIn in, const float *__restrict__ final_T, const int32_t *__restrict__ ncontrib,
     const int32_t *__restrict__ walk, const float *__restrict__ dout, float *__restrict__ duv,
     float *__restrict__ dconic, float *__restrict__ dopac, float *__restrict__ dfeat) {
@@ -426,11 +482,55 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   // are compared to a tolerance); every value the forward also computes
   // (power, alpha, T) goes through explicitly rounded code.
 #pragma clang fp contract(fast)
-  __shared__ BlendStage<CMAX> s;
-  const PixBlock<PPL> pb = pix_block<PPL>(in);
-  if (!pb.active) return;
-  const int lane = threadIdx.x;
+  static_assert(TG == 1 || (TG == 4 && PPL == 4 && kWalk), "tile groups: cell lists, 4 pixels per lane");
+  constexpr int NVA = 6 + CMAX;                     // accumulated values per Gaussian
+  constexpr int kAcc = TG > 1 ? kGroupAcc : 1;
+  __shared__ BlendStage<CMAX> stage_all[TG];
+  __shared__ float acc[kAcc * NVA];
+  __shared__ int32_t acc_id[kAcc];
+  __shared__ uint32_t waves_done;
+  const int wave = TG > 1 ? int(threadIdx.x / kBlendLanes) : 0;
+  const int lane = TG > 1 ? int(threadIdx.x % kBlendLanes) : int(threadIdx.x);
+  BlendStage<CMAX> &s = stage_all[wave];
   const int cc = in.C - in.c0 < CMAX ? in.C - in.c0 : CMAX;  // channels of this pass
+  PixBlock<PPL> pb;
+  if constexpr (TG > 1) {
+    pb = pix_block_group(in, wave);
+    for (int k = threadIdx.x; k < kAcc * NVA; k += kBlendLanes * TG) acc[k] = 0.f;
+    for (int k = threadIdx.x; k < kAcc; k += kBlendLanes * TG) acc_id[k] = -1;
+    if (threadIdx.x == 0) waves_done = 0u;
+    __syncthreads();  // the accumulator is clear before any wave adds to it
+  } else {
+    pb = pix_block<PPL>(in);
+  }
+  // The last wave of a group to finish adds the accumulated totals to the
+  // global gradients (every wave's LDS additions precede its increment).
+  auto finish_group = [&]() {
+    if constexpr (TG > 1) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      uint32_t before = 0u;
+      if (lane == 0) before = atomicAdd(&waves_done, 1u);
+      before = __shfl(before, 0, kWave);
+      if (before != uint32_t(TG - 1)) return;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      for (int k = lane; k < kAcc * NVA; k += kWave) {
+        const int slot = k / NVA, v = k - slot * NVA;
+        const int gidx = acc_id[slot];
+        const float val = acc[k];
+        if (gidx < 0 || v >= 6 + cc || val == 0.f) continue;
+        float *dst;
+        if (v < 2) dst = duv + 2 * int64_t(gidx) + v;
+        else if (v < 5) dst = dconic + 3 * int64_t(gidx) + (v - 2);
+        else if (v == 5) dst = dopac + (gidx + pb.gshift);
+        else dst = dfeat + feature_row(in, gidx, pb.gshift) * in.C + in.c0 + (v - 6);
+        unsafeAtomicAdd(dst, val);
+      }
+    }
+  };
+  if (!pb.active) {
+    finish_group();
+    return;
+  }
   const int64_t hw = int64_t(pb.H) * pb.W;
   const float *vdout = dout + int64_t(in.C) * pb.pix0;  // this view's [C, H, W]
 
@@ -487,12 +587,13 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   constexpr int NV = ((6 + CMAX) + 3) / 4 * 4;
   constexpr int NQ = (NV / 4 + 15) / 16;  // atomic rounds
   float *tbase[NQ];
-  int tstride[NQ];
+  int tstride[NQ], tval[NQ];
   bool tvalid[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int vec = (lane & 15) + 16 * q;
     const int v = 4 * vec + row_value(lane >> 4);
+    tval[q] = v;
     tvalid[q] = vec < NV / 4 && v < 6 + cc;
     const int64_t gsh = pb.gshift;
     if (v < 2) {
@@ -513,7 +614,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
   while (fcount > 0 && hi > lo_bound) {
     const int lo = hi - kBlendLanes > lo_bound ? hi - kBlendLanes : lo_bound;
     int cnt;
-    __syncthreads();
+    stage_sync<TG>();
     if constexpr (kWalk) {
       const int e = lo + lane;
       int gi = 0;
@@ -530,6 +631,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
         const int slot = mask_rank(m);
         stage_gaussian<CMAX>(s, slot, gi, in, cc, pb.gshift);
         s.strips[slot] = strips;
+        if constexpr (TG > 1) s.pos[slot] = e;
       }
     } else {
       cnt = hi - lo;
@@ -539,7 +641,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
       }
     }
     const int fbase = fcount - cnt;  // tile-list position of slot 0
-    __syncthreads();
+    stage_sync<TG>();
     for (int j = cnt - 1; j >= 0; --j) {
       const int rel = fbase + j;  // 0-based position in the tile list
       if (rel >= wl) continue;    // uniform: no pixel of this wave reaches it
@@ -625,12 +727,22 @@ __global__ __launch_bounds__(kBlendLanes) void blend_bwd_kernel(
 #pragma unroll
         for (int jj = 0; jj < NV / 4; ++jj)
           if (jj == vec) mine = tot[jj];
-        if (tvalid[q]) unsafeAtomicAdd(tbase[q] + int64_t(gidx) * tstride[q], mine);
+        if (!tvalid[q]) continue;
+        if constexpr (TG > 1) {
+          const int key = s.pos[j] - lo_bound;  // position in the group's cell list
+          if (key < kAcc) {
+            if (tval[q] == 0) acc_id[key] = gidx;
+            atomicAdd(&acc[key * NVA + tval[q]], mine);
+            continue;
+          }
+        }
+        unsafeAtomicAdd(tbase[q] + int64_t(gidx) * tstride[q], mine);
       }
     }
     fcount = fbase;
     hi = lo;
   }
+  finish_group();
 }
 
 }  // namespace msplat

@@ -87,6 +87,7 @@ struct Views {
   uint16_t gx[kMaxViews], gy[kMaxViews];  // tile grid (<= 4096 x 4096)
   uint16_t ncx[kMaxViews];                // cells per row (cell lists)
   int tile0[kMaxViews + 1];
+  int group0[kMaxViews + 1];  // 2 x 2 tile groups (the grouped blending adjoint)
   int cell0[kMaxViews + 1];
   int64_t pix0[kMaxViews + 1];
   int64_t vbase[kMaxViews + 1];
@@ -99,6 +100,16 @@ MSPLAT_DEV int view_of_tile(const Views &v, int t) {
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (v.tile0[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// View owning global 2 x 2 tile group q (binary search over group0).
+MSPLAT_DEV int view_of_group(const Views &v, int q) {
+  int lo = 0, hi = v.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (v.group0[mid] <= q) lo = mid; else hi = mid - 1;
   }
   return lo;
 }

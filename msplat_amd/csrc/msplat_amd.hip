@@ -6,6 +6,7 @@
 #include "../../include/msplat_amd.h"
 
 #include <atomic>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <map>
@@ -362,7 +363,7 @@ std::string make_views(int B, const int *W, const int *H, int S, const int64_t *
   if ((first == nullptr) != (count == nullptr)) return "first and count must both be given or both be NULL";
   v = Views{};
   v.count = B;
-  int64_t tiles = 0, cells = 0, pix = 0, rows = 0;
+  int64_t tiles = 0, cells = 0, pix = 0, rows = 0, groups = 0;
   for (int b = 0; b < B; ++b) {
     const int64_t f = first ? first[b] : 0, c = count ? count[b] : n;
     if (f < 0 || c < 0) return "negative Gaussian range of view " + std::to_string(b);
@@ -377,15 +378,18 @@ std::string make_views(int B, const int *W, const int *H, int S, const int64_t *
     v.gy[b] = uint16_t(tiles_y(H[b]));
     v.ncx[b] = uint16_t(cells_x(W[b], S));
     v.tile0[b] = int(tiles);
+    v.group0[b] = int(groups);
     v.cell0[b] = int(cells);
     v.pix0[b] = pix;
     tiles += int64_t(v.gx[b]) * v.gy[b];
+    groups += int64_t((v.gx[b] + 1) / 2) * ((v.gy[b] + 1) / 2);
     cells += int64_t(v.ncx[b]) * cells_y(H[b], S);
     pix += int64_t(W[b]) * H[b];
     if (tiles >= INT32_MAX || cells >= INT32_MAX) return "too many tiles";
   }
   if (rows > int64_t(INT32_MAX)) return "more than 2^31 per-view rows";
   v.tile0[B] = int(tiles);
+  v.group0[B] = int(groups);
   v.cell0[B] = int(cells);
   v.pix0[B] = pix;
   v.vbase[B] = rows;
@@ -399,6 +403,16 @@ Views one_view(int W, int H, int S, int64_t n) {
 }
 
 // ---- blending launchers (tile lists and cell lists) -----------------------
+
+// The grouped adjoint is the default; MSPLAT_GROUPED_ADJOINT=0 selects one
+// wave per tile (for comparison).
+bool grouped_adjoint() {
+  static const bool on = [] {
+    const char *e = std::getenv("MSPLAT_GROUPED_ADJOINT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
 
 int pick_ppl(int requested, int cm) {
   int p = requested <= 0 ? 4 : (requested >= 4 ? 4 : (requested >= 2 ? 2 : 1));
@@ -432,6 +446,16 @@ void bwd_launch(int p, BlendIn in, const float *final_T, const int32_t *ncontrib
   in.per_xcd = blend_per_xcd(in.ntiles, p);
   Timed timed_(kWalk ? "rasterize_bwd" : "blend_bwd", s);
   const dim3 grid(unsigned(8 * in.per_xcd));
+  if constexpr (kWalk && blend_max_ppl(CM) >= 4 && CM <= 8) {
+    // 2 x 2 tile groups sharing one LDS gradient accumulator (cells of an
+    // even number of tiles, so that a group lies in one cell)
+    if (p == 4 && in.cell % 2 == 0 && grouped_adjoint()) {
+      in.per_xcd = (in.ngroups + 7) / 8;
+      blend_bwd_kernel<CM, 4, true, 4><<<dim3(unsigned(8 * in.per_xcd)), 4 * kBlendLanes, 0, s>>>(
+          in, final_T, ncontrib, walk, dout, duv, dconic, dopac, dfeat);
+      return;
+    }
+  }
   if constexpr (blend_max_ppl(CM) >= 4) {
     if (p == 4) {
       blend_bwd_kernel<CM, 4, kWalk><<<grid, kBlendLanes, 0, s>>>(in, final_T, ncontrib, walk, dout, duv, dconic,
@@ -519,6 +543,7 @@ BlendIn blend_inputs(const float *uv, const float *conic, const float *opacity, 
   in.feat_per_view = feat_per_view;
   in.views = vw;
   in.ntiles = vw.tile0[vw.count];
+  in.ngroups = vw.group0[vw.count];
   in.per_xcd = 0;
   return in;
 }
