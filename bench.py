@@ -186,6 +186,19 @@ def cpu_baseline(cfg, rows: int = 64) -> dict:
                 seconds=time.perf_counter() - t0)
 
 
+_ZEROS = {}
+
+
+def _zeros(like):
+    """A cached all-zero target of like's shape: the per-view loss sum(im^2) = l2_loss(im, 0)."""
+    key = (tuple(like.shape), like.device)
+    if key not in _ZEROS:
+        import torch
+
+        _ZEROS[key] = torch.zeros(like.shape, dtype=like.dtype, device=like.device)
+    return _ZEROS[key]
+
+
 def _time_steps(fn, steps: int, warmup: int) -> float:
     """Average milliseconds of fn() over `steps` after `warmup` (synchronised)."""
     import torch
@@ -222,12 +235,12 @@ def bench_multiview(dev, steps: int = 5, warmup: int = 2) -> dict:
         loss = 0.0
         for b, c in enumerate(cams):
             out = ms.rasterization(*args, intrs[b], extrs[b], c.W, c.H, 0.0)
-            loss = loss + (out * out).sum()
+            loss = loss + ms.l2_loss(out, _zeros(out))
         return torch.autograd.grad(loss, args + [extrs])
 
     def batched_step():
         imgs = views.rasterization_views(*args, intrs, extrs, sizes, 0.0)
-        loss = sum((im * im).sum() for im in imgs)
+        loss = sum(ms.l2_loss(im, _zeros(im)) for im in imgs)
         return torch.autograd.grad(loss, args + [extrs])
 
     pix = sum(w * h for w, h in sizes)
@@ -274,15 +287,16 @@ def bench_batched_1080p(dev, views: int = 8, steps: int = 3, warmup: int = 1) ->
         return ms.alpha_blending(uv, conic, opacity, feature, ids, tile_range, 0.0, c.W, c.H)
 
     def plain_step():
-        loss = sum((im * im).sum() for im in (plain_view(c) for c in cams))
+        loss = sum(ms.l2_loss(im, _zeros(im)) for im in (plain_view(c) for c in cams))
         return torch.autograd.grad(loss, args)
 
     def loop_step():
-        loss = sum((im * im).sum() for im in (ms.rasterization(*args, c.intr, c.extr, c.W, c.H, 0.0) for c in cams))
+        loss = sum(ms.l2_loss(im, _zeros(im))
+                   for im in (ms.rasterization(*args, c.intr, c.extr, c.W, c.H, 0.0) for c in cams))
         return torch.autograd.grad(loss, args)
 
     def batched_step():
-        loss = sum((im * im).sum() for im in mv.rasterization_views(*args, intrs, extrs, sizes, 0.0))
+        loss = sum(ms.l2_loss(im, _zeros(im)) for im in mv.rasterization_views(*args, intrs, extrs, sizes, 0.0))
         return torch.autograd.grad(loss, args)
 
     with torch.no_grad():
