@@ -74,9 +74,9 @@ import sys as _sys
 
 # library kernel -> device function pattern (in rocprof's demangled names)
 KERNEL_PATTERNS = {
-    "rasterize_bwd": r"blend_bwd_kernel<\d+, \d+, true>",
+    "rasterize_bwd": r"blend_bwd_kernel<\d+, \d+, true(, \d+)?>",
     "rasterize_fwd": r"blend_fwd_kernel<\d+, \d+, true>",
-    "blend_bwd": r"blend_bwd_kernel<\d+, \d+, false>",
+    "blend_bwd": r"blend_bwd_kernel<\d+, \d+, false(, \d+)?>",
     "blend_fwd": r"blend_fwd_kernel<\d+, \d+, false>",
     "sh_fwd": r"sh_fwd_\w*kernel",
     "sh_bwd": r"sh_bwd_\w*kernel",
