@@ -153,9 +153,13 @@ int msplat_compute_sh_points_backward(const float *shs, const float *xyz,
  * sh_bwd, depth_sort, emit, class_sort, blend_fwd, blend_bwd,
  * rasterize_fwd, rasterize_bwd.  msplat_timing_enable returns the previous
  * state; msplat_timing_read waits for the recorded events of one kernel,
- * returns their total milliseconds and launch count, and forgets them. */
+ * returns their total milliseconds and launch count, and forgets them.
+ * msplat_timing_only restricts the timing to the comma-separated kernel
+ * names given (NULL or "" times every kernel): each timed launch adds two
+ * event records to the stream, which cost the stream some time of their own. */
 int msplat_timing_enable(int on);
 int msplat_timing_read(const char *kernel, double *total_ms, int64_t *launches);
+int msplat_timing_only(const char *kernels);
 
 /* ---- tile sort.  Replaces msplat/sort_gaussian.py:42-52 (cumsum, keygen
  * sort_gaussian.cu:74-113, torch.sort, gather, tile range

@@ -51,6 +51,7 @@ SIGNATURES = {
         I32, [P, P, P, P, I64, I32, P, P, F32, I32, I32, P, P, P, P, P, P, P, P]),
     "msplat_timing_enable": (I32, [I32]),
     "msplat_timing_read": (I32, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I64)]),
+    "msplat_timing_only": (I32, [ctypes.c_char_p]),
     "msplat_bin_prepare_workspace_bytes": (SZ, [I64, I32, I32, I32]),
     "msplat_bin_prepare": (I32, [P, P, P, I64, I32, I32, I32, P, P, SZ, P, P]),
     "msplat_bin_emit_workspace_bytes": (SZ, [I64, I64, I32, I32, I32]),

@@ -174,8 +174,9 @@ MSPLAT_DEV int64_t feature_row(const BlendIn &in, int64_t vg, int64_t gshift) {
 }
 
 // Stage list entry vg into LDS slot `slot`: channels [c0, c0 + cc) of its row.
+// Returns whether all those channels are finite.
 template <int CMAX>
-MSPLAT_DEV void stage_gaussian(BlendStage<CMAX> &s, int slot, int vg, const BlendIn &in, int cc, int64_t gshift) {
+MSPLAT_DEV bool stage_gaussian(BlendStage<CMAX> &s, int slot, int vg, const BlendIn &in, int cc, int64_t gshift) {
   constexpr int NQ = BlendStage<CMAX>::kQuads;
   float r[NQ * 4];
   r[0] = in.uv[2 * int64_t(vg)];
@@ -190,6 +191,10 @@ MSPLAT_DEV void stage_gaussian(BlendStage<CMAX> &s, int slot, int vg, const Blen
 #pragma unroll
   for (int q = 0; q < NQ; ++q) s.rec[slot * NQ + q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
   s.id[slot] = vg;
+  bool finite = true;
+#pragma unroll
+  for (int i = 6; i < NQ * 4; ++i) finite = finite && __builtin_isfinite(r[i]);
+  return finite;
 }
 
 // Geometry of record j: (u, v, a, b) and (c, o, f0, f1).
@@ -314,6 +319,17 @@ MSPLAT_DEV float gauss_power(float a, float b, float c, float dx, float dy) {
   return __builtin_fmaf(-0.5f, q, -((b * dx) * dy));
 }
 
+// gauss_power for the pixels of one column (a lane's pixels share dx): the
+// dx products computed once, the same roundings as gauss_power.
+struct PowerCol {
+  float dx, adx, bdx;
+};
+MSPLAT_DEV PowerCol power_col(float a, float b, float dx) { return PowerCol{dx, a * dx, b * dx}; }
+MSPLAT_DEV float gauss_power_col(const PowerCol &p, float c, float dy) {
+  const float q = __builtin_fmaf(p.adx, p.dx, (c * dy) * dy);
+  return __builtin_fmaf(-0.5f, q, -(p.bdx * dy));
+}
+
 // Forward.  Outputs rendered [C, H, W] (+ bg * T), final_T and ncontrib
 // [H, W] (the 1-based list position of each pixel's last contributor).  Cell
 // lists also record, per tile, the end of the walk that the adjoint starts
@@ -334,18 +350,19 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
   int pos = lst.x;
   const int end = lst.y;
 
-  bool done[PPL];
+  // T[i] > 0 is the pixel's transmittance; a saturated pixel (and one
+  // outside the image) holds -T, its final transmittance negated, so one
+  // compare tells whether the pixel still blends.
   float T[PPL], fx[PPL], fy[PPL];
-  int last[PPL], last_e[PPL];
+  int last[PPL];
+  int we = 0;  // cell lists: list index after the wave's last contribution
   float F[PPL][CMAX];
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
     fx[i] = float(pb.px[i]);
     fy[i] = float(pb.py[i]);
-    done[i] = !(pb.px[i] < pb.W && pb.py[i] < pb.H);
-    T[i] = 1.f;
+    T[i] = (pb.px[i] < pb.W && pb.py[i] < pb.H) ? 1.f : -1.f;
     last[i] = 0;
-    last_e[i] = 0;
 #pragma unroll
     for (int ch = 0; ch < CMAX; ++ch) F[i][ch] = 0.f;
   }
@@ -354,9 +371,10 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
   while (pos < end) {
     bool alive = false;
 #pragma unroll
-    for (int i = 0; i < PPL; ++i) alive |= !done[i];
+    for (int i = 0; i < PPL; ++i) alive |= T[i] > 0.f;
     if (__ballot(alive) == 0ull) break;
     int cnt;
+    bool nonfinite = false;  // a staged Gaussian has a non-finite feature
     __syncthreads();
     if constexpr (kWalk) {
       const int e = pos + lane;
@@ -372,7 +390,7 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
       cnt = __popcll(m);
       if (pass) {
         const int slot = mask_rank(m);
-        stage_gaussian<CMAX>(s, slot, g, in, cc, pb.gshift);
+        nonfinite = !stage_gaussian<CMAX>(s, slot, g, in, cc, pb.gshift);
         s.pos[slot] = e;
         s.strips[slot] = strips;
       }
@@ -380,17 +398,24 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
     } else {
       cnt = end - pos < kBlendLanes ? end - pos : kBlendLanes;
       if (lane < cnt) {
-        stage_gaussian<CMAX>(s, lane, in.list[pos + lane], in, cc, pb.gshift);
+        nonfinite = !stage_gaussian<CMAX>(s, lane, in.list[pos + lane], in, cc, pb.gshift);
         s.strips[lane] = 0xfu;
       }
       pos += cnt;
     }
     __syncthreads();
+    // Branch-free per pixel: every quantity is computed and the reference's
+    // early-outs become selects, so each update happens exactly where the
+    // reference's would (with the same operations) and nowhere else.  With
+    // finite features, f w' + F with w' = 0 leaves F unchanged (F starts at
+    // +0 and is never -0), so one select of the weight serves all channels; a
+    // batch holding a non-finite feature (f 0 = NaN) selects per channel.
+    auto walk_batch = [&](auto finite) {
     for (int j = 0; j < cnt; ++j) {
       if ((j & 7) == 7) {  // every 8 Gaussians: stop once every pixel of the wave is saturated
         bool live = false;
 #pragma unroll
-        for (int i = 0; i < PPL; ++i) live |= !done[i];
+        for (int i = 0; i < PPL; ++i) live |= T[i] > 0.f;
         if (__ballot(live) == 0ull) break;
       }
       float4 g0, g1;
@@ -398,32 +423,44 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
       float f[CMAX];
       stage_features(s, j, g1, f);
       const int contributor = fpos + j + 1;
-      int epos = 0;
-      if constexpr (kWalk) epos = s.pos[j] + 1;
       const uint32_t strips = __builtin_amdgcn_readfirstlane(s.strips[j]);
+      const PowerCol col = power_col(g0.z, g0.w, g0.x - fx[0]);  // a lane's pixels share one column
 #pragma unroll
-      for (int i = 0; i < PPL; ++i) {
-        if (!(strips & slot_strips<PPL>(pb.sub, i))) continue;  // uniform: culled for these rows
-        // Branch-free: every quantity is computed and the reference's
-        // early-outs become selects, so each update happens exactly where the
-        // reference's would (with the same operations) and nowhere else.
-        const float dx = g0.x - fx[i], dy = g0.y - fy[i];
-        const float power = gauss_power(g0.z, g0.w, g1.x, dx, dy);
-        const float alpha = fminf(0.99f, g1.y * fast_exp(power));
-        // a NaN power is not skipped, as in the reference
-        const bool hit = !done[i] && !(power > 0.f) && !(alpha < 1.f / 255.f);
-        const float nT = T[i] * (1.f - alpha);
-        const bool sat = hit && nT < 0.0001f;
-        const bool add = hit && !sat;
-        const float w = alpha * T[i];
+        for (int i = 0; i < PPL; ++i) {
+          if (!(strips & slot_strips<PPL>(pb.sub, i))) continue;  // uniform: culled for these rows
+          const float power = gauss_power_col(col, g1.x, g0.y - fy[i]);
+          const float alpha = fminf(0.99f, g1.y * fast_exp(power));
+          // a NaN power is not skipped, as in the reference
+          const bool hit = T[i] > 0.f && !(power > 0.f) && !(alpha < 1.f / 255.f);
+          const float nT = T[i] * (1.f - alpha);
+          const bool low = nT < 0.0001f;  // with hit: the pixel saturates here
+          const bool add = hit && !low;
+          const float w = alpha * T[i];
+          if constexpr (decltype(finite)::value) {
+            const float wa = add ? w : 0.f;
 #pragma unroll
-        for (int ch = 0; ch < CMAX; ++ch)  // channels >= cc are staged as 0 and never stored
-          F[i][ch] = add ? __builtin_fmaf(f[ch], w, F[i][ch]) : F[i][ch];
-        T[i] = add ? nT : T[i];
-        last[i] = add ? contributor : last[i];
-        if constexpr (kWalk) last_e[i] = add ? epos : last_e[i];
-        done[i] = done[i] || sat;
-      }
+            for (int ch = 0; ch < CMAX; ++ch)  // channels >= cc are staged as 0 and never stored
+              F[i][ch] = __builtin_fmaf(f[ch], wa, F[i][ch]);
+          } else {
+#pragma unroll
+            for (int ch = 0; ch < CMAX; ++ch)
+              F[i][ch] = add ? __builtin_fmaf(f[ch], w, F[i][ch]) : F[i][ch];
+          }
+          T[i] = hit ? (low ? -T[i] : nT) : T[i];
+          last[i] = add ? contributor : last[i];
+        }
+    }
+    };
+    if (__ballot(nonfinite) == 0ull)  // uniform
+      walk_batch(std::true_type{});
+    else
+      walk_batch(std::false_type{});
+    if constexpr (kWalk) {  // the wave's last contributor so far: its list index, if staged in this batch
+      int lm = 0;
+#pragma unroll
+      for (int i = 0; i < PPL; ++i) lm = imax(lm, last[i]);
+      lm = wave_max(lm);
+      if (lm > fpos) we = s.pos[lm - 1 - fpos] + 1;
     }
     fpos += cnt;
   }
@@ -433,23 +470,20 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
   for (int i = 0; i < PPL; ++i) {
     if (!(pb.px[i] < pb.W && pb.py[i] < pb.H)) continue;
     const int64_t pix = int64_t(pb.py[i]) * pb.W + pb.px[i];
+    const float Tf = fabsf(T[i]);
     if (in.c0 == 0) {
-      final_T[pb.pix0 + pix] = T[i];
+      final_T[pb.pix0 + pix] = Tf;
       ncontrib[pb.pix0 + pix] = last[i];
     }
 #pragma unroll
     for (int ch = 0; ch < CMAX; ++ch)
-      if (ch < cc) vout[(in.c0 + ch) * hw + pix] = F[i][ch] + T[i] * in.bg;
+      if (ch < cc) vout[(in.c0 + ch) * hw + pix] = F[i][ch] + Tf * in.bg;
   }
   if constexpr (kWalk) {
-    int wl = 0, we = 0;
+    int wl = 0;
 #pragma unroll
-    for (int i = 0; i < PPL; ++i) {
-      wl = imax(wl, last[i]);
-      we = imax(we, last_e[i]);
-    }
+    for (int i = 0; i < PPL; ++i) wl = imax(wl, last[i]);
     wl = wave_max(wl);
-    we = wave_max(we);
     if (lane == 0 && in.c0 == 0 && wl > 0) {
       atomicMax(walk + 2 * pb.tile, we);
       atomicMax(walk + 2 * pb.tile + 1, wl);
@@ -564,20 +598,23 @@ __global__ __launch_bounds__(kBlendLanes * TG) void blend_bwd_kernel(
     T[i] = T_final;
     tb[i] = -T_final * bg_dot;
   }
-  wl = wave_max(wl);
+  // Wave-uniform bounds are read into scalar registers: with several waves
+  // per workgroup the compiler cannot tell they are uniform, and the
+  // per-Gaussian and per-slot skips below would become per-lane masks.
+  wl = __builtin_amdgcn_readfirstlane(wave_max(wl));
   int wl_slot[PPL];  // per pixel slot: the wave's largest list end
 #pragma unroll
-  for (int i = 0; i < PPL; ++i) wl_slot[i] = wave_max(last[i]);
+  for (int i = 0; i < PPL; ++i) wl_slot[i] = __builtin_amdgcn_readfirstlane(wave_max(last[i]));
 
   const int2 lst = tile_list<kWalk>(in, pb);
-  const int lo_bound = lst.x;
+  const int lo_bound = __builtin_amdgcn_readfirstlane(lst.x);
   int hi, fcount;  // walk backwards over [lo_bound, hi); fcount Gaussians of the tile list lie below hi
   if constexpr (kWalk) {
-    hi = imin(walk[2 * pb.tile], lst.y);
-    fcount = walk[2 * pb.tile + 1];
+    hi = __builtin_amdgcn_readfirstlane(imin(walk[2 * pb.tile], lst.y));
+    fcount = __builtin_amdgcn_readfirstlane(walk[2 * pb.tile + 1]);
   } else {
-    hi = imin(lst.x + wl, lst.y);
-    fcount = hi - lst.x;
+    hi = __builtin_amdgcn_readfirstlane(imin(lst.x + wl, lst.y));
+    fcount = hi - lo_bound;
   }
 
   // After the transposed reduction, lane 16 r + k holds gradient value

@@ -14,6 +14,7 @@
 #include <climits>
 #include <cstdint>
 #include <initializer_list>
+#include <type_traits>
 #include <utility>
 
 #define MSPLAT_DEV __device__ __forceinline__

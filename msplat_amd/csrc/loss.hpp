@@ -8,36 +8,62 @@
 
 namespace msplat {
 
-constexpr int kLossThreads = 256;
-constexpr int kLossBlocks = 1024;  // partial sums: one per workgroup, summed in order by the last one
+constexpr int kLossThreads = 256;   // gradient kernel
+constexpr int kSumThreads = 1024;   // sum kernel: one 16-wave workgroup per CU
+constexpr int kSumBlocks = 256;     // partial sums: one per workgroup, summed in order by the last one
+constexpr int kSumUnroll = 4;       // float4 pairs loaded together per lane
+static_assert(kSumBlocks <= kSumThreads, "the last workgroup reads one partial per thread");
+
+MSPLAT_DEV float sq_diff4(float4 x, float4 y) {
+  const float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
+  return (d0 * d0 + d1 * d1) + (d2 * d2 + d3 * d3);
+}
 
 // partial[b] = sum over the grid-stride slice of workgroup b of (a - t)^2
 // (fp32 per thread, a fixed-order workgroup tree); the last workgroup to
 // finish (ticket) adds the partials in fp64 in a fixed order and writes the
 // total, so the result does not depend on scheduling.  ticket must be 0 on
 // entry; it is reset for the next launch.
-__global__ __launch_bounds__(kLossThreads) void sq_diff_sum_kernel(const float *__restrict__ a,
-                                                                   const float *__restrict__ t, int64_t n,
-                                                                   float *__restrict__ partial,
-                                                                   uint32_t *__restrict__ ticket,
-                                                                   float *__restrict__ out) {
-  __shared__ float ws[kLossThreads / kWave];
+//
+// Few, large workgroups: every workgroup's ticket is an atomic on one word,
+// and those serialise (2048 workgroups spent more time on their tickets
+// than on reading the arrays); each lane keeps kSumUnroll pairs of 16-byte
+// loads in flight instead.
+__global__ __launch_bounds__(kSumThreads) void sq_diff_sum_kernel(const float *__restrict__ a,
+                                                                  const float *__restrict__ t, int64_t n,
+                                                                  float *__restrict__ partial,
+                                                                  uint32_t *__restrict__ ticket,
+                                                                  float *__restrict__ out) {
+  __shared__ float ws[kSumThreads / kWave];
+  __shared__ double wd[kSumThreads / kWave];
   __shared__ bool last;
-  float acc = 0.f;
-  const int64_t stride = int64_t(gridDim.x) * kLossThreads * 4;
-  for (int64_t i = (int64_t(blockIdx.x) * kLossThreads + threadIdx.x) * 4; i < n; i += stride) {
-    if (i + 4 <= n && (reinterpret_cast<uintptr_t>(a + i) & 15) == 0 && (reinterpret_cast<uintptr_t>(t + i) & 15) == 0) {
-      const float4 x = *reinterpret_cast<const float4 *>(a + i);
-      const float4 y = *reinterpret_cast<const float4 *>(t + i);
-      const float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
-      acc += (d0 * d0 + d1 * d1) + (d2 * d2 + d3 * d3);
-    } else {
-      for (int64_t j = i; j < i + 4 && j < n; ++j) {
-        const float d = a[j] - t[j];
-        acc += d * d;
-      }
+  const int64_t tid = int64_t(blockIdx.x) * kSumThreads + threadIdx.x;
+  const int64_t nth = int64_t(gridDim.x) * kSumThreads;
+  // 16-byte loads when both arrays allow them; the remainder element-wise
+  const bool vec = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(t)) & 15) == 0;
+  const int64_t n4 = vec ? n / 4 : 0;
+  const float4 *a4 = reinterpret_cast<const float4 *>(a), *t4 = reinterpret_cast<const float4 *>(t);
+  float acc0 = 0.f, acc1 = 0.f;
+  int64_t i = tid;
+  for (; i + (kSumUnroll - 1) * nth < n4; i += kSumUnroll * nth) {
+    float4 x[kSumUnroll], y[kSumUnroll];
+#pragma unroll
+    for (int u = 0; u < kSumUnroll; ++u) {
+      x[u] = a4[i + u * nth];
+      y[u] = t4[i + u * nth];
+    }
+#pragma unroll
+    for (int u = 0; u < kSumUnroll; ++u) {
+      if (u & 1) acc1 += sq_diff4(x[u], y[u]);
+      else acc0 += sq_diff4(x[u], y[u]);
     }
   }
+  for (; i < n4; i += nth) acc0 += sq_diff4(a4[i], t4[i]);
+  for (int64_t j = n4 * 4 + tid; j < n; j += nth) {
+    const float d = a[j] - t[j];
+    acc1 += d * d;
+  }
+  float acc = acc0 + acc1;
   acc = wave_sum(acc);
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   if (lane == 0) ws[wave] = acc;
@@ -45,7 +71,7 @@ __global__ __launch_bounds__(kLossThreads) void sq_diff_sum_kernel(const float *
   if (threadIdx.x == 0) {
     float s = 0.f;
 #pragma unroll
-    for (int w = 0; w < kLossThreads / kWave; ++w) s += ws[w];
+    for (int w = 0; w < kSumThreads / kWave; ++w) s += ws[w];
     partial[blockIdx.x] = s;
     __threadfence();
     last = atomicAdd(ticket, 1u) == gridDim.x - 1;
@@ -53,23 +79,19 @@ __global__ __launch_bounds__(kLossThreads) void sq_diff_sum_kernel(const float *
   __syncthreads();
   if (!last) return;
   __threadfence();
-  // every thread adds its 4 consecutive partials, then a fixed shuffle tree
-  // and the waves in order: the same order on every launch
+  // thread b takes partial b, then a fixed shuffle tree and the waves in
+  // order: the same order on every launch
   double v = 0.0;
-#pragma unroll
-  for (int q = 0; q < kLossBlocks / kLossThreads; ++q) {
-    const unsigned b = threadIdx.x * (kLossBlocks / kLossThreads) + q;
-    if (b < gridDim.x) v += double(__hip_atomic_load(partial + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  }
+  if (threadIdx.x < gridDim.x)
+    v = double(__hip_atomic_load(partial + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
-  __shared__ double wd[kLossThreads / kWave];
   if (lane == 0) wd[wave] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double total = 0.0;
 #pragma unroll
-    for (int w = 0; w < kLossThreads / kWave; ++w) total += wd[w];
+    for (int w = 0; w < kSumThreads / kWave; ++w) total += wd[w];
     out[0] = float(total);
     __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }

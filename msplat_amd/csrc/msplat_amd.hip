@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -54,6 +55,7 @@ hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 struct TimingState {
   std::atomic<bool> on{false};
   std::mutex mu;
+  std::set<std::string> only;  // kernels to time; empty: all
   std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> recorded;
   std::vector<hipEvent_t> pool;
 };
@@ -79,6 +81,7 @@ class Timed {
     TimingState &t = timing();
     if (!t.on.load(std::memory_order_relaxed)) return;
     std::lock_guard<std::mutex> g(t.mu);
+    if (!t.only.empty() && !t.only.count(name)) return;
     a_ = pooled_event(t);
     b_ = pooled_event(t);
     if (a_ && b_ && hipEventRecord(a_, s_) == hipSuccess) return;
@@ -553,7 +556,7 @@ extern "C" {
 
 // ---------------------------------------------------------------- loss
 
-size_t msplat_sq_diff_sum_workspace_bytes(void) { return align256(sizeof(float) * kLossBlocks) + 256; }
+size_t msplat_sq_diff_sum_workspace_bytes(void) { return align256(sizeof(float) * kSumBlocks) + 256; }
 
 int msplat_sq_diff_sum(const float *a, const float *t, int64_t n, void *workspace, size_t workspace_bytes,
                        float *out, void *stream) {
@@ -565,12 +568,12 @@ int msplat_sq_diff_sum(const float *a, const float *t, int64_t n, void *workspac
   REQUIRE(workspace && (n == 0 || (a && t)), "sq_diff_sum: null pointer");
   hipStream_t s = as_stream(stream);
   float *partial = static_cast<float *>(workspace);
-  uint32_t *ticket = reinterpret_cast<uint32_t *>(static_cast<char *>(workspace) + align256(sizeof(float) * kLossBlocks));
+  uint32_t *ticket = reinterpret_cast<uint32_t *>(static_cast<char *>(workspace) + align256(sizeof(float) * kSumBlocks));
   launch_fill_zero_words({{ticket, 1}, {out, 1}}, s);
   if (n == 0) return launched("sq_diff_sum");
-  const int64_t want = (n + 4 * kLossThreads - 1) / (4 * kLossThreads);
-  const unsigned blocks = unsigned(want < kLossBlocks ? want : kLossBlocks);
-  sq_diff_sum_kernel<<<blocks, kLossThreads, 0, s>>>(a, t, n, partial, ticket, out);
+  const int64_t want = (n + 4 * kSumThreads - 1) / (4 * kSumThreads);
+  const unsigned blocks = unsigned(want < kSumBlocks ? want : kSumBlocks);
+  sq_diff_sum_kernel<<<blocks, kSumThreads, 0, s>>>(a, t, n, partial, ticket, out);
   return launched("sq_diff_sum");
 }
 
@@ -590,6 +593,24 @@ const char *msplat_last_error(void) { return g_error.c_str(); }
 
 int msplat_timing_enable(int on) {
   return timing().on.exchange(on != 0) ? 1 : 0;
+}
+
+int msplat_timing_only(const char *kernels) {
+  TimingState &t = timing();
+  std::lock_guard<std::mutex> g(t.mu);
+  t.only.clear();
+  if (!kernels) return MSPLAT_OK;
+  std::string cur;
+  for (const char *c = kernels;; ++c) {
+    if (*c == ',' || *c == '\0') {
+      if (!cur.empty()) t.only.insert(cur);
+      cur.clear();
+      if (*c == '\0') break;
+    } else if (*c != ' ') {
+      cur += *c;
+    }
+  }
+  return MSPLAT_OK;
 }
 
 int msplat_timing_read(const char *kernel, double *total_ms, int64_t *launches) {
