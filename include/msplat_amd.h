@@ -326,6 +326,24 @@ int msplat_bin_views_prepare(const float *uv, const float *depth,
                              const int64_t *count, int cell, uint32_t *rect,
                              void *workspace, size_t workspace_bytes,
                              int64_t *d_count, void *stream);
+/* msplat_bin_views_prepare in two calls with its arguments, on one stream:
+ * _count writes rect and *d_count (the host may copy the count as soon as
+ * _count's work is done), _order then puts the listed Gaussians in depth
+ * order (the bulk of step 1), so that the count's round trip to the host
+ * overlaps the sort.  *d_count = -1 reports an expired look-back wait;
+ * 2^31 means the entries reach 2^31 - 1 (more than int32 lists index). */
+int msplat_bin_views_count(const float *uv, const float *depth,
+                           const int32_t *radius, int64_t n, int B,
+                           const int *W, const int *H, const int64_t *first,
+                           const int64_t *count, int cell, uint32_t *rect,
+                           void *workspace, size_t workspace_bytes,
+                           int64_t *d_count, void *stream);
+int msplat_bin_views_order(const float *uv, const float *depth,
+                           const int32_t *radius, int64_t n, int B,
+                           const int *W, const int *H, const int64_t *first,
+                           const int64_t *count, int cell, uint32_t *rect,
+                           void *workspace, size_t workspace_bytes,
+                           int64_t *d_count, void *stream);
 size_t msplat_bin_views_emit_workspace_bytes(int64_t count);
 int msplat_bin_views_emit(const float *uv, const float *depth,
                           const int32_t *radius, int64_t n, int B,

@@ -63,6 +63,8 @@ SIGNATURES = {
     "msplat_sq_diff_grad": (I32, [P, P, I64, P, P, P]),
     "msplat_bin_views_prepare_workspace_bytes": (SZ, [I64]),
     "msplat_bin_views_prepare": (I32, [P, P, P, I64, I32, P, P, P, P, I32, P, P, SZ, P, P]),
+    "msplat_bin_views_count": (I32, [P, P, P, I64, I32, P, P, P, P, I32, P, P, SZ, P, P]),
+    "msplat_bin_views_order": (I32, [P, P, P, I64, I32, P, P, P, P, I32, P, P, SZ, P, P]),
     "msplat_bin_views_emit_workspace_bytes": (SZ, [I64]),
     "msplat_bin_views_emit": (I32, [P, P, P, I64, I32, P, P, P, P, I32, P, I64, P, P, SZ, P, P, P]),
     "msplat_rasterize_views_forward": (

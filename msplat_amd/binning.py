@@ -16,6 +16,7 @@ images, final_T and ncontrib are identical to sort_gaussian + alpha_blending
 from __future__ import annotations
 
 import dataclasses
+import threading
 
 import torch
 
@@ -54,9 +55,44 @@ def _planned(capacity):
     return _plan.capacity()
 
 
-def _observe(count: torch.Tensor, capacity):
+def _observe(count: torch.Tensor, capacity, k: int):
     if _plan is not None:
-        _plan.observe(int(count.item()) if capacity is None else -1, count)
+        _plan.observe(k if capacity is None else -1, count)
+
+
+_host = threading.local()
+
+
+def _host_count() -> torch.Tensor:
+    """A pinned host word per thread for the entry count's copy."""
+    buf = getattr(_host, "count", None)
+    if buf is None:
+        buf = _host.count = torch.empty(1, dtype=torch.int64, pin_memory=True)
+    return buf
+
+
+def prepare_views(u, d, r, n: int, B: int, W, H, F, N, cell: int, rect, prep, prep_bytes: int, count,
+                  capacity) -> int:
+    """Step 1 of the binning on the current stream: the counting pass, then the
+    depth order.  Without a capacity, the entry count is copied to the host
+    between the two and read once the sort is queued behind it (the GPU sorts
+    while the count travels); returns that count, or the capacity."""
+    args = (_lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, B, W, H, F, N, int(cell), _lib.ptr(rect), _lib.ptr(prep),
+            prep_bytes, _lib.ptr(count), _lib.stream_ptr())
+    _lib.call("msplat_bin_views_count", *args)
+    if capacity is None:
+        host = _host_count()
+        host.copy_(count, non_blocking=True)
+        ready = torch.cuda.Event()
+        ready.record()
+    _lib.call("msplat_bin_views_order", *args)
+    if capacity is not None:
+        return int(capacity)
+    ready.synchronize()  # the only host synchronisation of the binning
+    k = int(host[0])
+    if k < 0:
+        raise RuntimeError("cell binning: a bounded look-back wait expired")
+    return k
 
 
 def cell_grid(W: int, H: int, cell: int) -> tuple[int, int]:
@@ -82,11 +118,9 @@ def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8, capacity: in
         prep_bytes = int(_lib.query("msplat_bin_prepare_workspace_bytes", n, W, H, cell))
         prep = torch.empty(max(prep_bytes, 1), dtype=torch.uint8, device=dev)
         count = torch.empty(1, dtype=torch.int64, device=dev)
-        _lib.call("msplat_bin_prepare", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, W, H, cell, _lib.ptr(rect),
-                  _lib.ptr(prep), prep_bytes, _lib.ptr(count), stream)
-        # the only host synchronisation of the binning, unless a capacity is given
-        k = int(count.item()) if capacity is None else int(capacity)
-        _observe(count, capacity)
+        Wa, Ha = (_lib.I32 * 1)(W), (_lib.I32 * 1)(H)
+        k = prepare_views(u, d, r, n, 1, Wa, Ha, None, None, cell, rect, prep, prep_bytes, count, capacity)
+        _observe(count, capacity, k)
         cell_ids = torch.empty(k, dtype=torch.int32, device=dev)
         cell_range = torch.empty((ncx * ncy, 2), dtype=torch.int32, device=dev)
         emit_bytes = int(_lib.query("msplat_bin_emit_workspace_bytes", n, k, W, H, cell))

@@ -229,11 +229,12 @@ inline void launch_fill_zero(T *p, int64_t n, hipStream_t s) {
   fill_zero<T><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(p, n);
 }
 
-// Up to 4 ranges of 32-bit words zeroed by one launch (blockIdx.y picks the
-// range): one launch instead of one per gradient buffer.
+// Up to kZeroRanges ranges of 32-bit words zeroed by one launch (blockIdx.y
+// picks the range): one launch instead of one per buffer.
+constexpr int kZeroRanges = 6;
 struct ZeroRanges {
-  uint32_t *p[4];
-  int64_t n[4];
+  uint32_t *p[kZeroRanges];
+  int64_t n[kZeroRanges];
 };
 
 __global__ void fill_zero_ranges(ZeroRanges r) {
@@ -243,22 +244,29 @@ __global__ void fill_zero_ranges(ZeroRanges r) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
 }
 
-// ranges: (pointer, 32-bit words); null pointers and empty ranges are skipped.
+// ranges: (pointer, 32-bit words); null pointers and empty ranges are
+// skipped; more than kZeroRanges ranges take further launches.
 inline void launch_fill_zero_words(std::initializer_list<std::pair<void *, int64_t>> ranges, hipStream_t s) {
   ZeroRanges r{};
   unsigned k = 0;
   int64_t most = 0;
+  auto flush = [&]() {
+    if (k == 0) return;
+    int64_t blocks = (most + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    fill_zero_ranges<<<dim3(unsigned(blocks), k), dim3(256), 0, s>>>(r);
+    r = ZeroRanges{};
+    k = 0;
+    most = 0;
+  };
   for (const auto &pr : ranges) {
-    if (!pr.first || pr.second <= 0 || k == 4) continue;
+    if (!pr.first || pr.second <= 0) continue;
     r.p[k] = static_cast<uint32_t *>(pr.first);
     r.n[k] = pr.second;
     most = pr.second > most ? pr.second : most;
-    ++k;
+    if (++k == kZeroRanges) flush();
   }
-  if (k == 0) return;
-  int64_t blocks = (most + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  fill_zero_ranges<<<dim3(unsigned(blocks), k), dim3(256), 0, s>>>(r);
+  flush();
 }
 
 inline unsigned grid_1d(int64_t n, int threads) {

@@ -129,8 +129,12 @@ struct Carver {
 };
 
 struct PrepLayout {
-  size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, area_s, rect_s, total;
+  size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, area_s, rect_s, bstatus, total;
 };
+
+// Workgroups of the binning's single-pass scans over n rows (at least one
+// look-back word is always carved).
+int64_t bin_blocks(int64_t n) { return n > 0 ? (n + sortk::kThreads - 1) / sortk::kThreads : 1; }
 
 // (W, H are part of the size query so that the layout may depend on the
 // tile grid; it currently does not.)
@@ -155,6 +159,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
   L.nlisted = c.take(8);
   L.area_s = c.take(un * 4);  // cell counts and tile rectangles in depth order (cell binning)
   L.rect_s = c.take(un * 8);
+  L.bstatus = c.take(size_t(bin_blocks(n)) * 8);  // look-back words of the counting pass
   (void)W;
   (void)H;
   L.total = c.off;
@@ -162,7 +167,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
 }
 
 struct EmitLayout {
-  size_t ca, ga, cb, gb, owner, hist, sums, ghist, status, total;
+  size_t ca, ga, cb, gb, owner, hist, sums, ghist, status, estatus, total;
 };
 
 EmitLayout emit_layout(int64_t k, int W, int H) {
@@ -182,6 +187,7 @@ EmitLayout emit_layout(int64_t k, int W, int H) {
   L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
   L.ghist = c.take(size_t(sortk::ghist_words()) * 4);
   L.status = c.take(size_t(sortk::status_words(k)) * 4);
+  L.estatus = c.take(size_t(bin_blocks(k)) * 8);  // look-back words of the cell-list emission
   L.total = c.off;
   return L;
 }
@@ -287,27 +293,21 @@ void sh_bwd_launch(int64_t n, int C, const float *shs, const Dir &dir, const uin
     default: CALL(10); break;       \
   }
 
+sortk::RadixBuffers prep_radix(void *ws, const PrepLayout &L) {
+  return sortk::RadixBuffers{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
+                             at<uint32_t>(ws, L.status)};
+}
+
 // Stable order of the prepared (depth bits, id) pairs by depth -- four 8-bit
 // LSD passes, ids end in L.va; only the first *dlen pairs when dlen is given
 // -- and the exclusive scan of the per-Gaussian box counts in that order
 // (total -> *d_count).
 void depth_order(void *ws, const PrepLayout &L, int64_t n, int64_t *d_count, hipStream_t s,
-                 const int64_t *dlen = nullptr, const uint2 *rect = nullptr, int S = 1) {
+                 const int64_t *dlen = nullptr) {
   uint32_t *ka = at<uint32_t>(ws, L.ka), *va = at<uint32_t>(ws, L.va);
   uint32_t *kb = at<uint32_t>(ws, L.kb), *vb = at<uint32_t>(ws, L.vb);
-  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
-                               at<uint32_t>(ws, L.status)};
+  const sortk::RadixBuffers rb = prep_radix(ws, L);
   int32_t *area = at<int32_t>(ws, L.area);
-  if (rect) {  // cell binning: carry cell counts and rectangles into depth order, scan them in place
-    const sortk::RadixGather gather{rect, S, at<int32_t>(ws, L.area_s), at<uint2>(ws, L.rect_s)};
-    {
-      Timed timed_("depth_sort", s);
-      sortk::radix_sort(ka, va, kb, vb, n, 32, 4, rb, s, dlen, nullptr, gather);  // a -> b -> a -> b -> a
-    }
-    sortk::exclusive_scan<int32_t, int64_t>(gather.oarea, nullptr, n, at<int64_t>(ws, L.sums),
-                                            at<int64_t>(ws, L.offs), d_count, s, dlen);
-    return;
-  }
   {
     Timed timed_("depth_sort", s);
     sortk::radix_sort(ka, va, kb, vb, n, 32, 4, rb, s, dlen);  // a -> b -> a -> b -> a
@@ -330,11 +330,14 @@ const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t
   uint32_t *ca = at<uint32_t>(ws, L.ca), *ga = at<uint32_t>(ws, L.ga);
   uint32_t *cb = at<uint32_t>(ws, L.cb), *gb = at<uint32_t>(ws, L.gb);
   int64_t *owner = at<int64_t>(ws, L.owner);
-  if (direct) {  // a few entries per Gaussian: one thread each
+  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
+                               at<uint32_t>(ws, L.status)};
+  if (direct) {  // cell lists: one thread per listed row, offsets by a single-pass scan
     Timed timed_("emit", s);
-    sortk::emit_direct_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
-        order, offs, at<int32_t>(prepared, P.area_s), at<uint2>(prepared, P.rect_s), n,
-        at<int64_t>(prepared, P.nlisted), k, dlen, vw, S, ca, ga);
+    const int64_t rows = n < k ? n : k;  // every listed row has a cell: at most k rows hold slots below k
+    sortk::emit_direct_kernel<<<grid_1d(rows, sortk::kThreads), sortk::kThreads, 0, s>>>(
+        order, at<int32_t>(prepared, P.area_s), at<uint2>(prepared, P.rect_s), n, at<int64_t>(prepared, P.nlisted),
+        k, dlen, vw, S, ca, ga, at<uint64_t>(ws, L.estatus), bin_blocks(k), sortk::radix_error_word(rb));
   } else {  // equal output chunks per workgroup (tile rectangles of thousands of tiles)
     sortk::emit_owner_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
         at<int32_t>(prepared, P.area), order, offs, n, sortk::emit_chunks(k), owner);
@@ -342,12 +345,11 @@ const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t
     sortk::emit_chunk_kernel<<<dim3(unsigned(sortk::emit_chunks(k))), sortk::kThreads, 0, s>>>(
         uv, depth, radius, order, offs, owner, n, k, dlen, vw, S, ca, ga);
   }
-  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
-                               at<uint32_t>(ws, L.status)};
   const int bits = bit_length(uint32_t(ncls));  // classes 0..ncls
   const int npass = (bits + 7) / 8;
   Timed timed_("class_sort", s);
-  return sortk::radix_sort(ca, ga, cb, gb, k, bits, npass, rb, s, dlen, ids) ? cb : ca;
+  // cell lists: the emission's launch zeroed ghist
+  return sortk::radix_sort(ca, ga, cb, gb, k, bits, npass, rb, s, dlen, ids, {}, direct) ? cb : ca;
 }
 
 // Cells of S x S tiles on the tile grid of a W x H image.
@@ -932,9 +934,12 @@ size_t msplat_bin_views_prepare_workspace_bytes(int64_t rows) {
   return prep_layout(rows, 1, 1).total;
 }
 
-int msplat_bin_views_prepare(const float *uv, const float *depth, const int32_t *radius, int64_t n, int B,
-                             const int *W, const int *H, const int64_t *first, const int64_t *count, int cell,
-                             uint32_t *rect, void *workspace, size_t workspace_bytes, int64_t *d_count,
+// Step 1 of the cell binning in two halves (stages bit 1: the counting pass,
+// bit 2: the depth order), so that the host can read the count while the GPU
+// sorts.
+int bin_views_prepare_stages(int stages, const float *uv, const float *depth, const int32_t *radius, int64_t n,
+                             int B, const int *W, const int *H, const int64_t *first, const int64_t *count,
+                             int cell, uint32_t *rect, void *workspace, size_t workspace_bytes, int64_t *d_count,
                              void *stream) {
   REQUIRE(n >= 0 && B >= 1, "bin_prepare: n = %lld, views = %d", (long long)n, B);
   REQUIRE(valid_cell(cell), "bin_prepare: cell = %d tiles", cell);
@@ -947,25 +952,59 @@ int msplat_bin_views_prepare(const float *uv, const float *depth, const int32_t 
   if (workspace_bytes < L.total)
     return fail(MSPLAT_ERR_WORKSPACE, "bin_prepare: workspace %zu < %zu bytes", workspace_bytes, L.total);
   REQUIRE(workspace, "bin_prepare: null workspace");
-  hipStream_t s = as_stream(stream);
-  int32_t *flag = at<int32_t>(workspace, L.flag);
-  launch_fill_zero(flag, 1, s);
   if (m > 0) {
     REQUIRE(uv && depth && radius && rect, "bin_prepare: null pointer");
     REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "bin_prepare: rect must be 8-byte aligned");
-    int32_t *area = at<int32_t>(workspace, L.area);
-    uint32_t *listed = at<uint32_t>(workspace, L.kb), *pos = at<uint32_t>(workspace, L.vb);
-    int64_t *nlisted = at<int64_t>(workspace, L.nlisted);
-    sortk::bin_prepare_kernel<<<grid_1d(m, sortk::kThreads), sortk::kThreads, 0, s>>>(
-        uv, depth, radius, m, vw, cell, area, listed, reinterpret_cast<uint2 *>(rect));
-    // listed rows first (stable), then the depth sort of those only
-    sortk::exclusive_scan<uint32_t, uint32_t>(listed, nullptr, m, at<int64_t>(workspace, L.sums), pos, nlisted, s);
-    sortk::partition_kernel<<<grid_1d(m, sortk::kThreads), sortk::kThreads, 0, s>>>(
-        depth, area, pos, m, nlisted, at<uint32_t>(workspace, L.ka), at<uint32_t>(workspace, L.va));
-    depth_order(workspace, L, m, d_count, s, nlisted, reinterpret_cast<const uint2 *>(rect), cell);
   }
-  sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, d_count, m);
+  hipStream_t s = as_stream(stream);
+  const sortk::RadixBuffers rb = prep_radix(workspace, L);
+  int64_t *nlisted = at<int64_t>(workspace, L.nlisted);
+  uint32_t *ka = at<uint32_t>(workspace, L.ka), *va = at<uint32_t>(workspace, L.va);
+  if (stages & 1) {
+    // one launch zeroes the depth sort's digit totals, tickets and error
+    // word, the look-back words, and the counts (which stay 0 when m = 0)
+    uint64_t *bstatus = at<uint64_t>(workspace, L.bstatus);
+    launch_fill_zero_words(
+        {{rb.ghist, sortk::ghist_words()}, {bstatus, 2 * bin_blocks(m)}, {nlisted, 2}, {d_count, 2}}, s);
+    if (m > 0)
+      sortk::bin_prepare_kernel<<<grid_1d(m, sortk::kThreads), sortk::kThreads, 0, s>>>(
+          uv, depth, radius, m, vw, cell, reinterpret_cast<uint2 *>(rect), ka, va, bstatus, nlisted, d_count,
+          sortk::radix_error_word(rb));
+  }
+  if ((stages & 2) && m > 0) {
+    // the listed rows (first *nlisted pairs) by depth; the last pass carries
+    // their cell counts and rectangles into depth order
+    const sortk::RadixGather gather{reinterpret_cast<const uint2 *>(rect), cell, at<int32_t>(workspace, L.area_s),
+                                    at<uint2>(workspace, L.rect_s)};
+    uint32_t *kb = at<uint32_t>(workspace, L.kb), *vb = at<uint32_t>(workspace, L.vb);
+    Timed timed_("depth_sort", s);
+    sortk::radix_sort(ka, va, kb, vb, m, 32, 4, rb, s, nlisted, nullptr, gather, true);  // a -> b -> a -> b -> a
+  }
   return launched("bin_prepare");
+}
+
+int msplat_bin_views_prepare(const float *uv, const float *depth, const int32_t *radius, int64_t n, int B,
+                             const int *W, const int *H, const int64_t *first, const int64_t *count, int cell,
+                             uint32_t *rect, void *workspace, size_t workspace_bytes, int64_t *d_count,
+                             void *stream) {
+  return bin_views_prepare_stages(3, uv, depth, radius, n, B, W, H, first, count, cell, rect, workspace,
+                                  workspace_bytes, d_count, stream);
+}
+
+int msplat_bin_views_count(const float *uv, const float *depth, const int32_t *radius, int64_t n, int B,
+                           const int *W, const int *H, const int64_t *first, const int64_t *count, int cell,
+                           uint32_t *rect, void *workspace, size_t workspace_bytes, int64_t *d_count,
+                           void *stream) {
+  return bin_views_prepare_stages(1, uv, depth, radius, n, B, W, H, first, count, cell, rect, workspace,
+                                  workspace_bytes, d_count, stream);
+}
+
+int msplat_bin_views_order(const float *uv, const float *depth, const int32_t *radius, int64_t n, int B,
+                           const int *W, const int *H, const int64_t *first, const int64_t *count, int cell,
+                           uint32_t *rect, void *workspace, size_t workspace_bytes, int64_t *d_count,
+                           void *stream) {
+  return bin_views_prepare_stages(2, uv, depth, radius, n, B, W, H, first, count, cell, rect, workspace,
+                                  workspace_bytes, d_count, stream);
 }
 
 size_t msplat_bin_views_emit_workspace_bytes(int64_t count) {
@@ -988,10 +1027,15 @@ int msplat_bin_views_emit(const float *uv, const float *depth, const int32_t *ra
   const EmitLayout L = emit_layout(count, 1, 1);
   if (workspace_bytes < L.total)
     return fail(MSPLAT_ERR_WORKSPACE, "bin_emit: workspace %zu < %zu bytes", workspace_bytes, L.total);
+  REQUIRE(count == 0 || (uv && depth && radius && prepared && workspace && cell_ids), "bin_emit: null pointer");
   hipStream_t s = as_stream(stream);
-  launch_fill_zero(cell_range, 2 * int64_t(vw.cell0[B]), s);
+  // one launch zeroes the ranges, the class sort's digit totals and the
+  // emission's look-back words
+  launch_fill_zero_words({{cell_range, 2 * int64_t(vw.cell0[B])},
+                          {count > 0 ? at<uint32_t>(workspace, L.ghist) : nullptr, sortk::ghist_words()},
+                          {count > 0 ? at<uint64_t>(workspace, L.estatus) : nullptr, 2 * bin_blocks(count)}},
+                         s);
   if (count == 0) return launched("bin_emit");
-  REQUIRE(uv && depth && radius && prepared && workspace && cell_ids, "bin_emit: null pointer");
   const uint32_t *sk = emit_by_class(uv, depth, radius, vw.vbase[B], vw, cell, prepared, count, d_count, workspace,
                                      L, reinterpret_cast<uint32_t *>(cell_ids), s, true);
   sortk::range_vec_kernel<<<grid_1d((count + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, count, d_count,

@@ -404,6 +404,8 @@ struct RadixBuffers {
 };
 
 MSPLAT_HD int64_t ghist_words() { return int64_t(kMaxPasses) * 256 + kMaxPasses + 1; }
+// The error word at the end of ghist: set when a bounded wait expires.
+inline uint32_t *radix_error_word(const RadixBuffers &rb) { return rb.ghist + kMaxPasses * 256 + kMaxPasses; }
 MSPLAT_HD int64_t status_words(int64_t len) { return int64_t(kMaxPasses) * 256 * radix_tiles(len); }
 
 // Digit totals of every pass in one read of the keys (pass p covers bits
@@ -454,15 +456,16 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
 // ...; returns which buffer holds the result (0: a, 1: b).  Onesweep needs
 // cap < 2^30 (the look-back words carry 30-bit counts); larger inputs use the
 // multi-launch passes.  vlast, when given, receives the values of the last
-// pass instead of the other buffer.
+// pass instead of the other buffer.  ghist_zeroed: the caller has zeroed
+// rb.ghist (with other buffers, in one launch).
 inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, int64_t cap, int bits, int npass,
                       const RadixBuffers &rb, hipStream_t s, const int64_t *dlen = nullptr,
-                      uint32_t *vlast = nullptr, RadixGather gather = {}) {
+                      uint32_t *vlast = nullptr, RadixGather gather = {}, bool ghist_zeroed = false) {
   const int db = (bits + npass - 1) / npass;
   const int64_t nt = radix_tiles(cap);
   const bool onesweep = cap < (int64_t(1) << 30) && npass <= kMaxPasses;
   if (nt > 0 && onesweep) {
-    launch_fill_zero(rb.ghist, ghist_words(), s);
+    if (!ghist_zeroed) launch_fill_zero(rb.ghist, ghist_words(), s);
     const int64_t nst = int64_t(npass) * 256 * nt;
     const unsigned hb = unsigned(imin(int(nt), 1024));
     onesweep_hist_kernel<<<dim3(hb), dim3(kThreads), 0, s>>>(ka, cap, dlen, npass, db, bits, rb.ghist, rb.status,
@@ -513,84 +516,167 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(const float *__restri
   if (tiles && tiles[i] != a) atomicOr(flag, 1);
 }
 
-// Cell binning for the rasterizer: the number of S x S-tile cells that each
-// Gaussian's tile rectangle touches (0 for a negative depth, whose reference
-// key sorts into no tile, and for an empty rectangle), depth bits as the first
-// sort key, identity as the value, and the packed tile rectangle.
-// Entries are the per-view rows of the views in `v`; listed[i] = area > 0.
-__global__ __launch_bounds__(kThreads) void bin_prepare_kernel(const float *__restrict__ uv,
-                                                               const float *__restrict__ depth,
-                                                               const int32_t *__restrict__ radius, int64_t n,
-                                                               Views v, int S,
-                                                               int32_t *__restrict__ area,
-                                                               uint32_t *__restrict__ listed,
-                                                               uint2 *__restrict__ rect) {
-  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const int b = view_of_row(v, i);
-  const int r = radius[i];
-  const uint32_t db = fbits(depth[i]);
-  int a = 0;
-  TileRect t{0, 0, 0, 0};
-  if (r > 0 && !(db >> 31)) {
-    t = tile_rect(uv[2 * i], uv[2 * i + 1], r, v.gx[b], v.gy[b]);
-    if (t.area() > 0) a = rect_cells(pack_rect(t), S);
+// ---- single-pass scans with a decoupled look-back (cell binning) -----------
+// Workgroup b publishes its aggregate in status[b], then one wave adds up its
+// predecessors' words, 64 at a time, back to the nearest one that carries an
+// inclusive prefix, and b publishes its own inclusive prefix.  A status word
+// holds a flag (2 bits) and two 31-bit sums (saturating).  Workgroups are
+// dispatched in index order and never wait on a later one, so every
+// predecessor is running or done and every wait ends; waits are also bounded
+// (kLookbackSpins polls) and report an expired one.  No ticket counter: one
+// atomic per workgroup on a shared word would serialise thousands of
+// workgroups.  status must be zero before the launch.
+constexpr uint64_t kLbAgg = 1ull << 62, kLbIncl = 2ull << 62, kLbFlags = 3ull << 62;
+constexpr int64_t kLbField = (int64_t(1) << 31) - 1;
+
+MSPLAT_DEV uint64_t lb_word(uint64_t flag, int64_t hi, int64_t lo) {
+  const uint64_t h = uint64_t(hi < kLbField ? hi : kLbField), l = uint64_t(lo < kLbField ? lo : kLbField);
+  return flag | (h << 31) | l;
+}
+
+MSPLAT_DEV int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Called by one whole wave of workgroup b >= 1: the sums of both fields over
+// workgroups [0, b); false if a wait expired.
+MSPLAT_DEV bool lookback_pair(const uint64_t *status, int64_t b, int64_t &hi, int64_t &lo) {
+  const int lane = __lane_id();
+  hi = 0;
+  lo = 0;
+  int64_t spins = 0;
+  for (int64_t p = b - 1;; p -= kWave) {
+    const int64_t q = p - lane;  // before workgroup 0: an inclusive zero
+    uint64_t w = q >= 0 ? __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kLbIncl;
+    while (__ballot((w & kLbFlags) == 0ull) != 0ull) {
+      if (++spins >= kLookbackSpins) return false;
+      if ((w & kLbFlags) == 0ull) w = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint64_t incl = __ballot((w & kLbFlags) == kLbIncl);
+    const int stop = incl ? __ffsll((unsigned long long)incl) - 1 : kWave - 1;  // nearest inclusive word
+    const bool take = lane <= stop;
+    hi += wave_sum_i64(take ? int64_t((w >> 31) & uint64_t(kLbField)) : 0);
+    lo += wave_sum_i64(take ? int64_t(w & uint64_t(kLbField)) : 0);
+    if (incl) return true;
   }
-  area[i] = a;
-  listed[i] = a > 0 ? 1u : 0u;
-  rect[i] = pack_rect(t);
 }
 
-// Stable partition of the rows: listed rows (pos[i] = their rank, from an
-// exclusive scan of `listed`) to [0, nlisted) with their depth bits as sort
-// keys, the others after them, so the depth sort only has to order
-// [0, nlisted) and every row still appears in the order array.
-__global__ __launch_bounds__(kThreads) void partition_kernel(const float *__restrict__ depth,
-                                                             const int32_t *__restrict__ area,
-                                                             const uint32_t *__restrict__ pos, int64_t n,
-                                                             const int64_t *__restrict__ nlisted,
-                                                             uint32_t *__restrict__ keys,
-                                                             uint32_t *__restrict__ vals) {
+// Publishes workgroup b's sums (nh, nl) and returns its exclusive prefixes
+// (wave 0 calls it; lane 0 stores).  false if a wait expired.
+MSPLAT_DEV bool lookback_publish(uint64_t *status, int64_t b, int64_t nh, int64_t nl, int64_t &ph, int64_t &pl) {
+  const int lane = __lane_id();
+  ph = 0;
+  pl = 0;
+  bool ok = true;
+  if (b > 0) {
+    if (lane == 0) __hip_atomic_store(status + b, lb_word(kLbAgg, nh, nl), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ok = lookback_pair(status, b, ph, pl);
+  }
+  // an expired wait leaves the aggregate: later workgroups look past it
+  if (lane == 0 && ok)
+    __hip_atomic_store(status + b, lb_word(kLbIncl, ph + nh, pl + nl), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return ok;
+}
+
+// Cell binning, first pass (one launch).  Per row (the per-view rows of the
+// views in `v`): its packed tile rectangle, and the number of S x S-tile cells
+// the rectangle touches (0 for a negative depth, whose reference key sorts
+// into no tile, and for an empty rectangle).  The rows with cells ("listed")
+// get their depth bits and row index written, in row order, to the first
+// slots of keys / vals, at positions from a single-pass scan; the last
+// workgroup stores the number of listed rows and of cell entries (-1 when a
+// wait expired; 2^31 when the entries reach 2^31 - 1, which no int32 list
+// holds).
+__global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
+    const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius, int64_t n,
+    Views v, int S, uint2 *__restrict__ rect, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+    uint64_t *__restrict__ status, int64_t *__restrict__ nlisted, int64_t *__restrict__ count,
+    uint32_t *__restrict__ error) {
+  __shared__ int64_t s_base;
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const int64_t p = pos[i];
-  const int64_t j = area[i] > 0 ? p : *nlisted + (i - p);
-  keys[j] = fbits(depth[i]);
-  vals[j] = uint32_t(i);
+  int a = 0;
+  uint32_t db = 0u;
+  if (i < n) {
+    const int b = view_of_row(v, i);
+    const int r = radius[i];
+    db = fbits(depth[i]);
+    TileRect t{0, 0, 0, 0};
+    if (r > 0 && !(db >> 31)) {
+      t = tile_rect(uv[2 * i], uv[2 * i + 1], r, v.gx[b], v.gy[b]);
+      if (t.area() > 0) a = rect_cells(pack_rect(t), S);
+    }
+    rect[i] = pack_rect(t);
+  }
+  int64_t nl, ne;
+  const int64_t rank = block_excl_scan(a > 0 ? 1 : 0, &nl);
+  block_excl_scan(a, &ne);
+  if (threadIdx.x < kWave) {
+    int64_t pl, pe;
+    const bool ok = lookback_publish(status, blockIdx.x, nl, ne, pl, pe);
+    if (threadIdx.x == 0) {
+      s_base = pl;
+      if (!ok) atomicOr(error, 1u);
+      if (blockIdx.x == gridDim.x - 1) {
+        *nlisted = pl + nl;
+        const int64_t e = pe + ne;
+        *count = ok ? (e >= kLbField ? kLbField + 1 : e) : -1;
+      }
+    }
+  }
+  __syncthreads();
+  if (a > 0) {
+    keys[s_base + rank] = db;
+    vals[s_base + rank] = uint32_t(i);
+  }
 }
 
-// Emission for cell lists (a few cells per Gaussian): one thread per
-// depth-ordered position i writes its Gaussian's cells, row-major, at slots
-// offs[i]... (slots past the capacity are not written).
+// Cell binning, emission (one launch): row i of the depth order writes its
+// Gaussian's cells, row-major, at slots o(i)..., o = the exclusive scan of
+// the rows' cell counts in depth order, computed here by a single-pass scan
+// (status: one word per workgroup, zero before the launch).  Slots past the
+// capacity are not written; workgroups past nstatus hold no slot below it
+// (every listed row has a cell, so o(i) >= i) and leave at once.
 __global__ __launch_bounds__(kThreads) void emit_direct_kernel(
-    const uint32_t *__restrict__ order, const int64_t *__restrict__ offs, const int32_t *__restrict__ area_s,
-    const uint2 *__restrict__ rect_s, int64_t n, const int64_t *__restrict__ nrows, int64_t cap,
-    const int64_t *__restrict__ dlen, Views v, int S, uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
+    const uint32_t *__restrict__ order, const int32_t *__restrict__ area_s, const uint2 *__restrict__ rect_s,
+    int64_t n, const int64_t *__restrict__ nrows, int64_t cap, const int64_t *__restrict__ dlen, Views v, int S,
+    uint32_t *__restrict__ cls, uint32_t *__restrict__ gid, uint64_t *__restrict__ status, int64_t nstatus,
+    uint32_t *__restrict__ error) {
   // Row i of the depth order: its Gaussian row order[i], cell count area_s[i]
   // and tile rectangle rect_s[i] (both carried into depth order by the depth
   // sort's last pass, so every read here is coalesced).  Listed rows have
   // positive depth, so every entry has a cell class.
+  __shared__ int64_t s_base;
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   const int64_t rows = dev_len(nrows, n);  // rows past the listed ones have nothing to emit
-  if (int64_t(blockIdx.x) * kThreads >= rows) return;  // uniform
+  if (int64_t(blockIdx.x) * kThreads >= rows || int64_t(blockIdx.x) >= nstatus) return;  // uniform
   const int lane = threadIdx.x % kWave;
-  uint32_t g = 0u, base = 0u;
-  int a = 0, w = 1, ncx = 1;
-  int64_t o = 0;
-  const int64_t k = dev_len(dlen, cap);
-  if (i < rows) {
-    a = area_s[i];
-    if (a > 0) {
-      g = order[i];
-      o = offs[i];
-      const int b = view_of_row(v, g);
-      const uint2 rp = rect_s[i];
-      const int x0 = int(rp.x & 0xffffu), y0 = int(rp.x >> 16), x1 = int(rp.y & 0xffffu);
-      const int cx0 = x0 / S;
-      w = imax(1, (x1 + S - 1) / S - cx0);
-      ncx = v.ncx[b];
-      base = uint32_t(v.cell0[b] + (y0 / S) * ncx + cx0) + 1u;
+  const int a = i < rows ? area_s[i] : 0;
+  int64_t tot;
+  const int64_t ex = block_excl_scan(a, &tot);
+  if (threadIdx.x < kWave) {
+    int64_t unused, pe;
+    const bool ok = lookback_publish(status, blockIdx.x, 0, tot, unused, pe);
+    if (threadIdx.x == 0) {
+      s_base = pe;
+      if (!ok) atomicOr(error, 1u);
     }
+  }
+  __syncthreads();
+  uint32_t g = 0u, base = 0u;
+  int w = 1, ncx = 1;
+  const int64_t o = s_base + ex;
+  const int64_t k = dev_len(dlen, cap);
+  if (a > 0) {
+    g = order[i];
+    const int b = view_of_row(v, g);
+    const uint2 rp = rect_s[i];
+    const int x0 = int(rp.x & 0xffffu), y0 = int(rp.x >> 16), x1 = int(rp.y & 0xffffu);
+    const int cx0 = x0 / S;
+    w = imax(1, (x1 + S - 1) / S - cx0);
+    ncx = v.ncx[b];
+    base = uint32_t(v.cell0[b] + (y0 / S) * ncx + cx0) + 1u;
   }
   // A Gaussian's entries: the cells of its box, row by row, from offset o.
   // Small boxes are written by their own thread; large ones by the whole

@@ -121,11 +121,8 @@ def bin_views(uv, depth, radius, n: int, sizes, cell: int = 8, capacity: int | N
         prep_bytes = int(_lib.query("msplat_bin_views_prepare_workspace_bytes", R))
         prep = torch.empty(max(prep_bytes, 1), dtype=torch.uint8, device=dev)
         cnt = torch.empty(1, dtype=torch.int64, device=dev)
-        _lib.call("msplat_bin_views_prepare", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, B, W, H, F, N, int(cell),
-                  _lib.ptr(rect), _lib.ptr(prep), prep_bytes, _lib.ptr(cnt), stream)
-        # the only host synchronisation of the binning, unless a capacity is given
-        k = int(cnt.item()) if capacity is None else int(capacity)
-        binning._observe(cnt, capacity)
+        k = binning.prepare_views(u, d, r, n, B, W, H, F, N, cell, rect, prep, prep_bytes, cnt, capacity)
+        binning._observe(cnt, capacity, k)
         cell_ids = torch.empty(k, dtype=torch.int32, device=dev)
         cell_range = torch.empty((cells_of(sizes, cell), 2), dtype=torch.int32, device=dev)
         emit_bytes = int(_lib.query("msplat_bin_views_emit_workspace_bytes", k))
