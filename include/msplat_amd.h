@@ -114,6 +114,27 @@ int msplat_preprocess_backward(const float *xyz, const float *scale,
                                float *dL_dscale, float *dL_drotate,
                                float *dL_dintr, float *dL_dextr, void *stream);
 
+/* The same two, for a depth that is also a feature channel (no reference
+ * counterpart; rasterization_sh's depth channel): the forward also writes
+ * each depth to depth_col[i * ld_col] (NULL: nothing), the adjoint reads
+ * dL_ddepth[i * ld_ddepth], so neither needs a strided copy. */
+int msplat_preprocess_forward_col(const float *xyz, const float *scale,
+                                  const float *rotate, int64_t n,
+                                  const float *intr, const float *extr, int W,
+                                  int H, float nearest, float extent,
+                                  float *uv, float *depth, float *conic,
+                                  int32_t *radius, int32_t *tiles,
+                                  float *depth_col, int ld_col, void *stream);
+int msplat_preprocess_backward_ld(const float *xyz, const float *scale,
+                                  const float *rotate, int64_t n,
+                                  const float *intr, const float *extr,
+                                  const float *depth, const int32_t *radius,
+                                  const float *dL_duv, const float *dL_ddepth,
+                                  int ld_ddepth, const float *dL_dconic,
+                                  float *dL_dxyz, float *dL_dscale,
+                                  float *dL_drotate, float *dL_dintr,
+                                  float *dL_dextr, void *stream);
+
 /* ---- spherical harmonics, degree 0..10: msplat/src/compute_sh.cu:1696-1722
  *      (ext.cpp:22, compute_sh.py:39).  shs [n,C,D], D = (deg+1)^2. */
 int msplat_compute_sh_forward(const float *shs, const float *dirs,

@@ -727,28 +727,38 @@ int msplat_ewa_project_backward(const float *xyz, const float *cov3d, const floa
 
 // ---------------------------------------------------------------- fused
 
-int msplat_preprocess_forward(const float *xyz, const float *scale, const float *rotate, int64_t n,
-                              const float *intr, const float *extr, int W, int H, float nearest, float extent,
-                              float *uv, float *depth, float *conic, int32_t *radius, int32_t *tiles,
-                              void *stream) {
+int msplat_preprocess_forward_col(const float *xyz, const float *scale, const float *rotate, int64_t n,
+                                  const float *intr, const float *extr, int W, int H, float nearest, float extent,
+                                  float *uv, float *depth, float *conic, int32_t *radius, int32_t *tiles,
+                                  float *depth_col, int ld_col, void *stream) {
   REQUIRE(n >= 0, "preprocess_forward: n = %lld", (long long)n);
   REQUIRE(valid_image(W, H), "preprocess_forward: bad image size %dx%d", W, H);
+  REQUIRE(!depth_col || ld_col >= 1, "preprocess_forward: depth column stride %d", ld_col);
   if (n == 0) return MSPLAT_OK;
   REQUIRE(xyz && scale && rotate && intr && extr && uv && depth && conic && radius && tiles,
           "preprocess_forward: null pointer");
   Timed timed_("preprocess_fwd", as_stream(stream));
   preprocess_fwd_kernel<<<grid_1d(n, kGeomThreads), kGeomThreads, 0, as_stream(stream)>>>(
       xyz, scale, rotate, n, intr, extr, W, H, tiles_x(W), tiles_y(H), nearest, extent, uv, depth, conic, radius,
-      tiles);
+      tiles, depth_col, ld_col);
   return launched("preprocess_forward");
 }
 
-int msplat_preprocess_backward(const float *xyz, const float *scale, const float *rotate, int64_t n,
-                               const float *intr, const float *extr, const float *depth, const int32_t *radius,
-                               const float *dL_duv, const float *dL_ddepth, const float *dL_dconic, float *dL_dxyz,
-                               float *dL_dscale, float *dL_drotate, float *dL_dintr, float *dL_dextr,
-                               void *stream) {
+int msplat_preprocess_forward(const float *xyz, const float *scale, const float *rotate, int64_t n,
+                              const float *intr, const float *extr, int W, int H, float nearest, float extent,
+                              float *uv, float *depth, float *conic, int32_t *radius, int32_t *tiles,
+                              void *stream) {
+  return msplat_preprocess_forward_col(xyz, scale, rotate, n, intr, extr, W, H, nearest, extent, uv, depth, conic,
+                                       radius, tiles, nullptr, 1, stream);
+}
+
+int msplat_preprocess_backward_ld(const float *xyz, const float *scale, const float *rotate, int64_t n,
+                                  const float *intr, const float *extr, const float *depth, const int32_t *radius,
+                                  const float *dL_duv, const float *dL_ddepth, int ld_ddepth, const float *dL_dconic,
+                                  float *dL_dxyz, float *dL_dscale, float *dL_drotate, float *dL_dintr,
+                                  float *dL_dextr, void *stream) {
   REQUIRE(n >= 0, "preprocess_backward: n = %lld", (long long)n);
+  REQUIRE(!dL_ddepth || ld_ddepth >= 1, "preprocess_backward: dL_ddepth stride %d", ld_ddepth);
   hipStream_t s = as_stream(stream);
   const bool cam = dL_dintr || dL_dextr;
   launch_fill_zero_words({{dL_dintr, 4}, {dL_dextr, 12}}, s);
@@ -760,12 +770,21 @@ int msplat_preprocess_backward(const float *xyz, const float *scale, const float
   if (cam)
     preprocess_bwd_kernel<true><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
         xyz, scale, rotate, n, intr, extr, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
-        dL_drotate, dL_dintr, dL_dextr);
+        dL_drotate, dL_dintr, dL_dextr, ld_ddepth);
   else
     preprocess_bwd_kernel<false><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
         xyz, scale, rotate, n, intr, extr, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
-        dL_drotate, nullptr, nullptr);
+        dL_drotate, nullptr, nullptr, ld_ddepth);
   return launched("preprocess_backward");
+}
+
+int msplat_preprocess_backward(const float *xyz, const float *scale, const float *rotate, int64_t n,
+                               const float *intr, const float *extr, const float *depth, const int32_t *radius,
+                               const float *dL_duv, const float *dL_ddepth, const float *dL_dconic, float *dL_dxyz,
+                               float *dL_dscale, float *dL_drotate, float *dL_dintr, float *dL_dextr,
+                               void *stream) {
+  return msplat_preprocess_backward_ld(xyz, scale, rotate, n, intr, extr, depth, radius, dL_duv, dL_ddepth, 1,
+                                       dL_dconic, dL_dxyz, dL_dscale, dL_drotate, dL_dintr, dL_dextr, stream);
 }
 
 // ---------------------------------------------------------------- SH

@@ -480,7 +480,7 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_fwd_kernel(
     const float *__restrict__ xyz, const float *__restrict__ scale, const float *__restrict__ rot, int64_t n,
     const float *__restrict__ intr, const float *__restrict__ extr, int W, int H, int gx, int gy,
     float nearest, float extent, float *__restrict__ uv, float *__restrict__ depth, float *__restrict__ conic,
-    int32_t *__restrict__ radius, int32_t *__restrict__ tiles) {
+    int32_t *__restrict__ radius, int32_t *__restrict__ tiles, float *__restrict__ depth_col, int ld_col) {
   const int64_t i = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
   if (i >= n) return;
   const Camera c = load_camera(intr, extr);
@@ -494,6 +494,7 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_fwd_kernel(
   uv[2 * i] = u;
   uv[2 * i + 1] = v;
   depth[i] = d;
+  if (depth_col) depth_col[i * ld_col] = d;  // a feature column (the depth channel)
   float3 co = make_float3(0.f, 0.f, 0.f);
   int r = 0, nt = 0;
   if (d != 0.f) {
@@ -519,7 +520,7 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_bwd_kernel(
     const float *__restrict__ intr, const float *__restrict__ extr, const float *__restrict__ depth,
     const int32_t *__restrict__ radius, const float *__restrict__ duv, const float *__restrict__ ddepth,
     const float *__restrict__ dconic, float *__restrict__ dxyz, float *__restrict__ dscale,
-    float *__restrict__ drot, float *__restrict__ dintr, float *__restrict__ dextr) {
+    float *__restrict__ drot, float *__restrict__ dintr, float *__restrict__ dextr, int ld_ddepth) {
   const int64_t i = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
   const Camera c = load_camera(intr, extr);
   float cam[16] = {0.f};
@@ -530,7 +531,8 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_bwd_kernel(
     float4 dq = make_float4(0.f, 0.f, 0.f, 0.f);
     if (depth[i] != 0.f) {
       const float3 p = ld3(xyz, i);
-      project_adjoint<kCam>(c, p.x, p.y, p.z, duv[2 * i], duv[2 * i + 1], ddepth ? ddepth[i] : 0.f, dp, cam);
+      const float dd = ddepth ? ddepth[i * ld_ddepth] : 0.f;
+      project_adjoint<kCam>(c, p.x, p.y, p.z, duv[2 * i], duv[2 * i + 1], dd, dp, cam);
       const float3 s = ld3(scale, i);
       const float4 q = ld4(rot, i);
       float dc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};

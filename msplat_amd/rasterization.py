@@ -20,9 +20,20 @@ from .binning import CellLists, bin_gaussians, rasterize_backward, rasterize_for
 from .project_point import camera_grad_shape
 
 
+def _column(t, n: int):
+    """(pointer, stride) of a float32 device vector of n elements, or None when
+    it is not one (a copy is then needed)."""
+    if (t.dtype == torch.float32 and t.is_cuda and t.dim() in (1, 2) and t.shape[0] == n
+            and (t.dim() == 1 or t.shape[1] == 1) and (n <= 1 or t.stride(0) >= 1)):
+        return t.data_ptr(), max(int(t.stride(0)), 1)
+    return None
+
+
 def preprocess_forward(xyz, scale, rotate, intr, extr, W: int, H: int, nearest: float = 0.0,
-                       extent: float = 1.3):
-    """Fused projection + covariance + EWA: ``(uv, depth, conic, radius, tiles)``."""
+                       extent: float = 1.3, depth_col=None):
+    """Fused projection + covariance + EWA: ``(uv, depth, conic, radius, tiles)``.
+    ``depth_col`` (a float32 device column, e.g. a feature channel) also
+    receives the depths."""
     x = _lib.f32("xyz", xyz)
     n = x.shape[0]
     dev = x.device
@@ -31,11 +42,17 @@ def preprocess_forward(xyz, scale, rotate, intr, extr, W: int, H: int, nearest: 
     conic = torch.empty((n, 3), dtype=torch.float32, device=dev)
     radius = torch.empty(n, dtype=torch.int32, device=dev)
     tiles = torch.empty(n, dtype=torch.int32, device=dev)
+    col, ld = None, 1
+    if depth_col is not None:
+        c = _column(depth_col, n)
+        if c is None:
+            raise RuntimeError("preprocess_forward: depth_col must be a float32 device column of n elements")
+        col, ld = c
     with torch.cuda.device(dev):
-        _lib.call("msplat_preprocess_forward", _lib.ptr(x), _lib.ptr(_lib.f32("scale", scale)),
+        _lib.call("msplat_preprocess_forward_col", _lib.ptr(x), _lib.ptr(_lib.f32("scale", scale)),
                   _lib.ptr(_lib.f32("rotate", rotate)), n, _lib.ptr(_lib.f32("intr", intr)),
                   _lib.ptr(_lib.f32("extr", extr)), int(W), int(H), float(nearest), float(extent), _lib.ptr(uv),
-                  _lib.ptr(depth), _lib.ptr(conic), _lib.ptr(radius), _lib.ptr(tiles), _lib.stream_ptr())
+                  _lib.ptr(depth), _lib.ptr(conic), _lib.ptr(radius), _lib.ptr(tiles), col, ld, _lib.stream_ptr())
     return uv, depth, conic, radius, tiles
 
 
@@ -50,12 +67,19 @@ def preprocess_backward(xyz, scale, rotate, intr, extr, depth, radius, dL_duv, d
     drot = torch.empty((n, 4), dtype=torch.float32, device=dev)
     dintr = torch.empty(4, dtype=torch.float32, device=dev) if want_intr else None
     dextr = torch.empty(12, dtype=torch.float32, device=dev) if want_extr else None
-    g_d = _lib.f32("dL_ddepth", dL_ddepth) if dL_ddepth is not None else None
+    g_ptr, g_ld = None, 1
+    if dL_ddepth is not None:  # a strided column (a feature channel's gradient) is read in place
+        c = _column(dL_ddepth.detach(), n)
+        g_d = None
+        if c is None:
+            g_d = _lib.f32("dL_ddepth", dL_ddepth)
+            c = (g_d.data_ptr(), 1)
+        g_ptr, g_ld = c
     with torch.cuda.device(dev):
-        _lib.call("msplat_preprocess_backward", _lib.ptr(x), _lib.ptr(_lib.f32("scale", scale)),
+        _lib.call("msplat_preprocess_backward_ld", _lib.ptr(x), _lib.ptr(_lib.f32("scale", scale)),
                   _lib.ptr(_lib.f32("rotate", rotate)), n, _lib.ptr(_lib.f32("intr", intr)),
                   _lib.ptr(_lib.f32("extr", extr)), _lib.ptr(_lib.f32("depth", depth)),
-                  _lib.ptr(_lib.i32("radius", radius)), _lib.ptr(_lib.f32("dL_duv", dL_duv)), _lib.ptr(g_d),
+                  _lib.ptr(_lib.i32("radius", radius)), _lib.ptr(_lib.f32("dL_duv", dL_duv)), g_ptr, g_ld,
                   _lib.ptr(_lib.f32("dL_dconic", dL_dconic)), _lib.ptr(dxyz), _lib.ptr(dscale), _lib.ptr(drot),
                   _lib.ptr(dintr), _lib.ptr(dextr), _lib.stream_ptr())
     return dxyz, dscale, drot, dintr, dextr

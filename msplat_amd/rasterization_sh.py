@@ -77,11 +77,10 @@ class _RasterizeSH(torch.autograd.Function):
             for t in (s, x, cp, feat):
                 t.record_stream(side)
             # meanwhile: geometry, the other columns, binning
-            uv, depth, conic, radius, _ = preprocess_forward(xyz, scale, rotate, intr, extr, W, H)
+            uv, depth, conic, radius, _ = preprocess_forward(xyz, scale, rotate, intr, extr, W, H,
+                                                             depth_col=feat[:, C - 1] if depth_channel else None)
             if E:
                 feat[:, Csh:Csh + E].copy_(extra)
-            if depth_channel:
-                feat[:, C - 1].copy_(depth[:, 0])
             bins = bin_gaussians(uv, depth, W, H, radius, cfg.cell)
             main.wait_event(sh_done)
             out, final_T, ncontrib, walk = rasterize_forward(uv, conic, opacity, feat, bins, bg, W, H, cfg.ppl_fwd,
@@ -126,7 +125,7 @@ class _RasterizeSH(torch.autograd.Function):
                 dL_ddepth=dfeat[:, C - 1] if depth_channel else None,
                 want_intr=intr.requires_grad, want_extr=extr.requires_grad)
             main.wait_event(sh_done)
-            dxyz = dxyz + dxyz_sh
+            dxyz.add_(dxyz_sh)
             dcampos = (-dxyz_sh.sum(0)).view(ctx.campos_shape) if ctx.needs_input_grad[7] else None
         dextra = dfeat[:, Csh:Csh + E] if (E and ctx.needs_input_grad[11]) else None
         dndc = None
