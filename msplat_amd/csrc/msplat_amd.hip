@@ -159,7 +159,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
   L.nlisted = c.take(8);
   L.area_s = c.take(un * 4);  // cell counts and tile rectangles in depth order (cell binning)
   L.rect_s = c.take(un * 8);
-  L.bstatus = c.take(size_t(bin_blocks(n)) * 8);  // look-back words of the counting pass
+  L.bstatus = c.take(size_t(sortk::prep_blocks(n)) * 8);  // look-back words of the counting pass
   (void)W;
   (void)H;
   L.total = c.off;
@@ -984,9 +984,9 @@ int bin_views_prepare_stages(int stages, const float *uv, const float *depth, co
     // word, the look-back words, and the counts (which stay 0 when m = 0)
     uint64_t *bstatus = at<uint64_t>(workspace, L.bstatus);
     launch_fill_zero_words(
-        {{rb.ghist, sortk::ghist_words()}, {bstatus, 2 * bin_blocks(m)}, {nlisted, 2}, {d_count, 2}}, s);
+        {{rb.ghist, sortk::ghist_words()}, {bstatus, 2 * sortk::prep_blocks(m)}, {nlisted, 2}, {d_count, 2}}, s);
     if (m > 0)
-      sortk::bin_prepare_kernel<<<grid_1d(m, sortk::kThreads), sortk::kThreads, 0, s>>>(
+      sortk::bin_prepare_kernel<<<unsigned(sortk::prep_blocks(m)), sortk::kThreads, 0, s>>>(
           uv, depth, radius, m, vw, cell, reinterpret_cast<uint2 *>(rect), ka, va, bstatus, nlisted, d_count,
           sortk::radix_error_word(rb));
   }

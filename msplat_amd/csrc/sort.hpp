@@ -534,6 +534,11 @@ MSPLAT_DEV uint64_t lb_word(uint64_t flag, int64_t hi, int64_t lo) {
   return flag | (h << 31) | l;
 }
 
+// Lanes below this one in a ballot mask.
+MSPLAT_DEV uint32_t mask_rank_lo(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
 MSPLAT_DEV int64_t wave_sum_i64(int64_t v) {
 #pragma unroll
   for (int off = kWave / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -589,29 +594,68 @@ MSPLAT_DEV bool lookback_publish(uint64_t *status, int64_t b, int64_t nh, int64_
 // workgroup stores the number of listed rows and of cell entries (-1 when a
 // wait expired; 2^31 when the entries reach 2^31 - 1, which no int32 list
 // holds).
+constexpr int kPrepRows = 8;  // rows per thread of the counting pass (2048 per workgroup)
+
+MSPLAT_HD int64_t prep_blocks(int64_t n) {
+  const int64_t per = int64_t(kThreads) * kPrepRows;
+  return n > 0 ? (n + per - 1) / per : 1;
+}
+
+// Workgroup w covers rows w * 2048 + it * 256 + tid (it < kPrepRows): the
+// loads of all its rounds are in flight together, and a few hundred
+// workgroups (all resident at once) keep the look-back short.  Ranks follow
+// row order: round, then wave, then lane.
 __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
     const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius, int64_t n,
     Views v, int S, uint2 *__restrict__ rect, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
     uint64_t *__restrict__ status, int64_t *__restrict__ nlisted, int64_t *__restrict__ count,
     uint32_t *__restrict__ error) {
+  constexpr int kWaves = kThreads / kWave;
   __shared__ int64_t s_base;
-  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  int a = 0;
-  uint32_t db = 0u;
-  if (i < n) {
-    const int b = view_of_row(v, i);
-    const int r = radius[i];
-    db = fbits(depth[i]);
-    TileRect t{0, 0, 0, 0};
-    if (r > 0 && !(db >> 31)) {
-      t = tile_rect(uv[2 * i], uv[2 * i + 1], r, v.gx[b], v.gy[b]);
-      if (t.area() > 0) a = rect_cells(pack_rect(t), S);
+  __shared__ uint32_t s_cnt[kPrepRows][kWaves];
+  __shared__ int64_t s_area[kWaves];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int64_t base = int64_t(blockIdx.x) * kThreads * kPrepRows + threadIdx.x;
+  int a[kPrepRows];
+  uint32_t db[kPrepRows];
+  uint64_t m[kPrepRows];
+  int64_t my_area = 0;
+#pragma unroll
+  for (int it = 0; it < kPrepRows; ++it) {
+    const int64_t i = base + int64_t(it) * kThreads;
+    a[it] = 0;
+    db[it] = 0u;
+    if (i < n) {
+      const int b = view_of_row(v, i);
+      const int r = radius[i];
+      db[it] = fbits(depth[i]);
+      TileRect t{0, 0, 0, 0};
+      if (r > 0 && !(db[it] >> 31)) {
+        t = tile_rect(uv[2 * i], uv[2 * i + 1], r, v.gx[b], v.gy[b]);
+        if (t.area() > 0) a[it] = rect_cells(pack_rect(t), S);
+      }
+      rect[i] = pack_rect(t);
     }
-    rect[i] = pack_rect(t);
+    m[it] = __ballot(a[it] > 0);
+    my_area += a[it];
+    if (lane == 0) s_cnt[it][wave] = uint32_t(__popcll(m[it]));
   }
-  int64_t nl, ne;
-  const int64_t rank = block_excl_scan(a > 0 ? 1 : 0, &nl);
-  block_excl_scan(a, &ne);
+  my_area = wave_sum_i64(my_area);
+  if (lane == 0) s_area[wave] = my_area;
+  __syncthreads();
+  int64_t nl = 0, ne = 0;
+  uint32_t pre[kPrepRows];  // listed rows of this workgroup before (round it, this wave)
+#pragma unroll
+  for (int it = 0; it < kPrepRows; ++it) {
+    pre[it] = uint32_t(nl);
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      pre[it] += w < wave ? s_cnt[it][w] : 0u;
+      nl += s_cnt[it][w];
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) ne += s_area[w];
   if (threadIdx.x < kWave) {
     int64_t pl, pe;
     const bool ok = lookback_publish(status, blockIdx.x, nl, ne, pl, pe);
@@ -626,9 +670,13 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
     }
   }
   __syncthreads();
-  if (a > 0) {
-    keys[s_base + rank] = db;
-    vals[s_base + rank] = uint32_t(i);
+#pragma unroll
+  for (int it = 0; it < kPrepRows; ++it) {
+    if (a[it] > 0) {
+      const int64_t j = s_base + pre[it] + mask_rank_lo(m[it]);
+      keys[j] = db[it];
+      vals[j] = uint32_t(base + int64_t(it) * kThreads);
+    }
   }
 }
 
