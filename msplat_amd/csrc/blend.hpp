@@ -693,11 +693,15 @@ __global__ __launch_bounds__(kBlendLanes * TG) void blend_bwd_kernel(
       // sum w dy^2.  Accumulators start from an opaque zero: with a literal 0
       // the compiler rematerialises the constant on every skip path.
       const float dx = g0.x - fx[0];
-      float sw = opaque_zero(), swy = opaque_zero(), swyy = opaque_zero(), vo = opaque_zero();
+      // Per lane: the moments of t = G dL/dalpha (= dL/dG / opacity) over its
+      // pixels, sum t, sum t dy and sum t dy^2, and the colour sums.  The
+      // opacity factor of the conic and mean gradients is applied once, to
+      // the wave totals.
+      float m0 = opaque_zero(), m1 = opaque_zero(), m2 = opaque_zero();
       float vf[CMAX];
 #pragma unroll
       for (int ch = 0; ch < CMAX; ++ch) vf[ch] = opaque_zero();
-      bool any = false;
+      bool any = false;  // the lane took a contribution
       const uint32_t strips = __builtin_amdgcn_readfirstlane(s.strips[j]);
 #pragma unroll
       for (int i = 0; i < PPL; ++i) {
@@ -728,43 +732,40 @@ __global__ __launch_bounds__(kBlendLanes * TG) void blend_bwd_kernel(
         A[i] += alpha * d;
         float dL_dalpha = d * T[i];
         dL_dalpha += tb[i] * inv;
-        const float w = G * (op * dL_dalpha);  // G dL/dG
-        const float wy = w * dy;
-        sw += w;
-        swy += wy;
-        swyy += wy * dy;
-        vo += G * dL_dalpha;
+        const float t = G * dL_dalpha;
+        const float ty = t * dy;
+        m0 += t;
+        m1 += ty;
+        m2 += ty * dy;
       }
-      // dL/duv = -(sum w (a dx + b dy), sum w (b dx + c dy)),
-      // dL/dconic = -(sum w dx^2 / 2, sum w dx dy, sum w dy^2 / 2)
-      const float vuv0 = -(a * dx * sw + b * swy);
-      const float vuv1 = -(b * dx * sw + c * swy);
-      const float vc0 = -0.5f * dx * dx * sw;
-      const float vc1 = -dx * swy;
-      const float vc2 = -0.5f * swyy;
       if (__ballot(any) == 0ull) continue;
-      // Sum the 6 + cc per-lane contributions over the wave in one transposed
-      // reduction, then each lane adds its value to its target.
+      // With w = o t (= G dL/dG): dL/duv = -(sum w (a dx + b dy), sum w (b dx
+      // + c dy)), dL/dconic = -(sum w dx^2 / 2, sum w dx dy, sum w dy^2 / 2),
+      // dL/do = sum t.  The 6 + CMAX per-lane values are summed over the wave
+      // in one transposed reduction, then each lane scales its total (-o for
+      // the first five) and adds it to its target.
       const int gidx = s.id[j];
-      float vals[NV];
-      vals[0] = vuv0;
-      vals[1] = vuv1;
-      vals[2] = vc0;
-      vals[3] = vc1;
-      vals[4] = vc2;
-      vals[5] = vo;
+      constexpr int NR = 6 + CMAX;
+      float vals[NR];
+      vals[0] = a * dx * m0 + b * m1;
+      vals[1] = b * dx * m0 + c * m1;
+      vals[2] = 0.5f * dx * dx * m0;
+      vals[3] = dx * m1;
+      vals[4] = 0.5f * m2;
+      vals[5] = m0;
 #pragma unroll
-      for (int ch = 0; ch < NV - 6; ++ch) vals[6 + ch] = ch < CMAX ? vf[ch < CMAX ? ch : 0] : 0.f;
-      float tot[NV / 4];
-      wave_sum_transposed<NV>(vals, tot);
+      for (int ch = 0; ch < CMAX; ++ch) vals[6 + ch] = vf[ch];
+      float tot[(NR + 3) / 4];
+      wave_sum_transposed<NR>(vals, tot);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int vec = (lane & 15) + 16 * q;
         float mine = 0.f;
 #pragma unroll
-        for (int jj = 0; jj < NV / 4; ++jj)
+        for (int jj = 0; jj < (NR + 3) / 4; ++jj)
           if (jj == vec) mine = tot[jj];
         if (!tvalid[q]) continue;
+        mine = tval[q] < 5 ? -op * mine : mine;
         if constexpr (TG > 1) {
           const int key = s.pos[j] - lo_bound;  // position in the group's cell list
           if (key < kAcc) {

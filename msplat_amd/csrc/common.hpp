@@ -157,29 +157,37 @@ constexpr int kRowValue[4] = {0, 2, 1, 3};
 MSPLAT_HD int row_value(int r) { return ((r & 1) << 1) | (r >> 1); }
 
 MSPLAT_DEV float dpp_row_ror_add(float v, int ctrl) {
-  // ctrl: 0x121..0x12F = row_ror:1..15 (rotate within each row of 16 lanes)
+  // ctrl: 0x121..0x12F = row_ror:1..15 (rotate within each row of 16 lanes;
+  // every lane has a source, so bound_ctrl changes nothing but lets the move
+  // fold into the add)
   int r;
   switch (ctrl) {
-    case 8: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, false); break;
-    case 4: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, false); break;
-    case 2: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x122, 0xf, 0xf, false); break;
-    default: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xf, 0xf, false); break;
+    case 8: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xf, 0xf, true); break;
+    case 4: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xf, 0xf, true); break;
+    case 2: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x122, 0xf, 0xf, true); break;
+    default: r = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x121, 0xf, 0xf, true); break;
   }
   return v + __int_as_float(r);
 }
 
+// NV need not be a multiple of 4: values past NV count as zero, and pairs
+// made only of them cost nothing.
 template <int NV>
-MSPLAT_DEV void wave_sum_transposed(const float (&in)[NV], float (&out)[NV / 4]) {
-  static_assert(NV % 4 == 0, "pad to a multiple of 4");
-  float a[NV / 2];
+MSPLAT_DEV void wave_sum_transposed(const float (&in)[NV], float (&out)[(NV + 3) / 4]) {
+  constexpr int NP = (NV + 3) / 4 * 4;
+  float a[NP / 2];
 #pragma unroll
-  for (int i = 0; i < NV / 2; ++i) {
-    const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(in[2 * i]), __float_as_uint(in[2 * i + 1]),
-                                                    false, false);
+  for (int i = 0; i < NP / 2; ++i) {
+    if (2 * i >= NV) {
+      a[i] = 0.f;
+      continue;
+    }
+    const float hi = 2 * i + 1 < NV ? in[2 * i + 1 < NV ? 2 * i + 1 : 0] : 0.f;
+    const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(in[2 * i]), __float_as_uint(hi), false, false);
     a[i] = __uint_as_float(s[0]) + __uint_as_float(s[1]);
   }
 #pragma unroll
-  for (int j = 0; j < NV / 4; ++j) {
+  for (int j = 0; j < NP / 4; ++j) {
     const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(a[2 * j]), __float_as_uint(a[2 * j + 1]),
                                                     false, false);
     float b = __uint_as_float(s[0]) + __uint_as_float(s[1]);
