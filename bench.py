@@ -3,10 +3,12 @@
 Workload (the headline configuration): N = 1,000,000 Gaussians in
 [-3,3]^3, one 1920x1080 view (fx = fy = 1.171875 W, camera at distance 4),
 features = SH degree 3 ([N,3,16] ~ N(0,0.3)) of the view directions plus
-the camera-space depth (C = 4), rendered by rasterization_sh (one op; the
-composition of separate ops is --features ops / torch), L2 loss against a U(0,1) target (l2_loss: one fused launch
-each way; --torch-loss for the torch expression), backward to every
-Gaussian parameter.  A step is one forward + backward of that view; inputs
+the camera-space depth (C = 4), rendered with the L2 loss against a U(0,1)
+target by rasterization_sh_l2 (one op: the loss is summed in the blending
+forward and its gradient formed in the blending adjoint; --separate-loss
+renders with rasterization_sh and applies l2_loss, --torch-loss the torch
+expression, --features ops / torch the composition of separate ops),
+backward to every Gaussian parameter.  A step is one forward + backward of that view; inputs
 are resident in HBM before the timed region.  The blending launch parameters
 are chosen by msplat_amd.autotune before the warm-up (--no-autotune keeps
 the defaults).
@@ -50,7 +52,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_step(cfg, dev, features: str = "fused", fused_loss: bool = True):
+def build_step(cfg, dev, features: str = "fused", fused_loss: bool = True, loss_in_raster: bool = True):
     import torch
 
     import msplat_amd as ms
@@ -64,6 +66,14 @@ def build_step(cfg, dev, features: str = "fused", fused_loss: bool = True):
 
     def step():
         xyz = P["xyz"]
+        if features == "fused" and fused_loss and loss_in_raster:
+            # SH colours + depth rendered by one op with the L2 loss inside the blending kernels
+            out, loss = ms.rasterization_sh_l2(xyz, P["scale"], P["rotate"], P["opacity"], P["shs"], cam.intr,
+                                               cam.extr, cam.eye, W, H, 0.0, target, depth_channel=True)
+            loss.backward()
+            for p in P.values():
+                p.grad = None
+            return out
         if features == "fused":  # SH colours + depth rendered by one op (SH on a side stream)
             out = ms.rasterization_sh(xyz, P["scale"], P["rotate"], P["opacity"], P["shs"], cam.intr, cam.extr,
                                       cam.eye, W, H, 0.0, depth_channel=True)
@@ -359,6 +369,8 @@ def main():
     ap.add_argument("--no-multiview", action="store_true")
     ap.add_argument("--torch-loss", action="store_true",
                     help="the L2 loss as torch ops instead of the fused l2_loss")
+    ap.add_argument("--separate-loss", action="store_true",
+                    help="l2_loss after rasterization_sh instead of the loss inside the blending kernels")
     ap.add_argument("--features", choices=("fused", "ops", "torch"), default="fused",
                     help="fused: rasterization_sh renders SH colours + depth in one op; ops: compute_sh_points + "
                          "project_point + cat + rasterization; torch: view directions in torch + compute_sh + ...")
@@ -379,7 +391,8 @@ def main():
 
     dev = torch.device("cuda:0")
     cfg = scenes.config_sh3_1080p(n=args.n)
-    step, P, cam = build_step(cfg, dev, features=args.features, fused_loss=not args.torch_loss)
+    step, P, cam = build_step(cfg, dev, features=args.features, fused_loss=not args.torch_loss,
+                              loss_in_raster=not args.separate_loss)
     W, H = cam.W, cam.H
     C = 4
     log(f"workload: N={args.n} {W}x{H} C={C} (SH3 RGB + depth); warmup {args.warmup}, steps {args.steps}")
@@ -495,7 +508,8 @@ def main():
                                       "torch": "torch view directions + compute_sh + project_point + cat + "
                                                "rasterization"}[args.features],
                             loss="((out - target) ** 2).sum() in torch" if args.torch_loss else
-                            "l2_loss (one fused launch each way)",
+                            ("l2_loss (one fused launch each way)" if args.separate_loss or args.features != "fused"
+                             else "sum (out - target)^2 inside the blending kernels (rasterization_sh_l2)"),
                             parallelism="single GPU"),
                 roofline=roof, roofline_hbm=roof_hbm, cpu_baseline=cpu, multiview=multiview, batched_1080p=batched,
                 serving=serving_stats,

@@ -306,6 +306,34 @@ int msplat_rasterize_backward(
     const int32_t *walk, const float *dL_drendered, float *dL_duv,
     float *dL_dconic, float *dL_dopacity, float *dL_dfeature, void *stream);
 
+/* The same pair with a squared-error loss against target [C, H, W] fused in
+ * (no reference counterpart; the training loss of the reference's tutorials,
+ * tutorials/gs_2d.py:73-90, is separate torch kernels).  The forward also
+ * writes *loss = sum (rendered - target)^2 (per-wave partial sums in the
+ * workspace, added in a fixed order in fp64); the adjoint takes dL/drendered
+ * = 2 (rendered - target) dL_dloss[0], plus dL_drendered when that is not
+ * NULL (the image's gradient from other terms), without the separate
+ * gradient image of msplat_sq_diff_grad. */
+size_t msplat_rasterize_l2_workspace_bytes(int W, int H, int C);
+int msplat_rasterize_forward_l2(const float *uv, const float *conic,
+                                const float *opacity, const float *feature,
+                                int64_t n, int C, const uint32_t *rect,
+                                const int32_t *cell_ids,
+                                const int32_t *cell_range, int cell, float bg,
+                                int W, int H, int ppl, int cull,
+                                const float *target, void *workspace,
+                                size_t workspace_bytes, float *rendered,
+                                float *final_T, int32_t *ncontrib,
+                                int32_t *walk, float *loss, void *stream);
+int msplat_rasterize_backward_l2(
+    const float *uv, const float *conic, const float *opacity,
+    const float *feature, int64_t n, int C, const uint32_t *rect,
+    const int32_t *cell_ids, const int32_t *cell_range, int cell, float bg,
+    int W, int H, int ppl, int cull, const float *final_T, const int32_t *ncontrib,
+    const int32_t *walk, const float *dL_drendered, const float *rendered,
+    const float *target, const float *dL_dloss, float *dL_duv,
+    float *dL_dconic, float *dL_dopacity, float *dL_dfeature, void *stream);
+
 /* ---- many views in one launch (no reference counterpart: the reference
  * renders one camera per rasterization call, msplat/__init__.py:22-93).
  *

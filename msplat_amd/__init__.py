@@ -14,7 +14,7 @@ from .ewa_project import ewa_project
 from .loss import l2_loss
 from .project_point import project_point
 from .rasterization import rasterization
-from .rasterization_sh import rasterization_sh
+from .rasterization_sh import rasterization_sh, rasterization_sh_l2
 from .sort_gaussian import sort_gaussian
 
 __all__ = [
@@ -30,6 +30,7 @@ __all__ = [
     "compute_sh_points",
     "l2_loss",
     "rasterization_sh",
+    "rasterization_sh_l2",
 ]
 
 __version__ = "0.1.0"

@@ -76,6 +76,11 @@ SIGNATURES = {
     "msplat_rasterize_forward": (I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P]),
     "msplat_rasterize_backward": (
         I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P, P, P, P, P]),
+    "msplat_rasterize_l2_workspace_bytes": (SZ, [I32, I32, I32]),
+    "msplat_rasterize_forward_l2": (
+        I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, SZ, P, P, P, P, P, P]),
+    "msplat_rasterize_backward_l2": (
+        I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P, P, P, P, P, P, P, P]),
 }
 
 _lib = None

@@ -132,8 +132,10 @@ def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8, capacity: in
 
 
 def rasterize_forward(uv, conic, opacity, feature, bins: CellLists, bg: float, W: int, H: int, ppl: int = 0,
-                      cull: bool = True):
-    """``(rendered [C,H,W], final_T [H,W], ncontrib [H,W] int32, walk [T,2] int32)``.
+                      cull: bool = True, target=None):
+    """``(rendered [C,H,W], final_T [H,W], ncontrib [H,W] int32, walk [T,2] int32)``,
+    and with ``target`` [C,H,W] also the fused loss ``sum((rendered - target)^2)``
+    (a 0-d tensor) as a fifth element.
 
     ``cull`` skips, per tile, Gaussians that give no pixel of it alpha >= 1/255
     (rendered and final_T are unchanged; ncontrib counts culled-list
@@ -142,23 +144,38 @@ def rasterize_forward(uv, conic, opacity, feature, bins: CellLists, bg: float, W
     f = _lib.f32("feature", feature)
     n, C = f.shape
     dev = u.device
-    gx, gy = tile_grid(int(W), int(H))
+    W, H = int(W), int(H)
+    gx, gy = tile_grid(W, H)
     out = torch.empty((C, H, W), dtype=torch.float32, device=dev)
     final_T = torch.empty((H, W), dtype=torch.float32, device=dev)
     ncontrib = torch.empty((H, W), dtype=torch.int32, device=dev)
     walk = torch.empty((gx * gy, 2), dtype=torch.int32, device=dev)
+    common = (_lib.ptr(u), _lib.ptr(_lib.f32("conic", conic)), _lib.ptr(_lib.f32("opacity", opacity)), _lib.ptr(f),
+              n, C, _lib.ptr(bins.rect), _lib.ptr(bins.cell_ids), _lib.ptr(bins.cell_range), bins.cell, float(bg),
+              W, H, int(ppl), int(bool(cull)))
     with torch.cuda.device(dev):
-        _lib.call("msplat_rasterize_forward", _lib.ptr(u), _lib.ptr(_lib.f32("conic", conic)),
-                  _lib.ptr(_lib.f32("opacity", opacity)), _lib.ptr(f), n, C, _lib.ptr(bins.rect),
-                  _lib.ptr(bins.cell_ids), _lib.ptr(bins.cell_range), bins.cell, float(bg), int(W), int(H),
-                  int(ppl), int(bool(cull)), _lib.ptr(out), _lib.ptr(final_T), _lib.ptr(ncontrib), _lib.ptr(walk),
-                  _lib.stream_ptr())
-    return out, final_T, ncontrib, walk
+        if target is None:
+            _lib.call("msplat_rasterize_forward", *common, _lib.ptr(out), _lib.ptr(final_T), _lib.ptr(ncontrib),
+                      _lib.ptr(walk), _lib.stream_ptr())
+            return out, final_T, ncontrib, walk
+        t = _lib.f32("target", target)
+        if tuple(t.shape) != (C, H, W):
+            raise RuntimeError(f"rasterize_forward: target shape {tuple(t.shape)} != {(C, H, W)}")
+        nb = int(_lib.query("msplat_rasterize_l2_workspace_bytes", W, H, C))
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
+        loss = torch.empty((), dtype=torch.float32, device=dev)
+        _lib.call("msplat_rasterize_forward_l2", *common, _lib.ptr(t), _lib.ptr(ws), nb, _lib.ptr(out),
+                  _lib.ptr(final_T), _lib.ptr(ncontrib), _lib.ptr(walk), _lib.ptr(loss), _lib.stream_ptr())
+    return out, final_T, ncontrib, walk, loss
 
 
 def rasterize_backward(uv, conic, opacity, feature, bins: CellLists, bg: float, W: int, H: int, final_T, ncontrib,
-                       walk, dL_drendered, ppl: int = 0, cull: bool = True):
-    """``(dL_duv [P,2], dL_dconic [P,3], dL_dopacity [P,1], dL_dfeature [P,C])``."""
+                       walk, dL_drendered, ppl: int = 0, cull: bool = True, l2=None):
+    """``(dL_duv [P,2], dL_dconic [P,3], dL_dopacity [P,1], dL_dfeature [P,C])``.
+
+    ``l2 = (rendered, target, dL_dloss)``: the fused loss's adjoint, where
+    dL/drendered = 2 (rendered - target) dL_dloss, plus ``dL_drendered`` when
+    that is not None."""
     u = _lib.f32("uv", uv)
     f = _lib.f32("feature", feature)
     n, C = f.shape
@@ -167,11 +184,17 @@ def rasterize_backward(uv, conic, opacity, feature, bins: CellLists, bg: float, 
     dconic = torch.empty((n, 3), dtype=torch.float32, device=dev)
     dopac = torch.empty((n, 1), dtype=torch.float32, device=dev)
     dfeat = torch.empty((n, C), dtype=torch.float32, device=dev)
+    common = (_lib.ptr(u), _lib.ptr(_lib.f32("conic", conic)), _lib.ptr(_lib.f32("opacity", opacity)), _lib.ptr(f),
+              n, C, _lib.ptr(bins.rect), _lib.ptr(bins.cell_ids), _lib.ptr(bins.cell_range), bins.cell, float(bg),
+              int(W), int(H), int(ppl), int(bool(cull)), _lib.ptr(final_T), _lib.ptr(ncontrib), _lib.ptr(walk))
+    g = None if dL_drendered is None else _lib.f32("dL_drendered", dL_drendered)
     with torch.cuda.device(dev):
-        _lib.call("msplat_rasterize_backward", _lib.ptr(u), _lib.ptr(_lib.f32("conic", conic)),
-                  _lib.ptr(_lib.f32("opacity", opacity)), _lib.ptr(f), n, C, _lib.ptr(bins.rect),
-                  _lib.ptr(bins.cell_ids), _lib.ptr(bins.cell_range), bins.cell, float(bg), int(W), int(H),
-                  int(ppl), int(bool(cull)), _lib.ptr(final_T), _lib.ptr(ncontrib), _lib.ptr(walk),
-                  _lib.ptr(_lib.f32("dL_drendered", dL_drendered)), _lib.ptr(duv), _lib.ptr(dconic),
-                  _lib.ptr(dopac), _lib.ptr(dfeat), _lib.stream_ptr())
+        if l2 is None:
+            _lib.call("msplat_rasterize_backward", *common, _lib.ptr(g), _lib.ptr(duv), _lib.ptr(dconic),
+                      _lib.ptr(dopac), _lib.ptr(dfeat), _lib.stream_ptr())
+        else:
+            rendered, target, dL_dloss = l2
+            _lib.call("msplat_rasterize_backward_l2", *common, _lib.ptr(g), _lib.ptr(_lib.f32("rendered", rendered)),
+                      _lib.ptr(_lib.f32("target", target)), _lib.ptr(_lib.f32("dL_dloss", dL_dloss.reshape(1))),
+                      _lib.ptr(duv), _lib.ptr(dconic), _lib.ptr(dopac), _lib.ptr(dfeat), _lib.stream_ptr())
     return duv, dconic, dopac, dfeat

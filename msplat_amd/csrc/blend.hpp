@@ -53,6 +53,14 @@ struct BlendIn {
   int ntiles, per_xcd;     // global tiles of all views; workgroups per XCD band
   int ngroups;             // global 2 x 2 tile groups (grouped adjoint)
   Views views;
+  // Fused squared-error loss against a target image of the same layout
+  // (optional).  Forward: each wave stores sum (rendered - target)^2 over
+  // its pixels and the pass's channels in l2_part[pass * l2_slots + wave].
+  // Adjoint: dL/drendered = 2 (rendered - target) dL[0], plus dout when given.
+  const float *l2_target;
+  float *l2_part;
+  int64_t l2_slots;
+  const float *l2_rendered, *l2_dL;
 };
 
 template <int PPL>
@@ -466,6 +474,8 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
   }
   const int64_t hw = int64_t(pb.H) * pb.W;
   float *vout = out + int64_t(in.C) * pb.pix0;  // this view's [C, H, W]
+  const float *vtgt = in.l2_target ? in.l2_target + int64_t(in.C) * pb.pix0 : nullptr;
+  float lsum = 0.f;
 #pragma unroll
   for (int i = 0; i < PPL; ++i) {
     if (!(pb.px[i] < pb.W && pb.py[i] < pb.H)) continue;
@@ -476,8 +486,20 @@ __global__ __launch_bounds__(kBlendLanes) void blend_fwd_kernel(BlendIn in, floa
       ncontrib[pb.pix0 + pix] = last[i];
     }
 #pragma unroll
-    for (int ch = 0; ch < CMAX; ++ch)
-      if (ch < cc) vout[(in.c0 + ch) * hw + pix] = F[i][ch] + Tf * in.bg;
+    for (int ch = 0; ch < CMAX; ++ch) {
+      if (ch >= cc) continue;
+      const int64_t off = (in.c0 + ch) * hw + pix;
+      const float val = F[i][ch] + Tf * in.bg;
+      vout[off] = val;
+      if (vtgt) {
+        const float d = val - vtgt[off];
+        lsum += d * d;
+      }
+    }
+  }
+  if (in.l2_part) {  // uniform
+    lsum = wave_sum(lsum);
+    if (lane == 0) in.l2_part[int64_t(in.c0 / 64) * in.l2_slots + (blockIdx.x & 7) * in.per_xcd + (blockIdx.x >> 3)] = lsum;
   }
   if constexpr (kWalk) {
     int wl = 0;
@@ -566,7 +588,10 @@ __global__ __launch_bounds__(kBlendLanes * TG) void blend_bwd_kernel(
     return;
   }
   const int64_t hw = int64_t(pb.H) * pb.W;
-  const float *vdout = dout + int64_t(in.C) * pb.pix0;  // this view's [C, H, W]
+  const float *vdout = dout ? dout + int64_t(in.C) * pb.pix0 : nullptr;  // this view's [C, H, W]
+  const float *vrend = in.l2_dL ? in.l2_rendered + int64_t(in.C) * pb.pix0 : nullptr;
+  const float *vtgt = in.l2_dL ? in.l2_target + int64_t(in.C) * pb.pix0 : nullptr;
+  const float l2s = in.l2_dL ? 2.f * in.l2_dL[0] : 0.f;
 
   // Per pixel: T (recovered back to front), tb = -T_final (bg . dL/dpixel)
   // for the background term, the channels' dL/dpixel g and
@@ -591,7 +616,17 @@ __global__ __launch_bounds__(kBlendLanes * TG) void blend_bwd_kernel(
     float bg_dot = 0.f;
 #pragma unroll
     for (int ch = 0; ch < CMAX; ++ch) {
-      g[i][ch] = (inside && ch < cc) ? vdout[(in.c0 + ch) * hw + pix] : 0.f;
+      float gv = 0.f;
+      if (inside && ch < cc) {
+        const int64_t off = (in.c0 + ch) * hw + pix;
+        if (vrend) {  // the fused loss's gradient (+ the image's own, as autograd would add them)
+          gv = (vrend[off] - vtgt[off]) * l2s;
+          if (vdout) gv += vdout[off];
+        } else {
+          gv = vdout[off];
+        }
+      }
+      g[i][ch] = gv;
       if (ch < cc) bg_dot += in.bg * g[i][ch];
     }
     A[i] = 0.f;

@@ -97,6 +97,27 @@ __global__ __launch_bounds__(kSumThreads) void sq_diff_sum_kernel(const float *_
   }
 }
 
+// out = the sum of n floats, in fp64 and a fixed order (one workgroup):
+// thread t adds p[t], p[t + 1024], ..., then a fixed shuffle tree and the
+// waves in order.  The fused rasterizer loss's per-wave partials.
+__global__ __launch_bounds__(kSumThreads) void sum_floats_kernel(const float *__restrict__ p, int64_t n,
+                                                                 float *__restrict__ out) {
+  __shared__ double wd[kSumThreads / kWave];
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kSumThreads) v += double(p[i]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  if (lane == 0) wd[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double total = 0.0;
+#pragma unroll
+    for (int w = 0; w < kSumThreads / kWave; ++w) total += wd[w];
+    out[0] = float(total);
+  }
+}
+
 // grad = 2 (a - t) * dL (dL: the upstream gradient of the loss, one float in device memory).
 __global__ __launch_bounds__(kLossThreads) void sq_diff_grad_kernel(const float *__restrict__ a,
                                                                     const float *__restrict__ t, int64_t n,

@@ -492,10 +492,16 @@ void bwd_launch(int p, BlendIn in, const float *final_T, const int32_t *ncontrib
   }
 
 // Forward over all channels, 64 per pass (each pass walks the lists again).
+// Per-wave slots of the fused loss per 64-channel pass (any pixels-per-lane).
+int64_t l2_slots(int64_t ntiles) { return 4 * ntiles + 8; }
+int64_t l2_passes(int C) { return C > 0 ? (C + 63) / 64 : 1; }
+
 template <bool kWalk>
 void blend_forward(BlendIn in, int ppl, float *out, float *final_T, int32_t *ncontrib, int32_t *walk,
-                   hipStream_t s) {
-  if (kWalk) launch_fill_zero(walk, 2 * int64_t(in.ntiles), s);
+                   hipStream_t s, float *loss = nullptr) {
+  launch_fill_zero_words({{kWalk ? walk : nullptr, 2 * int64_t(in.ntiles)},
+                          {in.l2_part, in.l2_part ? in.l2_slots * l2_passes(in.C) : 0}},
+                         s);
   int c0 = 0;
   do {
     in.c0 = c0;
@@ -507,6 +513,8 @@ void blend_forward(BlendIn in, int ppl, float *out, float *final_T, int32_t *nco
 #undef CALL
     c0 += 64;
   } while (c0 < in.C);
+  if (in.l2_part && loss)
+    sum_floats_kernel<<<1, kSumThreads, 0, s>>>(in.l2_part, in.l2_slots * l2_passes(in.C), loss);
 }
 
 // Adjoint: zero-fills the gradients, then accumulates them pass by pass.
@@ -1150,12 +1158,17 @@ int msplat_rasterize_backward(const float *uv, const float *conic, const float *
                                          dL_duv, dL_dconic, dL_dopacity, dL_dfeature, stream);
 }
 
-int msplat_rasterize_views_forward(const float *uv, const float *conic, const float *opacity, const float *feature,
-                                   int64_t n, int C, int feat_per_view, int B, const int *W, const int *H,
-                                   const int64_t *first, const int64_t *count, const uint32_t *rect,
-                                   const int32_t *cell_ids, const int32_t *cell_range, int cell, float bg, int ppl,
-                                   int cull, float *rendered, float *final_T, int32_t *ncontrib, int32_t *walk,
-                                   void *stream) {
+}  // extern "C"
+
+namespace {
+// The cell-list forward of every entry point; target / part / loss: the
+// fused squared-error loss (optional; part holds l2_slots * l2_passes floats).
+int rasterize_forward_impl(const float *uv, const float *conic, const float *opacity, const float *feature,
+                           int64_t n, int C, int feat_per_view, int B, const int *W, const int *H,
+                           const int64_t *first, const int64_t *count, const uint32_t *rect,
+                           const int32_t *cell_ids, const int32_t *cell_range, int cell, float bg, int ppl, int cull,
+                           float *rendered, float *final_T, int32_t *ncontrib, int32_t *walk, const float *target,
+                           float *part, float *loss, void *stream) {
   REQUIRE(n >= 0 && C >= 0 && B >= 1, "rasterize_forward: n = %lld, C = %d", (long long)n, C);
   REQUIRE(valid_cell(cell), "rasterize_forward: cell = %d tiles", cell);
   Views vw;
@@ -1165,11 +1178,90 @@ int msplat_rasterize_views_forward(const float *uv, const float *conic, const fl
   REQUIRE(n == 0 || (uv && conic && opacity && rect && cell_ids && (C == 0 || feature)),
           "rasterize_forward: null input");
   REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "rasterize_forward: rect must be 8-byte aligned");
-  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, vw,
-                                  feat_per_view != 0);
-  blend_forward<true>(in, ppl, rendered, final_T, ncontrib, walk, as_stream(stream));
+  BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, vw,
+                            feat_per_view != 0);
+  if (target) {
+    REQUIRE(part && loss && C > 0, "rasterize_forward: the fused loss needs a workspace, a loss output and channels");
+    in.l2_target = target;
+    in.l2_part = part;
+    in.l2_slots = l2_slots(in.ntiles);
+  }
+  blend_forward<true>(in, ppl, rendered, final_T, ncontrib, walk, as_stream(stream), loss);
   return launched("rasterize_forward");
 }
+}  // namespace
+
+extern "C" {
+
+int msplat_rasterize_views_forward(const float *uv, const float *conic, const float *opacity, const float *feature,
+                                   int64_t n, int C, int feat_per_view, int B, const int *W, const int *H,
+                                   const int64_t *first, const int64_t *count, const uint32_t *rect,
+                                   const int32_t *cell_ids, const int32_t *cell_range, int cell, float bg, int ppl,
+                                   int cull, float *rendered, float *final_T, int32_t *ncontrib, int32_t *walk,
+                                   void *stream) {
+  return rasterize_forward_impl(uv, conic, opacity, feature, n, C, feat_per_view, B, W, H, first, count, rect,
+                                cell_ids, cell_range, cell, bg, ppl, cull, rendered, final_T, ncontrib, walk, nullptr,
+                                nullptr, nullptr, stream);
+}
+
+size_t msplat_rasterize_l2_workspace_bytes(int W, int H, int C) {
+  if (!valid_image(W, H) || C < 0) return 0;
+  return size_t(l2_slots(int64_t(tiles_x(W)) * tiles_y(H)) * l2_passes(C)) * sizeof(float);
+}
+
+int msplat_rasterize_forward_l2(const float *uv, const float *conic, const float *opacity, const float *feature,
+                                int64_t n, int C, const uint32_t *rect, const int32_t *cell_ids,
+                                const int32_t *cell_range, int cell, float bg, int W, int H, int ppl, int cull,
+                                const float *target, void *workspace, size_t workspace_bytes, float *rendered,
+                                float *final_T, int32_t *ncontrib, int32_t *walk, float *loss, void *stream) {
+  REQUIRE(target && loss, "rasterize_forward_l2: null target or loss");
+  const size_t need = msplat_rasterize_l2_workspace_bytes(W, H, C);
+  if (workspace_bytes < need)
+    return fail(MSPLAT_ERR_WORKSPACE, "rasterize_forward_l2: workspace %zu < %zu bytes", workspace_bytes, need);
+  REQUIRE(workspace, "rasterize_forward_l2: null workspace");
+  return rasterize_forward_impl(uv, conic, opacity, feature, n, C, 0, 1, &W, &H, nullptr, nullptr, rect, cell_ids,
+                                cell_range, cell, bg, ppl, cull, rendered, final_T, ncontrib, walk, target,
+                                static_cast<float *>(workspace), loss, stream);
+}
+
+}  // extern "C"
+
+namespace {
+// The cell-list adjoint of every entry point; rendered / target / dL_dloss:
+// the fused loss's gradient (optional; dL_drendered may then be NULL).
+int rasterize_backward_impl(const float *uv, const float *conic, const float *opacity, const float *feature,
+                            int64_t n, int C, int feat_per_view, int B, const int *W, const int *H,
+                            const uint32_t *rect, const int32_t *cell_ids, const int32_t *cell_range, int cell,
+                            float bg, int ppl, int cull, const float *final_T, const int32_t *ncontrib,
+                            const int32_t *walk, const float *dL_drendered, const float *rendered,
+                            const float *target, const float *dL_dloss, float *dL_duv, float *dL_dconic,
+                            float *dL_dopacity, float *dL_dfeature, void *stream) {
+  REQUIRE(n >= 0 && C >= 0 && B >= 1, "rasterize_backward: n = %lld, C = %d", (long long)n, C);
+  REQUIRE(valid_cell(cell), "rasterize_backward: cell = %d tiles", cell);
+  Views vw;
+  const std::string why = make_views(B, W, H, cell, nullptr, nullptr, n, vw);
+  REQUIRE(why.empty(), "rasterize_backward: %s", why.c_str());
+  if (n == 0) return MSPLAT_OK;
+  const bool l2 = dL_dloss != nullptr;
+  REQUIRE(!l2 || (rendered && target), "rasterize_backward: the fused loss needs the rendered and target images");
+  REQUIRE(uv && conic && opacity && rect && cell_ids && cell_range && final_T && ncontrib && walk && dL_duv &&
+              dL_dconic && dL_dopacity && (C == 0 || (feature && (dL_drendered || l2) && dL_dfeature)),
+          "rasterize_backward: null pointer");
+  REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "rasterize_backward: rect must be 8-byte aligned");
+  BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, vw,
+                            feat_per_view != 0);
+  if (l2) {
+    in.l2_rendered = rendered;
+    in.l2_target = target;
+    in.l2_dL = dL_dloss;
+  }
+  blend_backward<true>(in, ppl, vw.vbase[B], n, feat_per_view ? vw.vbase[B] : n, final_T, ncontrib, walk,
+                       dL_drendered, dL_duv, dL_dconic, dL_dopacity, dL_dfeature, as_stream(stream));
+  return launched("rasterize_backward");
+}
+}  // namespace
+
+extern "C" {
 
 int msplat_rasterize_views_backward(const float *uv, const float *conic, const float *opacity, const float *feature,
                                     int64_t n, int C, int feat_per_view, int B, const int *W, const int *H,
@@ -1178,21 +1270,22 @@ int msplat_rasterize_views_backward(const float *uv, const float *conic, const f
                                     const int32_t *ncontrib, const int32_t *walk, const float *dL_drendered,
                                     float *dL_duv, float *dL_dconic, float *dL_dopacity, float *dL_dfeature,
                                     void *stream) {
-  REQUIRE(n >= 0 && C >= 0 && B >= 1, "rasterize_backward: n = %lld, C = %d", (long long)n, C);
-  REQUIRE(valid_cell(cell), "rasterize_backward: cell = %d tiles", cell);
-  Views vw;
-  const std::string why = make_views(B, W, H, cell, nullptr, nullptr, n, vw);
-  REQUIRE(why.empty(), "rasterize_backward: %s", why.c_str());
-  if (n == 0) return MSPLAT_OK;
-  REQUIRE(uv && conic && opacity && rect && cell_ids && cell_range && final_T && ncontrib && walk && dL_duv &&
-              dL_dconic && dL_dopacity && (C == 0 || (feature && dL_drendered && dL_dfeature)),
-          "rasterize_backward: null pointer");
-  REQUIRE((reinterpret_cast<uintptr_t>(rect) & 7) == 0, "rasterize_backward: rect must be 8-byte aligned");
-  const BlendIn in = blend_inputs(uv, conic, opacity, feature, C, cell_ids, cell_range, rect, cell, cull, bg, vw,
-                                  feat_per_view != 0);
-  blend_backward<true>(in, ppl, vw.vbase[B], n, feat_per_view ? vw.vbase[B] : n, final_T, ncontrib, walk,
-                       dL_drendered, dL_duv, dL_dconic, dL_dopacity, dL_dfeature, as_stream(stream));
-  return launched("rasterize_backward");
+  return rasterize_backward_impl(uv, conic, opacity, feature, n, C, feat_per_view, B, W, H, rect, cell_ids,
+                                 cell_range, cell, bg, ppl, cull, final_T, ncontrib, walk, dL_drendered, nullptr,
+                                 nullptr, nullptr, dL_duv, dL_dconic, dL_dopacity, dL_dfeature, stream);
+}
+
+int msplat_rasterize_backward_l2(const float *uv, const float *conic, const float *opacity, const float *feature,
+                                 int64_t n, int C, const uint32_t *rect, const int32_t *cell_ids,
+                                 const int32_t *cell_range, int cell, float bg, int W, int H, int ppl, int cull,
+                                 const float *final_T, const int32_t *ncontrib, const int32_t *walk,
+                                 const float *dL_drendered, const float *rendered, const float *target,
+                                 const float *dL_dloss, float *dL_duv, float *dL_dconic, float *dL_dopacity,
+                                 float *dL_dfeature, void *stream) {
+  REQUIRE(dL_dloss, "rasterize_backward_l2: null dL_dloss");
+  return rasterize_backward_impl(uv, conic, opacity, feature, n, C, 0, 1, &W, &H, rect, cell_ids, cell_range, cell,
+                                 bg, ppl, cull, final_T, ncontrib, walk, dL_drendered, rendered, target, dL_dloss,
+                                 dL_duv, dL_dconic, dL_dopacity, dL_dfeature, stream);
 }
 
 // ---------------------------------------------------------------- many views: preprocessing

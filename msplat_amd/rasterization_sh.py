@@ -50,87 +50,136 @@ def rasterization_sh(xyz, scale, rotate, opacity, shs, intr, extr, campos, W: in
                               extra, bool(depth_channel), ndc)
 
 
+def rasterization_sh_l2(xyz, scale, rotate, opacity, shs, intr, extr, campos, W: int, H: int, bg: float, target,
+                        extra=None, depth_channel: bool = False, ndc=None):
+    """``(image, loss)``: ``rasterization_sh`` and ``l2_loss(image, target)`` in
+    one op.  The loss is summed inside the blending forward and its gradient
+    formed inside the blending adjoint (no loss or gradient image launches);
+    ``image`` stays differentiable for other loss terms.  Results: the image
+    as rasterization_sh's, the loss to fp32 rounding of the same sum,
+    gradients equal to the composition's."""
+    return _RasterizeSHL2.apply(xyz, scale, rotate, opacity, shs, intr, extr, campos, int(W), int(H), float(bg),
+                                extra, bool(depth_channel), ndc, target)
+
+
+def _forward(ctx, xyz, scale, rotate, opacity, shs, intr, extr, campos, W, H, bg, extra, depth_channel, ndc,
+             target=None):
+    s = _lib.f32("shs", shs)
+    n, Csh, D = s.shape
+    x = _lib.f32("xyz", xyz)
+    cp = _lib.f32("campos", campos).reshape(3)
+    E = 0 if extra is None else int(extra.shape[1])
+    C = Csh + E + (1 if depth_channel else 0)
+    dev = x.device
+    main = torch.cuda.current_stream(dev)
+    side = _side_stream(dev)
+    feat = torch.empty((n, C), dtype=torch.float32, device=dev)
+    cfg = tuning.config_for(W, H, C)
+    with torch.cuda.device(dev):
+        # SH colours into feature columns [0, Csh) on the side stream
+        ready = torch.cuda.Event()
+        ready.record(main)
+        side.wait_event(ready)
+        with torch.cuda.stream(side):
+            _lib.call("msplat_compute_sh_points_forward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None, n, Csh,
+                      D, _lib.ptr(feat), C, side.cuda_stream)
+            sh_done = torch.cuda.Event()
+            sh_done.record(side)
+        for t in (s, x, cp, feat):
+            t.record_stream(side)
+        # meanwhile: geometry, the other columns, binning
+        uv, depth, conic, radius, _ = preprocess_forward(xyz, scale, rotate, intr, extr, W, H,
+                                                         depth_col=feat[:, C - 1] if depth_channel else None)
+        if E:
+            feat[:, Csh:Csh + E].copy_(extra)
+        bins = bin_gaussians(uv, depth, W, H, radius, cfg.cell)
+        main.wait_event(sh_done)
+        res = rasterize_forward(uv, conic, opacity, feat, bins, bg, W, H, cfg.ppl_fwd, cfg.cull, target=target)
+    out, final_T, ncontrib, walk = res[:4]
+    ctx.W, ctx.H, ctx.bg, ctx.cell, ctx.ppl_bwd, ctx.cull = W, H, bg, cfg.cell, cfg.ppl_bwd, cfg.cull
+    ctx.dims = (Csh, D, E, C, depth_channel)
+    ctx.campos_shape = campos.shape
+    ctx.has_ndc = ndc is not None
+    ctx.l2 = target is not None
+    l2_saved = (out, _lib.f32("target", target)) if ctx.l2 else ()
+    ctx.save_for_backward(xyz, scale, rotate, opacity, s, intr, extr, cp, feat, uv, depth, conic, radius,
+                          bins.rect, bins.cell_ids, bins.cell_range, final_T, ncontrib, walk, *l2_saved)
+    return out if target is None else (out, res[4])
+
+
+def _backward(ctx, dL_dout, dL_dloss=None):
+    saved = ctx.saved_tensors
+    (xyz, scale, rotate, opacity, s, intr, extr, cp, feat, uv, depth, conic, radius, rect, cell_ids, cell_range,
+     final_T, ncontrib, walk) = saved[:19]
+    l2 = (saved[19], saved[20], dL_dloss) if (ctx.l2 and dL_dloss is not None) else None
+    Csh, D, E, C, depth_channel = ctx.dims
+    n = xyz.shape[0]
+    dev = xyz.device
+    main = torch.cuda.current_stream(dev)
+    side = _side_stream(dev)
+    bins = CellLists(rect, cell_ids, cell_range, ctx.cell)
+    if l2 is None and dL_dout is None:  # neither output reaches the loss
+        dL_dout = torch.zeros((C, ctx.H, ctx.W), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        duv, dconic, dopac, dfeat = rasterize_backward(uv, conic, opacity, feat, bins, ctx.bg, ctx.W, ctx.H,
+                                                       final_T, ncontrib, walk, dL_dout, ctx.ppl_bwd, ctx.cull,
+                                                       l2=l2)
+        # SH adjoint (feature columns [0, Csh)) on the side stream, next to the geometry adjoint
+        x = _lib.f32("xyz", xyz)
+        dshs = torch.empty((n, Csh, D), dtype=torch.float32, device=dev)
+        dxyz_sh = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        ready = torch.cuda.Event()
+        ready.record(main)
+        side.wait_event(ready)
+        with torch.cuda.stream(side):
+            _lib.call("msplat_compute_sh_points_backward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None,
+                      _lib.ptr(dfeat), C, n, Csh, D, _lib.ptr(dshs), _lib.ptr(dxyz_sh), side.cuda_stream)
+            sh_done = torch.cuda.Event()
+            sh_done.record(side)
+        for t in (s, x, cp, dfeat, dshs, dxyz_sh):
+            t.record_stream(side)
+        dxyz, dscale, drot, dintr, dextr = preprocess_backward(
+            xyz, scale, rotate, intr, extr, depth, radius, duv, dconic,
+            dL_ddepth=dfeat[:, C - 1] if depth_channel else None,
+            want_intr=intr.requires_grad, want_extr=extr.requires_grad)
+        main.wait_event(sh_done)
+        dxyz.add_(dxyz_sh)
+        dcampos = (-dxyz_sh.sum(0)).view(ctx.campos_shape) if ctx.needs_input_grad[7] else None
+    dextra = dfeat[:, Csh:Csh + E] if (E and ctx.needs_input_grad[11]) else None
+    dndc = None
+    if ctx.has_ndc:
+        dndc = duv * torch.tensor([0.5 * ctx.W, 0.5 * ctx.H], dtype=duv.dtype, device=duv.device)[None]
+    return (dxyz, dscale, drot, dopac.view_as(opacity), dshs, dintr,
+            camera_grad_shape(extr, dextr) if dextr is not None else None, dcampos, None, None, None,
+            dextra, None, dndc)
+
+
 class _RasterizeSH(torch.autograd.Function):
     @staticmethod
     def forward(ctx, xyz, scale, rotate, opacity, shs, intr, extr, campos, W, H, bg, extra, depth_channel, ndc):
-        s = _lib.f32("shs", shs)
-        n, Csh, D = s.shape
-        x = _lib.f32("xyz", xyz)
-        cp = _lib.f32("campos", campos).reshape(3)
-        E = 0 if extra is None else int(extra.shape[1])
-        C = Csh + E + (1 if depth_channel else 0)
-        dev = x.device
-        main = torch.cuda.current_stream(dev)
-        side = _side_stream(dev)
-        feat = torch.empty((n, C), dtype=torch.float32, device=dev)
-        cfg = tuning.config_for(W, H, C)
-        with torch.cuda.device(dev):
-            # SH colours into feature columns [0, Csh) on the side stream
-            ready = torch.cuda.Event()
-            ready.record(main)
-            side.wait_event(ready)
-            with torch.cuda.stream(side):
-                _lib.call("msplat_compute_sh_points_forward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None, n, Csh,
-                          D, _lib.ptr(feat), C, side.cuda_stream)
-                sh_done = torch.cuda.Event()
-                sh_done.record(side)
-            for t in (s, x, cp, feat):
-                t.record_stream(side)
-            # meanwhile: geometry, the other columns, binning
-            uv, depth, conic, radius, _ = preprocess_forward(xyz, scale, rotate, intr, extr, W, H,
-                                                             depth_col=feat[:, C - 1] if depth_channel else None)
-            if E:
-                feat[:, Csh:Csh + E].copy_(extra)
-            bins = bin_gaussians(uv, depth, W, H, radius, cfg.cell)
-            main.wait_event(sh_done)
-            out, final_T, ncontrib, walk = rasterize_forward(uv, conic, opacity, feat, bins, bg, W, H, cfg.ppl_fwd,
-                                                             cfg.cull)
-        ctx.W, ctx.H, ctx.bg, ctx.cell, ctx.ppl_bwd, ctx.cull = W, H, bg, cfg.cell, cfg.ppl_bwd, cfg.cull
-        ctx.dims = (Csh, D, E, C, depth_channel)
-        ctx.campos_shape = campos.shape
-        ctx.has_ndc = ndc is not None
-        ctx.save_for_backward(xyz, scale, rotate, opacity, s, intr, extr, cp, feat, uv, depth, conic, radius,
-                              bins.rect, bins.cell_ids, bins.cell_range, final_T, ncontrib, walk)
-        return out
+        return _forward(ctx, xyz, scale, rotate, opacity, shs, intr, extr, campos, W, H, bg, extra, depth_channel,
+                        ndc)
 
     @staticmethod
     def backward(ctx, dL_dout):
-        (xyz, scale, rotate, opacity, s, intr, extr, cp, feat, uv, depth, conic, radius, rect, cell_ids, cell_range,
-         final_T, ncontrib, walk) = ctx.saved_tensors
-        Csh, D, E, C, depth_channel = ctx.dims
-        n = xyz.shape[0]
-        dev = xyz.device
-        main = torch.cuda.current_stream(dev)
-        side = _side_stream(dev)
-        bins = CellLists(rect, cell_ids, cell_range, ctx.cell)
-        with torch.cuda.device(dev):
-            duv, dconic, dopac, dfeat = rasterize_backward(uv, conic, opacity, feat, bins, ctx.bg, ctx.W, ctx.H,
-                                                           final_T, ncontrib, walk, dL_dout, ctx.ppl_bwd, ctx.cull)
-            # SH adjoint (feature columns [0, Csh)) on the side stream, next to the geometry adjoint
-            x = _lib.f32("xyz", xyz)
-            dshs = torch.empty((n, Csh, D), dtype=torch.float32, device=dev)
-            dxyz_sh = torch.empty((n, 3), dtype=torch.float32, device=dev)
-            ready = torch.cuda.Event()
-            ready.record(main)
-            side.wait_event(ready)
-            with torch.cuda.stream(side):
-                _lib.call("msplat_compute_sh_points_backward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None,
-                          _lib.ptr(dfeat), C, n, Csh, D, _lib.ptr(dshs), _lib.ptr(dxyz_sh), side.cuda_stream)
-                sh_done = torch.cuda.Event()
-                sh_done.record(side)
-            for t in (s, x, cp, dfeat, dshs, dxyz_sh):
-                t.record_stream(side)
-            dxyz, dscale, drot, dintr, dextr = preprocess_backward(
-                xyz, scale, rotate, intr, extr, depth, radius, duv, dconic,
-                dL_ddepth=dfeat[:, C - 1] if depth_channel else None,
-                want_intr=intr.requires_grad, want_extr=extr.requires_grad)
-            main.wait_event(sh_done)
-            dxyz.add_(dxyz_sh)
-            dcampos = (-dxyz_sh.sum(0)).view(ctx.campos_shape) if ctx.needs_input_grad[7] else None
-        dextra = dfeat[:, Csh:Csh + E] if (E and ctx.needs_input_grad[11]) else None
-        dndc = None
-        if ctx.has_ndc:
-            dndc = duv * torch.tensor([0.5 * ctx.W, 0.5 * ctx.H], dtype=duv.dtype, device=duv.device)[None]
-        return (dxyz, dscale, drot, dopac.view_as(opacity), dshs, dintr,
-                camera_grad_shape(extr, dextr) if dextr is not None else None, dcampos, None, None, None,
-                dextra, None, dndc)
+        return _backward(ctx, dL_dout)
+
+
+class _RasterizeSHL2(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, scale, rotate, opacity, shs, intr, extr, campos, W, H, bg, extra, depth_channel, ndc,
+                target):
+        ctx.set_materialize_grads(False)  # an unused image passes no gradient image
+        out, loss = _forward(ctx, xyz, scale, rotate, opacity, shs, intr, extr, campos, W, H, bg, extra,
+                             depth_channel, ndc, target)
+        ctx.target_grad = ctx.needs_input_grad[14]
+        return out, loss
+
+    @staticmethod
+    def backward(ctx, dL_dout, dL_dloss):
+        grads = _backward(ctx, dL_dout, dL_dloss)
+        dtarget = None
+        if ctx.target_grad and dL_dloss is not None:  # d/dtarget of sum (out - t)^2 = -2 (out - t) dL
+            out, t = ctx.saved_tensors[19], ctx.saved_tensors[20]
+            dtarget = (t - out) * (2.0 * dL_dloss)
+        return (*grads, dtarget)

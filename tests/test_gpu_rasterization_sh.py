@@ -51,3 +51,63 @@ def test_matches_composition(deg, E, depth_channel):
         if A[k] is None:
             continue
         assert rel_err(np_(A[k].grad), np_(B[k].grad)) < 1e-5, k
+
+
+@pytest.mark.parametrize("other_term,target_grad", [(False, False), (True, False), (False, True), (True, True)])
+def test_l2_fused_matches_separate_loss(other_term, target_grad):
+    """rasterization_sh_l2 = rasterization_sh + l2_loss: the image bit for bit,
+    the loss to fp32 rounding, every gradient (the image's other terms and the
+    target's included) to float summation order."""
+    W, H = 176, 128
+    g, cam = small_scene(31, n=3000, W=W, H=H, half_extent=2.0, C=3)
+    gen = torch.Generator().manual_seed(31)
+    shs = torch.randn(3000, 3, 16, generator=gen) * 0.3
+    target = torch.rand(4, H, W, generator=gen).to(DEV)
+    wimg = torch.randn(4, H, W, generator=gen).to(DEV)
+    intr = cam.intr.to(DEV)
+
+    def leaves():
+        P = {k: g[k].to(DEV).clone().requires_grad_() for k in ("xyz", "scale", "rotate", "opacity")}
+        P["shs"] = shs.to(DEV).clone().requires_grad_()
+        P["extr"] = cam.extr.to(DEV).clone().requires_grad_()
+        P["target"] = target.clone().requires_grad_(target_grad)
+        return P
+
+    A, B = leaves(), leaves()
+    img_a, loss_a = ms.rasterization_sh_l2(A["xyz"], A["scale"], A["rotate"], A["opacity"], A["shs"], intr, A["extr"],
+                                           cam.eye.to(DEV), W, H, 0.1, A["target"], depth_channel=True)
+    img_b = ms.rasterization_sh(B["xyz"], B["scale"], B["rotate"], B["opacity"], B["shs"], intr, B["extr"],
+                                cam.eye.to(DEV), W, H, 0.1, depth_channel=True)
+    loss_b = ms.l2_loss(img_b, B["target"])
+    assert torch.equal(img_a, img_b)
+    ref64 = ((img_b.detach().double() - target.double()) ** 2).sum()
+    assert abs(float(loss_a) - float(ref64)) <= 1e-6 * float(ref64)
+    assert abs(float(loss_a) - float(loss_b)) <= 1e-6 * float(ref64)
+    total_a = loss_a * 0.75 + ((img_a * wimg).sum() if other_term else 0.0)
+    total_b = loss_b * 0.75 + ((img_b * wimg).sum() if other_term else 0.0)
+    total_a.backward()
+    total_b.backward()
+    for k in ("xyz", "scale", "rotate", "opacity", "shs", "extr") + (("target",) if target_grad else ()):
+        assert rel_err(np_(A[k].grad), np_(B[k].grad)) < 1e-5, k
+
+
+def test_l2_fused_image_only():
+    """Gradients through the image alone (the loss unused) equal rasterization_sh's."""
+    W, H = 128, 96
+    g, cam = small_scene(32, n=2000, W=W, H=H, half_extent=2.0, C=3)
+    gen = torch.Generator().manual_seed(32)
+    shs = torch.randn(2000, 3, 4, generator=gen) * 0.3
+    target = torch.rand(3, H, W, generator=gen).to(DEV)
+    dout = torch.randn(3, H, W, generator=gen).to(DEV)
+    A = {k: g[k].to(DEV).clone().requires_grad_() for k in ("xyz", "opacity")}
+    B = {k: g[k].to(DEV).clone().requires_grad_() for k in ("xyz", "opacity")}
+    args = (g["scale"].to(DEV), g["rotate"].to(DEV))
+    img_a, _ = ms.rasterization_sh_l2(A["xyz"], *args, A["opacity"], shs.to(DEV), cam.intr.to(DEV),
+                                      cam.extr.to(DEV), cam.eye.to(DEV), W, H, 0.0, target)
+    img_b = ms.rasterization_sh(B["xyz"], *args, B["opacity"], shs.to(DEV), cam.intr.to(DEV), cam.extr.to(DEV),
+                                cam.eye.to(DEV), W, H, 0.0)
+    assert torch.equal(img_a, img_b)
+    (img_a * dout).sum().backward()
+    (img_b * dout).sum().backward()
+    for k in ("xyz", "opacity"):
+        assert rel_err(np_(A[k].grad), np_(B[k].grad)) < 1e-5, k
