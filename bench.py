@@ -213,6 +213,16 @@ def _zeros(like):
     return _ZEROS[key]
 
 
+def _quiet_gc():
+    """Collect, then freeze every object alive so far: a full collection of the
+    interpreter's heap (torch and its modules) otherwise lands inside a timed
+    step now and then, a host pause of tens of milliseconds."""
+    import gc
+
+    gc.collect()
+    gc.freeze()
+
+
 def _time_steps(fn, steps: int, warmup: int) -> float:
     """Average milliseconds of fn() over `steps` after `warmup` (synchronised)."""
     import torch
@@ -220,6 +230,7 @@ def _time_steps(fn, steps: int, warmup: int) -> float:
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    _quiet_gc()
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
@@ -348,6 +359,7 @@ def bench_serving(dev, rate: float, duration: float) -> dict:
         return serving.Request(rng.choice(names), cam.intr, cam.extr, W, H)
 
     server = serving.RenderServer(store, hbm_budget=32 << 30, max_views=16, streams=3)
+    _quiet_gc()
     stats = serving.run_poisson(server, make, rate=rate, duration=duration, seed=5)
     log(f"serving: {stats}")
     stats["workload"] = ("4 resident scenes (100k, 500k, 1.5M, 3M), Poisson arrivals, 640x480 / 1280x720 / "
@@ -460,6 +472,7 @@ def main():
     lib.msplat_timing_only(",".join(k for k in (dom, hd) if k).encode())
     lib.msplat_timing_enable(1)
     torch.cuda.synchronize()
+    _quiet_gc()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()

@@ -63,11 +63,12 @@ def _observe(count: torch.Tensor, capacity, k: int):
 _host = threading.local()
 
 
-def _host_count() -> torch.Tensor:
-    """A pinned host word per thread for the entry count's copy."""
+def _host_count():
+    """A pinned host word and an event per thread for the entry count's copy
+    (kept: creating and destroying an event per call costs more than the wait)."""
     buf = getattr(_host, "count", None)
     if buf is None:
-        buf = _host.count = torch.empty(1, dtype=torch.int64, pin_memory=True)
+        buf = _host.count = (torch.empty(1, dtype=torch.int64, pin_memory=True), torch.cuda.Event())
     return buf
 
 
@@ -81,14 +82,16 @@ def prepare_views(u, d, r, n: int, B: int, W, H, F, N, cell: int, rect, prep, pr
             prep_bytes, _lib.ptr(count), _lib.stream_ptr())
     _lib.call("msplat_bin_views_count", *args)
     if capacity is None:
-        host = _host_count()
+        host, ready = _host_count()
         host.copy_(count, non_blocking=True)
-        ready = torch.cuda.Event()
         ready.record()
     _lib.call("msplat_bin_views_order", *args)
     if capacity is not None:
         return int(capacity)
-    ready.synchronize()  # the only host synchronisation of the binning
+    # the only host synchronisation of the binning: poll the event (a blocking
+    # event wait can sleep far longer than the copy takes when the GPU is idle)
+    while not ready.query():
+        pass
     k = int(host[0])
     if k < 0:
         raise RuntimeError("cell binning: a bounded look-back wait expired")
