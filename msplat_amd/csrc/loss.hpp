@@ -98,13 +98,28 @@ __global__ __launch_bounds__(kSumThreads) void sq_diff_sum_kernel(const float *_
 }
 
 // out = the sum of n floats, in fp64 and a fixed order (one workgroup):
-// thread t adds p[t], p[t + 1024], ..., then a fixed shuffle tree and the
-// waves in order.  The fused rasterizer loss's per-wave partials.
+// thread t adds p[t + 1024 (8 k + u)] into running sum u, then the sums,
+// a fixed shuffle tree and the waves in order.  The fused rasterizer loss's
+// per-wave partials.
 __global__ __launch_bounds__(kSumThreads) void sum_floats_kernel(const float *__restrict__ p, int64_t n,
                                                                  float *__restrict__ out) {
   __shared__ double wd[kSumThreads / kWave];
-  double v = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kSumThreads) v += double(p[i]);
+  // 8 loads in flight per thread, 8 running sums, added in a fixed order
+  constexpr int kU = 8;
+  double acc[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) acc[u] = 0.0;
+  for (int64_t base = threadIdx.x; base < n; base += int64_t(kU) * kSumThreads) {
+    float x[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = base + int64_t(u) * kSumThreads;
+      x[u] = i < n ? p[i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) acc[u] += double(x[u]);
+  }
+  double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
