@@ -73,11 +73,14 @@ def _host_count():
 
 
 def prepare_views(u, d, r, n: int, B: int, W, H, F, N, cell: int, rect, prep, prep_bytes: int, count,
-                  capacity) -> int:
+                  capacity, queued=None) -> int:
     """Step 1 of the binning on the current stream: the counting pass, then the
     depth order.  Without a capacity, the entry count is copied to the host
     between the two and read once the sort is queued behind it (the GPU sorts
-    while the count travels); returns that count, or the capacity."""
+    while the count travels); returns that count, or the capacity.
+    ``queued()``, when given, enqueues further independent work on the stream
+    before the host waits: the GPU runs it after the sort instead of idling
+    while the host issues the emission."""
     args = (_lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, B, W, H, F, N, int(cell), _lib.ptr(rect), _lib.ptr(prep),
             prep_bytes, _lib.ptr(count), _lib.stream_ptr())
     _lib.call("msplat_bin_views_count", *args)
@@ -86,6 +89,8 @@ def prepare_views(u, d, r, n: int, B: int, W, H, F, N, cell: int, rect, prep, pr
         host.copy_(count, non_blocking=True)
         ready.record()
     _lib.call("msplat_bin_views_order", *args)
+    if queued is not None:
+        queued()
     if capacity is not None:
         return int(capacity)
     # the only host synchronisation of the binning: poll the event (a blocking
@@ -103,10 +108,12 @@ def cell_grid(W: int, H: int, cell: int) -> tuple[int, int]:
     return (gx + cell - 1) // cell, (gy + cell - 1) // cell
 
 
-def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8, capacity: int | None = None) -> CellLists:
+def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8, capacity: int | None = None,
+                  queued=None) -> CellLists:
     """Cell lists of one view.  With ``capacity`` the entry count stays on the
     device (no host synchronisation: the call can be captured in a graph);
-    ``overflowed()`` tells afterwards whether the capacity was too small."""
+    ``overflowed()`` tells afterwards whether the capacity was too small.
+    ``queued``: see prepare_views."""
     u = _lib.f32("uv", uv)
     d = _lib.f32("depth", depth).view(-1)
     r = _lib.i32("radius", radius).view(-1)
@@ -122,7 +129,7 @@ def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8, capacity: in
         prep = torch.empty(max(prep_bytes, 1), dtype=torch.uint8, device=dev)
         count = torch.empty(1, dtype=torch.int64, device=dev)
         Wa, Ha = (_lib.I32 * 1)(W), (_lib.I32 * 1)(H)
-        k = prepare_views(u, d, r, n, 1, Wa, Ha, None, None, cell, rect, prep, prep_bytes, count, capacity)
+        k = prepare_views(u, d, r, n, 1, Wa, Ha, None, None, cell, rect, prep, prep_bytes, count, capacity, queued)
         _observe(count, capacity, k)
         cell_ids = torch.empty(k, dtype=torch.int32, device=dev)
         cell_range = torch.empty((ncx * ncy, 2), dtype=torch.int32, device=dev)

@@ -10,9 +10,12 @@ msplat/project_point.py, msplat/__init__.py:22-93), with three differences
 in how it runs:
 
 * the SH kernel writes its channels straight into the feature rows (no
-  ``cat``) and runs on a side stream while the main stream projects and
-  bins the Gaussians, whose sort kernels leave most of the GPU idle; the
-  adjoint likewise overlaps the SH adjoint with the geometry adjoint;
+  ``cat``) and evaluates only the Gaussians the projection kept (no others
+  enter a cell list); it is queued behind the binning's depth sort, so the
+  GPU runs it while the host waits for the entry count (beside the sort's
+  latency-bound kernels, a bandwidth-bound kernel over every Gaussian slowed
+  them more than it saved); the adjoint runs the SH adjoint on a side
+  stream beside the geometry adjoint;
 * the depth channel is the fused preprocessing's own depth (no second
   projection), and its gradient enters the geometry adjoint directly;
 * one autograd node instead of four.
@@ -71,29 +74,23 @@ def _forward(ctx, xyz, scale, rotate, opacity, shs, intr, extr, campos, W, H, bg
     E = 0 if extra is None else int(extra.shape[1])
     C = Csh + E + (1 if depth_channel else 0)
     dev = x.device
-    main = torch.cuda.current_stream(dev)
-    side = _side_stream(dev)
     feat = torch.empty((n, C), dtype=torch.float32, device=dev)
     cfg = tuning.config_for(W, H, C)
     with torch.cuda.device(dev):
-        # SH colours into feature columns [0, Csh) on the side stream
-        ready = torch.cuda.Event()
-        ready.record(main)
-        side.wait_event(ready)
-        with torch.cuda.stream(side):
-            _lib.call("msplat_compute_sh_points_forward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None, n, Csh,
-                      D, _lib.ptr(feat), C, side.cuda_stream)
-            sh_done = torch.cuda.Event()
-            sh_done.record(side)
-        for t in (s, x, cp, feat):
-            t.record_stream(side)
-        # meanwhile: geometry, the other columns, binning
         uv, depth, conic, radius, _ = preprocess_forward(xyz, scale, rotate, intr, extr, W, H,
                                                          depth_col=feat[:, C - 1] if depth_channel else None)
         if E:
             feat[:, Csh:Csh + E].copy_(extra)
-        bins = bin_gaussians(uv, depth, W, H, radius, cfg.cell)
-        main.wait_event(sh_done)
+        # SH colours into feature columns [0, Csh) for the projected Gaussians
+        # only (the others are in no cell list: their rows get zeros and their
+        # coefficients are not read), queued behind the binning's depth sort
+        vis = (radius > 0).view(torch.uint8)
+
+        def sh_colours():
+            _lib.call("msplat_compute_sh_points_forward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), _lib.ptr(vis), n,
+                      Csh, D, _lib.ptr(feat), C, _lib.stream_ptr())
+
+        bins = bin_gaussians(uv, depth, W, H, radius, cfg.cell, queued=sh_colours)
         res = rasterize_forward(uv, conic, opacity, feat, bins, bg, W, H, cfg.ppl_fwd, cfg.cull, target=target)
     out, final_T, ncontrib, walk = res[:4]
     ctx.W, ctx.H, ctx.bg, ctx.cell, ctx.ppl_bwd, ctx.cull = W, H, bg, cfg.cell, cfg.ppl_bwd, cfg.cull
