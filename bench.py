@@ -375,6 +375,8 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--blend-config", default=None,
+                    help="cell,ppl_fwd,ppl_bwd: use these blending launch parameters (no autotune)")
     ap.add_argument("--cpu-rows", type=int, default=64, help="pixel rows of the CPU blending sample")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a PMC calibration kernel)")
@@ -417,7 +419,10 @@ def main():
         log(f"calibration copy: {args.calibrate_copy_mib} MiB read + {args.calibrate_copy_mib} MiB written")
         del src, dst
 
-    if not args.no_autotune:
+    if args.blend_config:
+        cell, pf, pb = (int(v) for v in args.blend_config.split(","))
+        tuning.record(W, H, C, tuning.BlendConfig(cell, pf, pb))
+    elif not args.no_autotune:
         with torch.no_grad():
             _, depth = preprocess_forward(P["xyz"], P["scale"], P["rotate"], cam.intr, cam.extr, W, H)[:2]
             feat = torch.cat([torch.rand(args.n, 3, device=dev), depth], 1)

@@ -97,39 +97,75 @@ __global__ __launch_bounds__(kSumThreads) void sq_diff_sum_kernel(const float *_
   }
 }
 
-// out = the sum of n floats, in fp64 and a fixed order (one workgroup):
-// thread t adds p[t + 1024 (8 k + u)] into running sum u, then the sums,
-// a fixed shuffle tree and the waves in order.  The fused rasterizer loss's
-// per-wave partials.
+// out = the sum of n floats in a fixed order: workgroup b sums its slice
+// p[b * 1024 * kSumRounds ...] (each thread kSumRounds strided values in
+// fp64, a fixed shuffle tree, the waves in order) into part[b]; the last
+// workgroup to finish (ticket) adds part[0..) in order and writes out.
+// Many workgroups, one load per thread per round: the partials of a whole
+// frame in about one memory latency.  ticket must be 0 on entry; it is
+// reset for the next launch.  The fused rasterizer loss's per-wave partials.
+constexpr int kSumRounds = 4;
+constexpr int kSumParts = 256;  // workgroups at most; more values take more rounds each
+
+inline unsigned sum_floats_blocks(int64_t n) {
+  const int64_t per = int64_t(kSumThreads) * kSumRounds;
+  const int64_t b = (n + per - 1) / per;
+  return unsigned(b < 1 ? 1 : (b > kSumParts ? kSumParts : b));
+}
+
 __global__ __launch_bounds__(kSumThreads) void sum_floats_kernel(const float *__restrict__ p, int64_t n,
+                                                                 double *__restrict__ part,
+                                                                 uint32_t *__restrict__ ticket,
                                                                  float *__restrict__ out) {
   __shared__ double wd[kSumThreads / kWave];
-  // 8 loads in flight per thread, 8 running sums, added in a fixed order
-  constexpr int kU = 8;
-  double acc[kU];
+  __shared__ bool last;
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  // this workgroup's slice of the values, in kSumRounds strided rounds
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = int64_t(blockIdx.x) * per, hi = lo + per < n ? lo + per : n;
+  double acc[kSumRounds];
 #pragma unroll
-  for (int u = 0; u < kU; ++u) acc[u] = 0.0;
-  for (int64_t base = threadIdx.x; base < n; base += int64_t(kU) * kSumThreads) {
-    float x[kU];
+  for (int u = 0; u < kSumRounds; ++u) acc[u] = 0.0;
+  for (int64_t base = lo + threadIdx.x; base < hi; base += int64_t(kSumRounds) * kSumThreads) {
+    float x[kSumRounds];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
+    for (int u = 0; u < kSumRounds; ++u) {
       const int64_t i = base + int64_t(u) * kSumThreads;
-      x[u] = i < n ? p[i] : 0.f;
+      x[u] = i < hi ? p[i] : 0.f;
     }
 #pragma unroll
-    for (int u = 0; u < kU; ++u) acc[u] += double(x[u]);
+    for (int u = 0; u < kSumRounds; ++u) acc[u] += double(x[u]);
   }
-  double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  double v = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
-  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   if (lane == 0) wd[wave] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double total = 0.0;
+    double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kSumThreads / kWave; ++w) total += wd[w];
-    out[0] = float(total);
+    for (int w = 0; w < kSumThreads / kWave; ++w) s += wd[w];
+    part[blockIdx.x] = s;
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (wave == 0) {  // the parts in order: lane l adds parts l, l + 64, ..., then a fixed tree
+    double t = 0.0;
+    for (unsigned b = lane; b < gridDim.x; b += kWave) {
+      const unsigned long long bits =
+          __hip_atomic_load(reinterpret_cast<unsigned long long *>(part) + b, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+      t += __longlong_as_double(static_cast<long long>(bits));
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off, kWave);
+    if (lane == 0) {
+      out[0] = float(t);
+      __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 

@@ -492,16 +492,21 @@ void bwd_launch(int p, BlendIn in, const float *final_T, const int32_t *ncontrib
   }
 
 // Forward over all channels, 64 per pass (each pass walks the lists again).
-// Per-wave slots of the fused loss per 64-channel pass (any pixels-per-lane).
+// Per-wave slots of the fused loss per 64-channel pass (any pixels-per-lane);
+// after them the workspace holds the sum's per-workgroup parts and ticket.
 int64_t l2_slots(int64_t ntiles) { return 4 * ntiles + 8; }
 int64_t l2_passes(int C) { return C > 0 ? (C + 63) / 64 : 1; }
+int64_t l2_part_floats(int64_t ntiles, int C) { return (l2_slots(ntiles) * l2_passes(C) + 63) / 64 * 64; }
+size_t l2_workspace(int64_t ntiles, int C) {
+  return size_t(l2_part_floats(ntiles, C)) * sizeof(float) + size_t(kSumParts) * sizeof(double) + 256;
+}
 
 template <bool kWalk>
 void blend_forward(BlendIn in, int ppl, float *out, float *final_T, int32_t *ncontrib, int32_t *walk,
                    hipStream_t s, float *loss = nullptr) {
-  launch_fill_zero_words({{kWalk ? walk : nullptr, 2 * int64_t(in.ntiles)},
-                          {in.l2_part, in.l2_part ? in.l2_slots * l2_passes(in.C) : 0}},
-                         s);
+  // (the sum's ticket, after its parts, is zeroed with the partials)
+  const int64_t l2_words = in.l2_part ? l2_part_floats(in.ntiles, in.C) + 2 * kSumParts + 1 : 0;
+  launch_fill_zero_words({{kWalk ? walk : nullptr, 2 * int64_t(in.ntiles)}, {in.l2_part, l2_words}}, s);
   int c0 = 0;
   do {
     in.c0 = c0;
@@ -513,8 +518,12 @@ void blend_forward(BlendIn in, int ppl, float *out, float *final_T, int32_t *nco
 #undef CALL
     c0 += 64;
   } while (c0 < in.C);
-  if (in.l2_part && loss)
-    sum_floats_kernel<<<1, kSumThreads, 0, s>>>(in.l2_part, in.l2_slots * l2_passes(in.C), loss);
+  if (in.l2_part && loss) {
+    const int64_t nv = in.l2_slots * l2_passes(in.C);
+    double *parts = reinterpret_cast<double *>(in.l2_part + l2_part_floats(in.ntiles, in.C));
+    uint32_t *ticket = reinterpret_cast<uint32_t *>(parts + kSumParts);
+    sum_floats_kernel<<<sum_floats_blocks(nv), kSumThreads, 0, s>>>(in.l2_part, nv, parts, ticket, loss);
+  }
 }
 
 // Adjoint: zero-fills the gradients, then accumulates them pass by pass.
@@ -1206,7 +1215,7 @@ int msplat_rasterize_views_forward(const float *uv, const float *conic, const fl
 
 size_t msplat_rasterize_l2_workspace_bytes(int W, int H, int C) {
   if (!valid_image(W, H) || C < 0) return 0;
-  return size_t(l2_slots(int64_t(tiles_x(W)) * tiles_y(H)) * l2_passes(C)) * sizeof(float);
+  return l2_workspace(int64_t(tiles_x(W)) * tiles_y(H), C);
 }
 
 int msplat_rasterize_forward_l2(const float *uv, const float *conic, const float *opacity, const float *feature,
