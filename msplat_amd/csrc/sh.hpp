@@ -412,20 +412,37 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
                                                         const float *__restrict__ dval, int ld,
                                                         float *__restrict__ dshs) {
   constexpr int D = (L + 1) * (L + 1);
+  // The wave's rows are contiguous in dL/dshs: for D a multiple of 4 (up to
+  // 16) they are staged in LDS and stored a 16-byte piece per lane, so each
+  // store instruction writes 1 KB in one run instead of 64 pieces 4 D bytes
+  // apart (the kernel is bound by these writes).
+  constexpr bool kStage = D % 4 == 0 && D <= 16;
+  constexpr int kQ = kStage ? D / 4 : 1;
+  __shared__ float4 stage[kStage ? 256 / kWave : 1][kStage ? kWave * kQ : 1];
   const int G = kWave / C;  // Gaussians per wave
   const int lane = threadIdx.x % kWave;
+  const int wv = threadIdx.x / kWave;
   const int64_t wave = (int64_t(blockIdx.x) * 256 + threadIdx.x) / kWave;
   const int j = lane / C, c = lane - j * C;
   const int64_t i = wave * G + j;
   const bool active = j < G && i < n;
   float gx = 0.f, gy = 0.f, gz = 0.f;
+  auto put_row = [&](int64_t t, const float (&r)[D]) {
+    if constexpr (kStage) {
+#pragma unroll
+      for (int q = 0; q < kQ; ++q)
+        stage[wv][lane * kQ + q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+    } else {
+      sh_store_row<D>(dshs + t * D, r);
+    }
+  };
   if (active) {
     const int64_t t = i * C + c;
     if (vis && !vis[i]) {
       float z[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) z[k] = 0.f;
-      sh_store_row<D>(dshs + t * D, z);
+      put_row(t, z);
     } else {
       float x, y, zz;
       dir.load(i, x, y, zz);
@@ -448,7 +465,20 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
         gy += w * dy;
         gz += w * dz;
       });
-      sh_store_row<D>(dshs + t * D, Y);
+      put_row(t, Y);
+    }
+  }
+  if constexpr (kStage) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t row0 = wave * G * C;  // the wave's first row
+    const int64_t rows = n * C - row0 < int64_t(G * C) ? n * C - row0 : int64_t(G * C);
+    float4 *dst = reinterpret_cast<float4 *>(dshs) + row0 * kQ;
+#pragma unroll
+    for (int k = 0; k < kQ; ++k) {
+      const int idx = lane + kWave * k;
+      if (idx < rows * kQ) dst[idx] = stage[wv][idx];
     }
   }
   // channel partials of one Gaussian sit in lanes j*C .. j*C + C - 1
