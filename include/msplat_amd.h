@@ -116,15 +116,18 @@ int msplat_preprocess_backward(const float *xyz, const float *scale,
 
 /* The same two, for a depth that is also a feature channel (no reference
  * counterpart; rasterization_sh's depth channel): the forward also writes
- * each depth to depth_col[i * ld_col] (NULL: nothing), the adjoint reads
- * dL_ddepth[i * ld_ddepth], so neither needs a strided copy. */
+ * each depth to depth_col[i * ld_col] and visible[i] = radius[i] > 0 (a
+ * uint8 mask, e.g. for msplat_compute_sh_points_forward; each NULL:
+ * nothing), the adjoint reads dL_ddepth[i * ld_ddepth], so neither needs a
+ * strided copy. */
 int msplat_preprocess_forward_col(const float *xyz, const float *scale,
                                   const float *rotate, int64_t n,
                                   const float *intr, const float *extr, int W,
                                   int H, float nearest, float extent,
                                   float *uv, float *depth, float *conic,
                                   int32_t *radius, int32_t *tiles,
-                                  float *depth_col, int ld_col, void *stream);
+                                  float *depth_col, int ld_col,
+                                  uint8_t *visible, void *stream);
 int msplat_preprocess_backward_ld(const float *xyz, const float *scale,
                                   const float *rotate, int64_t n,
                                   const float *intr, const float *extr,

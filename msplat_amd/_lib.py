@@ -36,7 +36,7 @@ SIGNATURES = {
     "msplat_ewa_project_backward": (I32, [P, P, P, P, P, P, I64, P, P, P, P, P]),
     "msplat_preprocess_forward": (I32, [P, P, P, I64, P, P, I32, I32, F32, F32, P, P, P, P, P, P]),
     "msplat_preprocess_backward": (I32, [P, P, P, I64, P, P, P, P, P, P, P, P, P, P, P, P, P]),
-    "msplat_preprocess_forward_col": (I32, [P, P, P, I64, P, P, I32, I32, F32, F32, P, P, P, P, P, P, I32, P]),
+    "msplat_preprocess_forward_col": (I32, [P, P, P, I64, P, P, I32, I32, F32, F32, P, P, P, P, P, P, I32, P, P]),
     "msplat_preprocess_backward_ld": (I32, [P, P, P, I64, P, P, P, P, P, P, I32, P, P, P, P, P, P, P]),
     "msplat_compute_sh_forward": (I32, [P, P, P, I64, I32, I32, P, P]),
     "msplat_compute_sh_backward": (I32, [P, P, P, P, I64, I32, I32, P, P, P]),

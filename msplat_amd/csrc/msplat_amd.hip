@@ -747,7 +747,7 @@ int msplat_ewa_project_backward(const float *xyz, const float *cov3d, const floa
 int msplat_preprocess_forward_col(const float *xyz, const float *scale, const float *rotate, int64_t n,
                                   const float *intr, const float *extr, int W, int H, float nearest, float extent,
                                   float *uv, float *depth, float *conic, int32_t *radius, int32_t *tiles,
-                                  float *depth_col, int ld_col, void *stream) {
+                                  float *depth_col, int ld_col, uint8_t *visible, void *stream) {
   REQUIRE(n >= 0, "preprocess_forward: n = %lld", (long long)n);
   REQUIRE(valid_image(W, H), "preprocess_forward: bad image size %dx%d", W, H);
   REQUIRE(!depth_col || ld_col >= 1, "preprocess_forward: depth column stride %d", ld_col);
@@ -757,7 +757,7 @@ int msplat_preprocess_forward_col(const float *xyz, const float *scale, const fl
   Timed timed_("preprocess_fwd", as_stream(stream));
   preprocess_fwd_kernel<<<grid_1d(n, kGeomThreads), kGeomThreads, 0, as_stream(stream)>>>(
       xyz, scale, rotate, n, intr, extr, W, H, tiles_x(W), tiles_y(H), nearest, extent, uv, depth, conic, radius,
-      tiles, depth_col, ld_col);
+      tiles, depth_col, ld_col, visible);
   return launched("preprocess_forward");
 }
 
@@ -766,7 +766,7 @@ int msplat_preprocess_forward(const float *xyz, const float *scale, const float 
                               float *uv, float *depth, float *conic, int32_t *radius, int32_t *tiles,
                               void *stream) {
   return msplat_preprocess_forward_col(xyz, scale, rotate, n, intr, extr, W, H, nearest, extent, uv, depth, conic,
-                                       radius, tiles, nullptr, 1, stream);
+                                       radius, tiles, nullptr, 1, nullptr, stream);
 }
 
 int msplat_preprocess_backward_ld(const float *xyz, const float *scale, const float *rotate, int64_t n,

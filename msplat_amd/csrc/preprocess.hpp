@@ -480,7 +480,8 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_fwd_kernel(
     const float *__restrict__ xyz, const float *__restrict__ scale, const float *__restrict__ rot, int64_t n,
     const float *__restrict__ intr, const float *__restrict__ extr, int W, int H, int gx, int gy,
     float nearest, float extent, float *__restrict__ uv, float *__restrict__ depth, float *__restrict__ conic,
-    int32_t *__restrict__ radius, int32_t *__restrict__ tiles, float *__restrict__ depth_col, int ld_col) {
+    int32_t *__restrict__ radius, int32_t *__restrict__ tiles, float *__restrict__ depth_col, int ld_col,
+    uint8_t *__restrict__ visible) {
   const int64_t i = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
   if (i >= n) return;
   const Camera c = load_camera(intr, extr);
@@ -508,6 +509,7 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_fwd_kernel(
   }
   st3(conic, i, co.x, co.y, co.z);
   radius[i] = r;
+  if (visible) visible[i] = r > 0 ? 1 : 0;  // projected: in some tile
   tiles[i] = nt;
 }
 

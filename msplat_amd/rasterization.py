@@ -30,10 +30,11 @@ def _column(t, n: int):
 
 
 def preprocess_forward(xyz, scale, rotate, intr, extr, W: int, H: int, nearest: float = 0.0,
-                       extent: float = 1.3, depth_col=None):
+                       extent: float = 1.3, depth_col=None, visible=None):
     """Fused projection + covariance + EWA: ``(uv, depth, conic, radius, tiles)``.
     ``depth_col`` (a float32 device column, e.g. a feature channel) also
-    receives the depths."""
+    receives the depths; ``visible`` (a uint8 [n] device tensor) receives
+    radius > 0."""
     x = _lib.f32("xyz", xyz)
     n = x.shape[0]
     dev = x.device
@@ -48,11 +49,15 @@ def preprocess_forward(xyz, scale, rotate, intr, extr, W: int, H: int, nearest: 
         if c is None:
             raise RuntimeError("preprocess_forward: depth_col must be a float32 device column of n elements")
         col, ld = c
+    if visible is not None and (visible.dtype != torch.uint8 or not visible.is_cuda or visible.numel() != n
+                                or not visible.is_contiguous()):
+        raise RuntimeError("preprocess_forward: visible must be a contiguous uint8 device tensor of n elements")
     with torch.cuda.device(dev):
         _lib.call("msplat_preprocess_forward_col", _lib.ptr(x), _lib.ptr(_lib.f32("scale", scale)),
                   _lib.ptr(_lib.f32("rotate", rotate)), n, _lib.ptr(_lib.f32("intr", intr)),
                   _lib.ptr(_lib.f32("extr", extr)), int(W), int(H), float(nearest), float(extent), _lib.ptr(uv),
-                  _lib.ptr(depth), _lib.ptr(conic), _lib.ptr(radius), _lib.ptr(tiles), col, ld, _lib.stream_ptr())
+                  _lib.ptr(depth), _lib.ptr(conic), _lib.ptr(radius), _lib.ptr(tiles), col, ld, _lib.ptr(visible),
+                  _lib.stream_ptr())
     return uv, depth, conic, radius, tiles
 
 

@@ -76,15 +76,16 @@ def _forward(ctx, xyz, scale, rotate, opacity, shs, intr, extr, campos, W, H, bg
     dev = x.device
     feat = torch.empty((n, C), dtype=torch.float32, device=dev)
     cfg = tuning.config_for(W, H, C)
+    vis = torch.empty(n, dtype=torch.uint8, device=dev)  # radius > 0, written by the preprocessing
     with torch.cuda.device(dev):
         uv, depth, conic, radius, _ = preprocess_forward(xyz, scale, rotate, intr, extr, W, H,
-                                                         depth_col=feat[:, C - 1] if depth_channel else None)
+                                                         depth_col=feat[:, C - 1] if depth_channel else None,
+                                                         visible=vis)
         if E:
             feat[:, Csh:Csh + E].copy_(extra)
         # SH colours into feature columns [0, Csh) for the projected Gaussians
         # only (the others are in no cell list: their rows get zeros and their
         # coefficients are not read), queued behind the binning's depth sort
-        vis = (radius > 0).view(torch.uint8)
 
         def sh_colours():
             _lib.call("msplat_compute_sh_points_forward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), _lib.ptr(vis), n,
