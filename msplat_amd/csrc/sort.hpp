@@ -27,6 +27,11 @@ namespace sortk {
 constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kRadixTile = kThreads * kItems;  // keys per radix workgroup
+// The scatter pass runs a tile as 16 waves of 4 keys per lane: with few
+// tiles per pass (a few hundred thousand keys) a pass is a chain of
+// latencies, and 16 short waves rank a tile in less time than 4 long ones.
+constexpr int kScatterThreads = 1024;
+constexpr int kScatterItems = kRadixTile / kScatterThreads;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kThreads * kScanItems;  // elements per scan workgroup
 
@@ -62,17 +67,18 @@ MSPLAT_DEV int64_t wave_incl_scan(int64_t v) {
   return v;
 }
 
-// Exclusive scan of one value per thread over the workgroup; returns the
-// workgroup total through *total.
+// Exclusive scan of one value per thread over a workgroup of NT threads;
+// returns the workgroup total through *total.
+template <int NT = kThreads>
 MSPLAT_DEV int64_t block_excl_scan(int64_t v, int64_t *total) {
-  __shared__ int64_t wsum[kThreads / kWave];
+  __shared__ int64_t wsum[NT / kWave];
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const int64_t incl = wave_incl_scan(v);
   if (lane == kWave - 1) wsum[wave] = incl;
   __syncthreads();
   int64_t before = 0, all = 0;
 #pragma unroll
-  for (int w = 0; w < kThreads / kWave; ++w) {
+  for (int w = 0; w < NT / kWave; ++w) {
     before += (w < wave) ? wsum[w] : 0;
     all += wsum[w];
   }
@@ -219,7 +225,7 @@ constexpr int64_t kLookbackSpins = int64_t(1) << 16;
 constexpr int kLookbackWidth = 8;
 
 template <bool kOnesweep>
-__global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
+__global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t cap,
     const int64_t *__restrict__ dlen, int shift, int bits, const uint32_t *__restrict__ offsets, int64_t ntiles,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t *__restrict__ ticket,
@@ -237,8 +243,8 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     __syncthreads();
     tile = s_ticket;
   }
-  constexpr int kWaves = kThreads / kWave;
-  constexpr int kPerWave = kRadixTile / kWaves;  // 1024 keys, contiguous
+  constexpr int kWaves = kScatterThreads / kWave;
+  constexpr int kPerWave = kRadixTile / kWaves;  // 256 keys, contiguous
   __shared__ uint32_t s_key[kRadixTile];
   __shared__ uint32_t s_val[kRadixTile];
   __shared__ uint32_t s_run[kWaves][256];  // per-wave running digit counts, then wave offsets
@@ -247,18 +253,17 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
   const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const uint32_t mask = (1u << bits) - 1u;
   const int ndig = 1 << bits;
-#pragma unroll
-  for (int w = 0; w < kWaves; ++w) s_run[w][tid] = 0u;
+  for (int j = tid; j < kWaves * 256; j += kScatterThreads) s_run[j / 256][j % 256] = 0u;
   if (!kOnesweep && tid < ndig) s_gstart[tid] = offsets[int64_t(tid) * ntiles + tile];
   __syncthreads();
 
   const int64_t base = tile * kRadixTile + int64_t(wave) * kPerWave;
-  uint32_t key[kItems], val[kItems], rnk[kItems];
+  uint32_t key[kScatterItems], val[kScatterItems], rnk[kScatterItems];
   // All loads first: the ranking rounds below are separated by LDS fences,
   // which would otherwise hold each round's load until the previous round's
-  // LDS traffic completes (kItems memory latencies in a row).
+  // LDS traffic completes (one memory latency per round, in a row).
 #pragma unroll
-  for (int it = 0; it < kItems; ++it) {
+  for (int it = 0; it < kScatterItems; ++it) {
     const int64_t i = base + it * kWave + lane;
     key[it] = i < len ? kin[i] : 0u;
     val[it] = i < len ? vin[i] : 0u;
@@ -272,10 +277,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
   // operations execute in program order and each digit has one leader per
   // round, so ranks follow key order (stable), with no fence or wait between
   // rounds.
-  uint32_t dig[kItems];
-  uint64_t grp[kItems];
+  uint32_t dig[kScatterItems];
+  uint64_t grp[kScatterItems];
 #pragma unroll
-  for (int it = 0; it < kItems; ++it) {
+  for (int it = 0; it < kScatterItems; ++it) {
     const bool valid = base + it * kWave + lane < len;
     const uint32_t d = (key[it] >> shift) & mask;
     uint64_t peers = __ballot(valid);
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
     grp[it] = valid ? peers : 0ull;
   }
 #pragma unroll
-  for (int it = 0; it < kItems; ++it) {
+  for (int it = 0; it < kScatterItems; ++it) {
     const uint64_t peers = grp[it];
     const uint32_t below =
         __builtin_amdgcn_mbcnt_hi(uint32_t(peers >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(peers), 0u));
@@ -312,7 +317,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
   if constexpr (kOnesweep) {
     // this tile's first slot per digit = digits below it in the whole pass
     // (offsets holds the pass's digit totals) + this digit in earlier tiles
-    const uint32_t doff = uint32_t(block_excl_scan(tid < ndig ? int64_t(offsets[tid]) : 0, &dummy));
+    const uint32_t doff = uint32_t(block_excl_scan<kScatterThreads>(tid < ndig ? int64_t(offsets[tid]) : 0, &dummy));
     if (tid < ndig) {
       uint32_t *st = status + tile * 256;
       uint32_t prefix = 0;
@@ -354,7 +359,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
       s_gstart[tid] = doff + prefix;
     }
   }
-  const uint32_t lstart = uint32_t(block_excl_scan(tid < ndig ? int64_t(tot) : 0, &dummy));
+  const uint32_t lstart = uint32_t(block_excl_scan<kScatterThreads>(tid < ndig ? int64_t(tot) : 0, &dummy));
   if (tid < ndig) {
     s_lstart[tid] = lstart;
     uint32_t off = lstart;
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
   }
   __syncthreads();
 #pragma unroll
-  for (int it = 0; it < kItems; ++it) {
+  for (int it = 0; it < kScatterItems; ++it) {
     if (rnk[it] != 0xffffffffu) {
       const uint32_t d = (key[it] >> shift) & mask;
       const uint32_t slot = s_run[wave][d] + rnk[it];
@@ -378,7 +383,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_tile_kernel(
   __syncthreads();
   const int64_t tbase = tile * kRadixTile;
   const int cnt = len - tbase < kRadixTile ? int(len - tbase) : kRadixTile;
-  for (int e = tid; e < cnt; e += kThreads) {
+  for (int e = tid; e < cnt; e += kScatterThreads) {
     const uint32_t k = s_key[e];
     const uint32_t d = (k >> shift) & mask;
     const uint32_t pos = s_gstart[d] + (uint32_t(e) - s_lstart[d]);
@@ -447,7 +452,7 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
   radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, cap, dlen, shift, bits, rb.hist, nt);
   const int64_t hlen = int64_t(1 << bits) * nt;
   exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
-  radix_scatter_tile_kernel<false><<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(
+  radix_scatter_tile_kernel<false><<<dim3(unsigned(nt)), dim3(kScatterThreads), 0, s>>>(
       kin, vin, cap, dlen, shift, bits, rb.hist, nt, kout, vout, nullptr, nullptr, nullptr, gather);
 }
 
@@ -481,7 +486,7 @@ inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, in
     const RadixGather g = p == npass - 1 ? gather : RadixGather{};
     if (nt > 0) {
       if (onesweep)
-        radix_scatter_tile_kernel<true><<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(
+        radix_scatter_tile_kernel<true><<<dim3(unsigned(nt)), dim3(kScatterThreads), 0, s>>>(
             sk, sv, cap, dlen, shift, pb, rb.ghist + p * 256, nt, dk, dv, rb.ghist + kMaxPasses * 256 + p,
             rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses, g);
       else
