@@ -170,6 +170,20 @@ int msplat_compute_sh_points_backward(const float *shs, const float *xyz,
                                       const float *dL_dvalue, int ld_dvalue,
                                       int64_t n, int C, int D, float *dL_dshs,
                                       float *dL_dxyz, void *stream);
+/* As msplat_compute_sh_points_backward, but the xyz gradient is added into
+ * dL_dxyz (one fp32 addition per element, zeros included: the same values as
+ * storing it and adding the two arrays; with C = 0 nothing is added).  No
+ * reference counterpart: autograd's sum of the xyz gradients that reach a
+ * training step's xyz through compute_sh (msplat/compute_sh.py:47) and
+ * through project_point (msplat/project_point.py:71), done inside the SH
+ * adjoint. */
+int msplat_compute_sh_points_backward_add(const float *shs, const float *xyz,
+                                          const float *campos,
+                                          const uint8_t *visible,
+                                          const float *dL_dvalue, int ld_dvalue,
+                                          int64_t n, int C, int D,
+                                          float *dL_dshs, float *dL_dxyz,
+                                          void *stream);
 
 /* ---- kernel timing (no reference counterpart; for benchmarks).  While
  * enabled, the main kernel of each entry point is bracketed by HIP events on

@@ -42,6 +42,7 @@ SIGNATURES = {
     "msplat_compute_sh_backward": (I32, [P, P, P, P, I64, I32, I32, P, P, P]),
     "msplat_compute_sh_points_forward": (I32, [P, P, P, P, I64, I32, I32, P, I32, P]),
     "msplat_compute_sh_points_backward": (I32, [P, P, P, P, P, I32, I64, I32, I32, P, P, P]),
+    "msplat_compute_sh_points_backward_add": (I32, [P, P, P, P, P, I32, I64, I32, I32, P, P, P]),
     "msplat_sort_prepare_workspace_bytes": (SZ, [I64, I32, I32]),
     "msplat_sort_prepare": (I32, [P, P, P, P, I64, I32, I32, P, SZ, P, P]),
     "msplat_sort_emit_workspace_bytes": (SZ, [I64, I64, I32, I32]),

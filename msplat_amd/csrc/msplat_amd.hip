@@ -855,25 +855,45 @@ int msplat_compute_sh_points_forward(const float *shs, const float *xyz, const f
   return launched("compute_sh_points_forward");
 }
 
-int msplat_compute_sh_points_backward(const float *shs, const float *xyz, const float *campos,
-                                      const uint8_t *visible, const float *dL_dvalue, int ld_dvalue, int64_t n,
-                                      int C, int D, float *dL_dshs, float *dL_dxyz, void *stream) {
-  REQUIRE(n >= 0 && C >= 0, "compute_sh_points_backward: n = %lld, C = %d", (long long)n, C);
+}  // extern "C"
+
+namespace {
+int compute_sh_points_backward_impl(const char *name, const float *shs, const float *xyz, const float *campos,
+                                    const uint8_t *visible, const float *dL_dvalue, int ld_dvalue, int64_t n, int C,
+                                    int D, float *dL_dshs, float *dL_dxyz, bool add, void *stream) {
+  REQUIRE(n >= 0 && C >= 0, "%s: n = %lld, C = %d", name, (long long)n, C);
   const int L = sh_degree(D);
-  REQUIRE(L >= 0, "compute_sh_points_backward: D = %d is not (deg+1)^2 with deg <= 10", D);
+  REQUIRE(L >= 0, "%s: D = %d is not (deg+1)^2 with deg <= 10", name, D);
   if (n == 0) return MSPLAT_OK;
   hipStream_t s = as_stream(stream);
   if (C == 0) {
-    launch_fill_zero(dL_dxyz, 3 * n, s);
-    return launched("compute_sh_points_backward");
+    if (!add) launch_fill_zero(dL_dxyz, 3 * n, s);
+    return launched(name);
   }
-  REQUIRE(shs && xyz && campos && dL_dvalue && dL_dshs && dL_dxyz, "compute_sh_points_backward: null pointer");
-  REQUIRE(ld_dvalue >= C, "compute_sh_points_backward: ld_dvalue = %d < C = %d", ld_dvalue, C);
-  const ShDirFromPoints dir{xyz, campos, dL_dxyz};
+  REQUIRE(shs && xyz && campos && dL_dvalue && dL_dshs && dL_dxyz, "%s: null pointer", name);
+  REQUIRE(ld_dvalue >= C, "%s: ld_dvalue = %d < C = %d", name, ld_dvalue, C);
+  const ShDirFromPoints dir{xyz, campos, dL_dxyz, add};
 #define CALL(LL) sh_bwd_launch<LL>(n, C, shs, dir, visible, dL_dvalue, ld_dvalue, dL_dshs, s)
   MSPLAT_SH_DISPATCH(L, CALL)
 #undef CALL
-  return launched("compute_sh_points_backward");
+  return launched(name);
+}
+}  // namespace
+
+extern "C" {
+
+int msplat_compute_sh_points_backward(const float *shs, const float *xyz, const float *campos,
+                                      const uint8_t *visible, const float *dL_dvalue, int ld_dvalue, int64_t n,
+                                      int C, int D, float *dL_dshs, float *dL_dxyz, void *stream) {
+  return compute_sh_points_backward_impl("compute_sh_points_backward", shs, xyz, campos, visible, dL_dvalue,
+                                         ld_dvalue, n, C, D, dL_dshs, dL_dxyz, false, stream);
+}
+
+int msplat_compute_sh_points_backward_add(const float *shs, const float *xyz, const float *campos,
+                                          const uint8_t *visible, const float *dL_dvalue, int ld_dvalue, int64_t n,
+                                          int C, int D, float *dL_dshs, float *dL_dxyz, void *stream) {
+  return compute_sh_points_backward_impl("compute_sh_points_backward_add", shs, xyz, campos, visible, dL_dvalue,
+                                         ld_dvalue, n, C, D, dL_dshs, dL_dxyz, true, stream);
 }
 
 // ---------------------------------------------------------------- sort

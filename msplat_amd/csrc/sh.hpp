@@ -192,6 +192,7 @@ struct ShDirFromPoints {
   const float *xyz;
   const float *campos;  // [3] in device memory
   float *grad;
+  bool add = false;  // add the xyz gradient into grad (one fp32 addition per element) instead of storing it
   MSPLAT_DEV float offset(int64_t i, float &vx, float &vy, float &vz) const {
     vx = xyz[3 * i] - campos[0];
     vy = xyz[3 * i + 1] - campos[1];
@@ -210,14 +211,18 @@ struct ShDirFromPoints {
     const float r = offset(i, vx, vy, vz);
     const float x = vx / r, y = vy / r, z = vz / r;
     const float dot = x * gx + y * gy + z * gz;
-    grad[3 * i] = (gx - x * dot) / r;
-    grad[3 * i + 1] = (gy - y * dot) / r;
-    grad[3 * i + 2] = (gz - z * dot) / r;
+    put(i, (gx - x * dot) / r, (gy - y * dot) / r, (gz - z * dot) / r);
   }
-  MSPLAT_DEV void store_zero(int64_t i) const {
-    grad[3 * i] = 0.f;
-    grad[3 * i + 1] = 0.f;
-    grad[3 * i + 2] = 0.f;
+  MSPLAT_DEV void store_zero(int64_t i) const { put(i, 0.f, 0.f, 0.f); }
+  MSPLAT_DEV void put(int64_t i, float gx, float gy, float gz) const {
+    if (add) {  // uniform; a zero is added too (-0 + 0 = +0, as a separate addition gives)
+      gx = grad[3 * i] + gx;
+      gy = grad[3 * i + 1] + gy;
+      gz = grad[3 * i + 2] + gz;
+    }
+    grad[3 * i] = gx;
+    grad[3 * i + 1] = gy;
+    grad[3 * i + 2] = gz;
   }
 };
 

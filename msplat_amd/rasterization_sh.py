@@ -14,8 +14,11 @@ in how it runs:
   enter a cell list); it is queued behind the binning's depth sort, so the
   GPU runs it while the host waits for the entry count (beside the sort's
   latency-bound kernels, a bandwidth-bound kernel over every Gaussian slowed
-  them more than it saved); the adjoint runs the SH adjoint on a side
-  stream beside the geometry adjoint;
+  them more than it saved); the SH adjoint runs after the geometry adjoint
+  and adds its xyz gradient into the geometry's (beside it, on a side
+  stream, both contending for bandwidth, plus a separate addition, the pair
+  took longer); only a campos gradient, the negated sum of the SH part
+  alone, keeps the side-stream form;
 * the depth channel is the fused preprocessing's own depth (no second
   projection), and its gradient enters the geometry adjoint directly;
 * one autograd node instead of four.
@@ -122,27 +125,22 @@ def _backward(ctx, dL_dout, dL_dloss=None):
         duv, dconic, dopac, dfeat = rasterize_backward(uv, conic, opacity, feat, bins, ctx.bg, ctx.W, ctx.H,
                                                        final_T, ncontrib, walk, dL_dout, ctx.ppl_bwd, ctx.cull,
                                                        l2=l2)
-        # SH adjoint (feature columns [0, Csh)) on the side stream, next to the geometry adjoint
         x = _lib.f32("xyz", xyz)
         dshs = torch.empty((n, Csh, D), dtype=torch.float32, device=dev)
-        dxyz_sh = torch.empty((n, 3), dtype=torch.float32, device=dev)
-        ready = torch.cuda.Event()
-        ready.record(main)
-        side.wait_event(ready)
-        with torch.cuda.stream(side):
-            _lib.call("msplat_compute_sh_points_backward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None,
-                      _lib.ptr(dfeat), C, n, Csh, D, _lib.ptr(dshs), _lib.ptr(dxyz_sh), side.cuda_stream)
-            sh_done = torch.cuda.Event()
-            sh_done.record(side)
-        for t in (s, x, cp, dfeat, dshs, dxyz_sh):
-            t.record_stream(side)
-        dxyz, dscale, drot, dintr, dextr = preprocess_backward(
-            xyz, scale, rotate, intr, extr, depth, radius, duv, dconic,
-            dL_ddepth=dfeat[:, C - 1] if depth_channel else None,
-            want_intr=intr.requires_grad, want_extr=extr.requires_grad)
-        main.wait_event(sh_done)
-        dxyz.add_(dxyz_sh)
-        dcampos = (-dxyz_sh.sum(0)).view(ctx.campos_shape) if ctx.needs_input_grad[7] else None
+        if Csh > 0 and not ctx.needs_input_grad[7]:
+            # the geometry adjoint, then the SH adjoint (feature columns [0, Csh))
+            # adding its xyz gradient into the geometry's: no gradient array or
+            # addition of its own
+            dxyz, dscale, drot, dintr, dextr = preprocess_backward(
+                xyz, scale, rotate, intr, extr, depth, radius, duv, dconic,
+                dL_ddepth=dfeat[:, C - 1] if depth_channel else None,
+                want_intr=intr.requires_grad, want_extr=extr.requires_grad)
+            _lib.call("msplat_compute_sh_points_backward_add", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None,
+                      _lib.ptr(dfeat), C, n, Csh, D, _lib.ptr(dshs), _lib.ptr(dxyz), main.cuda_stream)
+            dcampos = None
+        else:
+            dxyz, dscale, drot, dintr, dextr, dcampos = _geometry_and_sh_adjoints(
+                ctx, main, side, xyz, scale, rotate, intr, extr, depth, radius, duv, dconic, dfeat, s, x, cp, dshs)
     dextra = dfeat[:, Csh:Csh + E] if (E and ctx.needs_input_grad[11]) else None
     dndc = None
     if ctx.has_ndc:
@@ -150,6 +148,34 @@ def _backward(ctx, dL_dout, dL_dloss=None):
     return (dxyz, dscale, drot, dopac.view_as(opacity), dshs, dintr,
             camera_grad_shape(extr, dextr) if dextr is not None else None, dcampos, None, None, None,
             dextra, None, dndc)
+
+
+def _geometry_and_sh_adjoints(ctx, main, side, xyz, scale, rotate, intr, extr, depth, radius, duv, dconic, dfeat, s,
+                              x, cp, dshs):
+    """The SH adjoint on a side stream beside the geometry adjoint, with its own
+    xyz gradient (the campos gradient is its negated sum)."""
+    Csh, D, E, C, depth_channel = ctx.dims
+    n = x.shape[0]
+    dev = x.device
+    dxyz_sh = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    ready = torch.cuda.Event()
+    ready.record(main)
+    side.wait_event(ready)
+    with torch.cuda.stream(side):
+        _lib.call("msplat_compute_sh_points_backward", _lib.ptr(s), _lib.ptr(x), _lib.ptr(cp), None,
+                  _lib.ptr(dfeat), C, n, Csh, D, _lib.ptr(dshs), _lib.ptr(dxyz_sh), side.cuda_stream)
+        sh_done = torch.cuda.Event()
+        sh_done.record(side)
+    for t in (s, x, cp, dfeat, dshs, dxyz_sh):
+        t.record_stream(side)
+    dxyz, dscale, drot, dintr, dextr = preprocess_backward(
+        xyz, scale, rotate, intr, extr, depth, radius, duv, dconic,
+        dL_ddepth=dfeat[:, C - 1] if depth_channel else None,
+        want_intr=intr.requires_grad, want_extr=extr.requires_grad)
+    main.wait_event(sh_done)
+    dxyz.add_(dxyz_sh)
+    dcampos = (-dxyz_sh.sum(0)).view(ctx.campos_shape) if ctx.needs_input_grad[7] else None
+    return dxyz, dscale, drot, dintr, dextr, dcampos
 
 
 class _RasterizeSH(torch.autograd.Function):
