@@ -27,11 +27,13 @@ namespace sortk {
 constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kRadixTile = kThreads * kItems;  // keys per radix workgroup
-// The scatter pass runs a tile as 16 waves of 4 keys per lane: with few
-// tiles per pass (a few hundred thousand keys) a pass is a chain of
-// latencies, and 16 short waves rank a tile in less time than 4 long ones.
-constexpr int kScatterThreads = 1024;
-constexpr int kScatterItems = kRadixTile / kScatterThreads;
+// A scatter pass over few tiles (a few hundred thousand keys: less than
+// the chip holds at once) is a chain of latencies, and 16 short waves of 4
+// keys per lane rank a tile sooner than 4 waves of 16.  Over many tiles the
+// pass is bound by throughput, and the 4-wave form does less LDS work per
+// key (per-wave digit counts for 4 waves instead of 16).
+constexpr int kScatterWide = 1024, kScatterNarrow = kThreads;
+constexpr int64_t kScatterWideTiles = 512;  // at most this many tiles: the 16-wave form
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kThreads * kScanItems;  // elements per scan workgroup
 
@@ -224,7 +226,7 @@ constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kValueMask = (1u <
 constexpr int64_t kLookbackSpins = int64_t(1) << 16;
 constexpr int kLookbackWidth = 8;
 
-template <bool kOnesweep>
+template <bool kOnesweep, int kScatterThreads>
 __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t cap,
     const int64_t *__restrict__ dlen, int shift, int bits, const uint32_t *__restrict__ offsets, int64_t ntiles,
@@ -243,8 +245,9 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
     __syncthreads();
     tile = s_ticket;
   }
+  constexpr int kScatterItems = kRadixTile / kScatterThreads;  // keys per lane
   constexpr int kWaves = kScatterThreads / kWave;
-  constexpr int kPerWave = kRadixTile / kWaves;  // 256 keys, contiguous
+  constexpr int kPerWave = kRadixTile / kWaves;  // contiguous keys per wave
   __shared__ uint32_t s_key[kRadixTile];
   __shared__ uint32_t s_val[kRadixTile];
   __shared__ uint32_t s_run[kWaves][256];  // per-wave running digit counts, then wave offsets
@@ -452,7 +455,7 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
   radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, cap, dlen, shift, bits, rb.hist, nt);
   const int64_t hlen = int64_t(1 << bits) * nt;
   exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
-  radix_scatter_tile_kernel<false><<<dim3(unsigned(nt)), dim3(kScatterThreads), 0, s>>>(
+  radix_scatter_tile_kernel<false, kScatterNarrow><<<dim3(unsigned(nt)), dim3(kScatterNarrow), 0, s>>>(
       kin, vin, cap, dlen, shift, bits, rb.hist, nt, kout, vout, nullptr, nullptr, nullptr, gather);
 }
 
@@ -485,8 +488,12 @@ inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, in
     uint32_t *dv = (p == npass - 1 && vlast) ? vlast : (in_b ? va : vb);
     const RadixGather g = p == npass - 1 ? gather : RadixGather{};
     if (nt > 0) {
-      if (onesweep)
-        radix_scatter_tile_kernel<true><<<dim3(unsigned(nt)), dim3(kScatterThreads), 0, s>>>(
+      if (onesweep && nt <= kScatterWideTiles)
+        radix_scatter_tile_kernel<true, kScatterWide><<<dim3(unsigned(nt)), dim3(kScatterWide), 0, s>>>(
+            sk, sv, cap, dlen, shift, pb, rb.ghist + p * 256, nt, dk, dv, rb.ghist + kMaxPasses * 256 + p,
+            rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses, g);
+      else if (onesweep)
+        radix_scatter_tile_kernel<true, kScatterNarrow><<<dim3(unsigned(nt)), dim3(kScatterNarrow), 0, s>>>(
             sk, sv, cap, dlen, shift, pb, rb.ghist + p * 256, nt, dk, dv, rb.ghist + kMaxPasses * 256 + p,
             rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses, g);
       else

@@ -46,6 +46,14 @@ def test_bins_bitexact(seed, n, W, H, cell):
     _check_bins(uv, depth, radius, W, H, cell)
 
 
+def test_bins_bitexact_many_tiles():
+    """2.2 M Gaussians: the depth sort's capacity spans more than 512 tiles of
+    4096 keys, so its passes take the 4-wave scatter form (smaller inputs take
+    the 16-wave form, covered above)."""
+    _, uv, depth, _, radius, _ = _projected(7, 2_200_000, 640, 480)
+    _check_bins(uv, depth, radius, 640, 480, 4)
+
+
 def test_bins_ties_and_negative_depth():
     gen = np.random.default_rng(11)
     n, W, H = 3000, 160, 112
