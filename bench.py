@@ -74,7 +74,7 @@ def build_step(cfg, dev, features: str = "fused", fused_loss: bool = True, loss_
             for p in P.values():
                 p.grad = None
             return out
-        if features == "fused":  # SH colours + depth rendered by one op (SH on a side stream)
+        if features == "fused":  # SH colours + depth rendered by one op
             out = ms.rasterization_sh(xyz, P["scale"], P["rotate"], P["opacity"], P["shs"], cam.intr, cam.extr,
                                       cam.eye, W, H, 0.0, depth_channel=True)
         else:
@@ -108,7 +108,8 @@ def algorithmic_work(name, w):
         "preprocess_fwd": ("hbm", 72 * n),          # xyz, scale, rot in; uv, depth, conic, radius, tiles out
         "preprocess_bwd": ("hbm", 108 * n),         # + depth, radius, dL/duv, dL/dconic in; 3 grads out
         "sh_fwd": ("hbm", n * (4 * Csh * D + 12 + 4 * Csh)),
-        "sh_bwd": ("hbm", n * (8 * Csh * D + 24 + 4 * Csh)),
+        # shs in, dL/dshs out, xyz in, the geometry's xyz gradient in and out (the SH part added), dL/dvalue in
+        "sh_bwd": ("hbm", n * (8 * Csh * D + 36 + 4 * Csh)),
         # the binned Gaussians only: one key read for all digit histograms, 4 x (pair read + pair write)
         "depth_sort": ("hbm", w["nvis"] * (4 + 4 * 16)),
         "emit": ("hbm", w["count"] * 8 + w["nvis"] * 28),
@@ -526,7 +527,7 @@ def main():
                             gaussians=args.n, width=W, height=H, channels=C, intersections=K,
                             cell_entries=w["count"], blend=dict(cell=cfg_used.cell, ppl_fwd=cfg_used.ppl_fwd,
                                                                 ppl_bwd=cfg_used.ppl_bwd),
-                            features={"fused": "rasterization_sh (SH colours + depth in one op, SH on a side stream)",
+                            features={"fused": "rasterization_sh (SH colours + depth in one op, SH behind the depth sort)",
                                       "ops": "compute_sh_points + project_point + cat + rasterization",
                                       "torch": "torch view directions + compute_sh + project_point + cat + "
                                                "rasterization"}[args.features],

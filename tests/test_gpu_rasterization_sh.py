@@ -114,7 +114,7 @@ def test_l2_fused_matches_separate_loss(other_term, target_grad):
     assert torch.equal(img_a, img_b)
     ref64 = ((img_b.detach().double() - target.double()) ** 2).sum()
     assert abs(float(loss_a.detach()) - float(ref64)) <= 1e-6 * float(ref64)
-    assert abs(float(loss_a) - float(loss_b)) <= 1e-6 * float(ref64)
+    assert abs(float(loss_a.detach()) - float(loss_b.detach())) <= 1e-6 * float(ref64)
     total_a = loss_a * 0.75 + ((img_a * wimg).sum() if other_term else 0.0)
     total_b = loss_b * 0.75 + ((img_b * wimg).sum() if other_term else 0.0)
     total_a.backward()
