@@ -18,7 +18,7 @@ def test_l2_loss_matches_torch(shape):
     t = torch.rand(shape, generator=gen).to(DEV)
     loss = ms.l2_loss(x, t)
     ref64 = ((x.detach().double() - t.double()) ** 2).sum()
-    assert abs(float(loss) - float(ref64)) <= 1e-6 * float(ref64) + 1e-12
+    assert abs(float(loss.detach()) - float(ref64)) <= 1e-6 * float(ref64) + 1e-12
     loss.backward(torch.tensor(0.75, device=DEV))
     xr = x.detach().clone().requires_grad_()
     ((xr - t) ** 2).sum().backward(torch.tensor(0.75, device=DEV))
