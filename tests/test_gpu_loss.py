@@ -23,7 +23,7 @@ def test_l2_loss_matches_torch(shape):
     xr = x.detach().clone().requires_grad_()
     ((xr - t) ** 2).sum().backward(torch.tensor(0.75, device=DEV))
     assert torch.equal(x.grad, xr.grad)
-    assert float(ms.l2_loss(x.detach(), t)) == float(loss)  # the same bits every call
+    assert float(ms.l2_loss(x.detach(), t)) == float(loss.detach())  # the same bits every call
 
 
 def test_l2_loss_misaligned_and_target_grad():
