@@ -134,7 +134,7 @@ struct PrepLayout {
 
 // Workgroups of the binning's single-pass scans over n rows (at least one
 // look-back word is always carved).
-int64_t bin_blocks(int64_t n) { return n > 0 ? (n + sortk::kThreads - 1) / sortk::kThreads : 1; }
+int64_t bin_blocks(int64_t n) { return n > 0 ? (n + sortk::kEmitThreads - 1) / sortk::kEmitThreads : 1; }
 
 // (W, H are part of the size query so that the layout may depend on the
 // tile grid; it currently does not.)
@@ -335,7 +335,7 @@ const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t
   if (direct) {  // cell lists: one thread per listed row, offsets by a single-pass scan
     Timed timed_("emit", s);
     const int64_t rows = n < k ? n : k;  // every listed row has a cell: at most k rows hold slots below k
-    sortk::emit_direct_kernel<<<grid_1d(rows, sortk::kThreads), sortk::kThreads, 0, s>>>(
+    sortk::emit_direct_kernel<<<grid_1d(rows, sortk::kEmitThreads), sortk::kEmitThreads, 0, s>>>(
         order, at<int32_t>(prepared, P.area_s), at<uint2>(prepared, P.rect_s), n, at<int64_t>(prepared, P.nlisted),
         k, dlen, vw, S, ca, ga, at<uint64_t>(ws, L.estatus), bin_blocks(k), sortk::radix_error_word(rb));
   } else {  // equal output chunks per workgroup (tile rectangles of thousands of tiles)

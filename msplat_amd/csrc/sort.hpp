@@ -698,7 +698,11 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
 // (status: one word per workgroup, zero before the launch).  Slots past the
 // capacity are not written; workgroups past nstatus hold no slot below it
 // (every listed row has a cell, so o(i) >= i) and leave at once.
-__global__ __launch_bounds__(kThreads) void emit_direct_kernel(
+// 1024-thread workgroups: a quarter of the look-back chain of 256-thread
+// ones (same single-view step; the 16-view step 0.5 % shorter).
+constexpr int kEmitThreads = 1024;
+
+__global__ __launch_bounds__(kEmitThreads) void emit_direct_kernel(
     const uint32_t *__restrict__ order, const int32_t *__restrict__ area_s, const uint2 *__restrict__ rect_s,
     int64_t n, const int64_t *__restrict__ nrows, int64_t cap, const int64_t *__restrict__ dlen, Views v, int S,
     uint32_t *__restrict__ cls, uint32_t *__restrict__ gid, uint64_t *__restrict__ status, int64_t nstatus,
@@ -708,13 +712,13 @@ __global__ __launch_bounds__(kThreads) void emit_direct_kernel(
   // sort's last pass, so every read here is coalesced).  Listed rows have
   // positive depth, so every entry has a cell class.
   __shared__ int64_t s_base;
-  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t i = int64_t(blockIdx.x) * kEmitThreads + threadIdx.x;
   const int64_t rows = dev_len(nrows, n);  // rows past the listed ones have nothing to emit
-  if (int64_t(blockIdx.x) * kThreads >= rows || int64_t(blockIdx.x) >= nstatus) return;  // uniform
+  if (int64_t(blockIdx.x) * kEmitThreads >= rows || int64_t(blockIdx.x) >= nstatus) return;  // uniform
   const int lane = threadIdx.x % kWave;
   const int a = i < rows ? area_s[i] : 0;
   int64_t tot;
-  const int64_t ex = block_excl_scan(a, &tot);
+  const int64_t ex = block_excl_scan<kEmitThreads>(a, &tot);
   if (threadIdx.x < kWave) {
     int64_t unused, pe;
     const bool ok = lookback_publish(status, blockIdx.x, 0, tot, unused, pe);
