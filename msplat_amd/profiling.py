@@ -160,6 +160,30 @@ def summarize_pmc(fetch_dir, write_dir, mib):
     return "\n".join(lines)
 
 
+def summarize_counters(d):
+    """Per-launch averages of every counter of one PMC run, for the kernels of
+    KERNEL_PATTERNS (counters summed over a dispatch's dimensions first)."""
+    per = {}
+    names = []
+    for r in _rows(d, "counter_collection.csv"):
+        c = r.get("Counter_Name")
+        if c not in names:
+            names.append(c)
+        key = (r["Dispatch_Id"], c)
+        name, v = per.get(key, (r["Kernel_Name"], 0.0))
+        per[key] = (name, v + float(r["Counter_Value"]))
+    lines = [f"{'kernel':16s} {'launches':>8s} " + " ".join(f"{c:>20s}" for c in names)]
+    for k, pat in KERNEL_PATTERNS.items():
+        cols, n = [], 0
+        for c in names:
+            vals = [v for (disp, cn), (name, v) in per.items() if cn == c and _re.search(pat, name)]
+            n = max(n, len(vals))
+            cols.append(sum(vals) / len(vals) if vals else float("nan"))
+        if n:
+            lines.append(f"{k:16s} {n:8d} " + " ".join(f"{v:20.4g}" for v in cols))
+    return "\n".join(lines)
+
+
 if __name__ == "__main__":
     if len(_sys.argv) >= 3 and _sys.argv[1] == "stats":
         print(summarize_stats(_sys.argv[2]))
@@ -167,5 +191,7 @@ if __name__ == "__main__":
         print(summarize_steps(_sys.argv[2], _sys.argv[3]))
     elif len(_sys.argv) >= 5 and _sys.argv[1] == "pmc":
         print(summarize_pmc(_sys.argv[2], _sys.argv[3], int(_sys.argv[4])))
+    elif len(_sys.argv) >= 3 and _sys.argv[1] == "counters":
+        print(summarize_counters(_sys.argv[2]))
     else:
         raise SystemExit(__doc__)
