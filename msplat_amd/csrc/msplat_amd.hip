@@ -249,7 +249,7 @@ void sh_fwd_launch(int64_t n, int C, const float *shs, const Dir &dir, const uin
   Timed timed_("sh_fwd", s);
   if constexpr (L <= kShGcMaxDeg) {
     if (C <= kWave && aligned16(shs)) {
-      sh_fwd_gc_kernel<L, Dir><<<grid_1d(n * C, 256), 256, 0, s>>>(n, C, shs, dir, vis, val, ld);
+      sh_fwd_gc_kernel<L, Dir><<<grid_1d(n * C, 256 * sh_fwd_rows(L)), 256, 0, s>>>(n, C, shs, dir, vis, val, ld);
       return;
     }
   }

@@ -382,30 +382,52 @@ MSPLAT_DEV void sh_store_row(float *__restrict__ row, const float (&r)[D]) {
 }
 
 // value[i, c] = sum_k Y_k(dir_i) shs[i, c, k], k ascending.
+// Rows per thread of sh_fwd_gc_kernel: 2 up to degree 3; higher degrees keep
+// one (two rows of 25+ coefficients would cost occupancy).
+constexpr int sh_fwd_rows(int L) { return L <= 3 ? 2 : 1; }
 template <int L, class Dir>
 __global__ __launch_bounds__(256) void sh_fwd_gc_kernel(int64_t n, int C, const float *__restrict__ shs,
                                                         const Dir dir,
                                                         const uint8_t *__restrict__ vis,
                                                         float *__restrict__ value, int ld) {
   constexpr int D = (L + 1) * (L + 1);
-  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;  // row (i, c) = i * C + c
-  if (t >= n * C) return;
-  const int64_t i = t / C;
-  float *out = value + i * ld + (t - i * C);
-  if (vis && !vis[i]) {
-    *out = 0.f;
-    return;
-  }
-  float dx, dy, dz;
-  dir.load(i, dx, dy, dz);
-  float Y[D];
-  sh_basis<L>(dx, dy, dz, [&](int k, float v, float, float, float) { Y[k] = v; });
-  float r[D];
-  sh_load_row<D>(shs + t * D, r);
-  float acc = 0.f;
+  // R rows per thread, 256 apart: the visibility loads of all of them
+  // first, then all their coefficient loads, so each thread has R dependent
+  // load chains in flight (the kernel is bound by the latency of
+  // visibility -> coefficients more than by bandwidth).
+  constexpr int R = sh_fwd_rows(L);
+  const int64_t rows = n * C;
+  int64_t t[R], i[R];
+  bool drawn[R];
 #pragma unroll
-  for (int k = 0; k < D; ++k) acc += Y[k] * r[k];
-  *out = acc;
+  for (int u = 0; u < R; ++u) {
+    t[u] = (int64_t(blockIdx.x) * R + u) * 256 + threadIdx.x;  // row (i, c) = i * C + c
+    i[u] = t[u] < rows ? t[u] / C : 0;
+    drawn[u] = t[u] < rows && (!vis || vis[i[u]]);
+  }
+  float r[R][D], dx[R], dy[R], dz[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    if (drawn[u]) {
+      dir.load(i[u], dx[u], dy[u], dz[u]);
+      sh_load_row<D>(shs + t[u] * D, r[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    if (t[u] >= rows) continue;
+    float *out = value + i[u] * ld + (t[u] - i[u] * C);
+    if (!drawn[u]) {
+      *out = 0.f;
+      continue;
+    }
+    float Y[D];
+    sh_basis<L>(dx[u], dy[u], dz[u], [&](int k, float v, float, float, float) { Y[k] = v; });
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc += Y[k] * r[u][k];
+    *out = acc;
+  }
 }
 
 // dL/dshs[i,c,k] = Y_k g_c;  dL/ddir = sum_c sum_k (g_c shs[i,c,k]) grad Y_k,
