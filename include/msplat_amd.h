@@ -45,6 +45,23 @@ int msplat_version(void);
  * Replaces the TORCH_CHECK text of msplat/include/utils.h:9-10. */
 const char *msplat_last_error(void);
 
+/* ---- expired look-back waits (no reference counterpart: the reference's
+ * sort is torch.sort, msplat/sort_gaussian.py:49, which has no such wait).
+ * Every look-back wait of the sorts and the cell binning is bounded.  When
+ * one expires (a predecessor workgroup made no progress for 2^16 polls), the
+ * lists it wrote are undefined; the sort's or binning's last launch then sets
+ * a word per device that stays set until cleared.  While it is set, the
+ * cell-list blending forward writes NaN images and transmittances, and every
+ * binning's counting pass copies it into d_count[1], which the host reads with
+ * the count.  msplat_wait_status writes the word to d_out[0] (device int64;
+ * may be NULL) in stream order and, when `clear` is non-zero, clears it.
+ * msplat_debug_force_expiry (tests only) makes every look-back wait on the
+ * device give up at once, without reading its predecessors, until it is
+ * called again with on = 0; every kernel still runs to its end and writes
+ * only inside its outputs. */
+int msplat_wait_status(int64_t *d_out, int clear, void *stream);
+int msplat_debug_force_expiry(int on, void *stream);
+
 /* ---- projection: msplat/src/project_point.cu:147-179 (_C.project_point_forward,
  *      bound at msplat/src/ext.cpp:14, called from msplat/project_point.py:59) */
 int msplat_project_point_forward(const float *xyz, int64_t n, const float *intr,
@@ -208,7 +225,8 @@ int msplat_timing_only(const char *kernels);
  * keys per tile.  It writes the intersection count K to *d_count (device
  * int64).  If `tiles` is non-NULL it is checked against the rectangles and
  * *d_count is set to -1 on any disagreement (the reference would write out
- * of bounds).  Size queries return 0 for an invalid image size. */
+ * of bounds); -2 reports an expired look-back wait of the depth sort.  Size
+ * queries return 0 for an invalid image size. */
 size_t msplat_sort_prepare_workspace_bytes(int64_t n, int W, int H);
 int msplat_sort_prepare(const float *uv, const float *depth,
                         const int32_t *radius, const int32_t *tiles,
@@ -273,8 +291,9 @@ int msplat_alpha_blending_backward(
  * materialised.  Two calls around one 8-byte device read, like the sort.
  *
  * Step 1 writes each Gaussian's packed tile rectangle to rect [n, 2] uint32
- * (x0 | y0 << 16, x1 | y1 << 16; 8-byte aligned) and the number of cell
- * entries to *d_count. */
+ * (x0 | y0 << 16, x1 | y1 << 16; 8-byte aligned), the number of cell
+ * entries to d_count[0] and the device's expired-wait word (see
+ * msplat_wait_status) to d_count[1]: d_count holds two int64. */
 size_t msplat_bin_prepare_workspace_bytes(int64_t n, int W, int H, int cell);
 int msplat_bin_prepare(const float *uv, const float *depth,
                        const int32_t *radius, int64_t n, int W, int H,
@@ -396,8 +415,10 @@ int msplat_bin_views_prepare(const float *uv, const float *depth,
  * _count writes rect and *d_count (the host may copy the count as soon as
  * _count's work is done), _order then puts the listed Gaussians in depth
  * order (the bulk of step 1), so that the count's round trip to the host
- * overlaps the sort.  *d_count = -1 reports an expired look-back wait;
- * 2^31 means the entries reach 2^31 - 1 (more than int32 lists index). */
+ * overlaps the sort.  d_count[0] = -1 reports an expired look-back wait of
+ * this counting pass, d_count[1] != 0 one of an earlier sort or binning on
+ * the device (see msplat_wait_status); d_count[0] = 2^31 means the entries
+ * reach 2^31 - 1 (more than int32 lists index).  d_count holds two int64. */
 int msplat_bin_views_count(const float *uv, const float *depth,
                            const int32_t *radius, int64_t n, int B,
                            const int *W, const int *H, const int64_t *first,

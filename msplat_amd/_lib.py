@@ -28,6 +28,8 @@ SZ = ctypes.c_size_t
 SIGNATURES = {
     "msplat_version": (I32, []),
     "msplat_last_error": (ctypes.c_char_p, []),
+    "msplat_wait_status": (I32, [P, I32, P]),
+    "msplat_debug_force_expiry": (I32, [I32, P]),
     "msplat_project_point_forward": (I32, [P, I64, P, P, I32, I32, F32, F32, P, P, P]),
     "msplat_project_point_backward": (I32, [P, I64, P, P, P, P, P, P, P, P, P]),
     "msplat_compute_cov3d_forward": (I32, [P, P, P, I64, P, P]),

@@ -30,11 +30,37 @@ class CellLists:
     cell_ids: torch.Tensor    # [capacity] int32: Gaussian ids, cell-major, (depth, id) order within a cell
     cell_range: torch.Tensor  # [cells, 2] int32: [start, end) per cell
     cell: int                 # cell edge in tiles
-    count: torch.Tensor | None = None  # [1] int64 on the device: the true number of entries
+    count: torch.Tensor | None = None  # [2] int64 on the device: the true number of entries, wait state
 
     def overflowed(self) -> bool:
         """True if the entries did not fit the capacity (synchronises)."""
-        return self.count is not None and int(self.count.item()) > self.cell_ids.numel()
+        return self.count is not None and int(self.count[0].item()) > self.cell_ids.numel()
+
+    def check(self) -> None:
+        """Raise RuntimeError if a bounded look-back wait of this binning (or of
+        an earlier one on the device) expired, which leaves the lists undefined
+        (synchronises; clears the device's wait state)."""
+        check_waits(self.cell_range.device)
+
+
+WAIT_EXPIRED = ("a bounded look-back wait of a sort or cell binning on this device expired: the lists it wrote "
+                "were undefined and the images rendered from them were written as NaN")
+
+
+def check_waits(device=None, clear: bool = True) -> None:
+    """Raise RuntimeError if the device's expired-wait word is set (synchronises);
+    the word is cleared first when ``clear``."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    with torch.cuda.device(dev):
+        out = torch.empty(1, dtype=torch.int64, device=dev)
+        _lib.call("msplat_wait_status", _lib.ptr(out), int(bool(clear)), _lib.stream_ptr())
+        if int(out.item()) != 0:
+            raise RuntimeError(WAIT_EXPIRED)
+
+
+def _raise_expired() -> None:
+    _lib.call("msplat_wait_status", None, 1, _lib.stream_ptr())  # cleared: the next binning starts afresh
+    raise RuntimeError(WAIT_EXPIRED)
 
 
 # Capacity plan of a graph capture in progress (msplat_amd.graphs): binning
@@ -68,7 +94,7 @@ def _host_count():
     (kept: creating and destroying an event per call costs more than the wait)."""
     buf = getattr(_host, "count", None)
     if buf is None:
-        buf = _host.count = (torch.empty(1, dtype=torch.int64, pin_memory=True), torch.cuda.Event())
+        buf = _host.count = (torch.empty(2, dtype=torch.int64, pin_memory=True), torch.cuda.Event())
     return buf
 
 
@@ -77,7 +103,9 @@ def prepare_views(u, d, r, n: int, B: int, W, H, F, N, cell: int, rect, prep, pr
     """Step 1 of the binning on the current stream: the counting pass, then the
     depth order.  Without a capacity, the entry count is copied to the host
     between the two and read once the sort is queued behind it (the GPU sorts
-    while the count travels); returns that count, or the capacity.
+    while the count travels); returns that count, or the capacity.  ``count``
+    is a [2] int64 device tensor: the entries, then the device's wait state
+    (non-zero: an earlier binning's look-back wait expired; raises).
     ``queued()``, when given, enqueues further independent work on the stream
     before the host waits: the GPU runs it after the sort instead of idling
     while the host issues the emission."""
@@ -97,9 +125,9 @@ def prepare_views(u, d, r, n: int, B: int, W, H, F, N, cell: int, rect, prep, pr
     # event wait can sleep far longer than the copy takes when the GPU is idle)
     while not ready.query():
         pass
-    k = int(host[0])
-    if k < 0:
-        raise RuntimeError("cell binning: a bounded look-back wait expired")
+    k, expired = int(host[0]), int(host[1])
+    if k < 0 or expired:
+        _raise_expired()
     return k
 
 
@@ -127,7 +155,7 @@ def bin_gaussians(uv, depth, W: int, H: int, radius, cell: int = 8, capacity: in
         rect = torch.empty((n, 2), dtype=torch.int32, device=dev)
         prep_bytes = int(_lib.query("msplat_bin_prepare_workspace_bytes", n, W, H, cell))
         prep = torch.empty(max(prep_bytes, 1), dtype=torch.uint8, device=dev)
-        count = torch.empty(1, dtype=torch.int64, device=dev)
+        count = torch.empty(2, dtype=torch.int64, device=dev)
         Wa, Ha = (_lib.I32 * 1)(W), (_lib.I32 * 1)(H)
         k = prepare_views(u, d, r, n, 1, Wa, Ha, None, None, cell, rect, prep, prep_bytes, count, capacity, queued)
         _observe(count, capacity, k)

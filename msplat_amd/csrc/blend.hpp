@@ -350,6 +350,28 @@ MSPLAT_DEV float gauss_power_col(const PowerCol &p, float c, float dy) {
 // measured 13 us shorter).  More channels would spill under that bound.
 constexpr int blend_min_waves(int cmax) { return cmax <= 4 ? 6 : 1; }
 
+// The forward's output when a sort or binning on this device had an expired
+// look-back wait (its lists are undefined and are not read): NaN image and
+// transmittance, no contributor, NaN loss part.
+template <int PPL>
+MSPLAT_DEV void blend_fwd_undefined(const BlendIn &in, const PixBlock<PPL> &pb, int cc, float *out, float *final_T,
+                                    int32_t *ncontrib) {
+  const float nan = __builtin_nanf("");
+  const int64_t hw = int64_t(pb.H) * pb.W;
+#pragma unroll
+  for (int i = 0; i < PPL; ++i) {
+    if (!(pb.px[i] < pb.W && pb.py[i] < pb.H)) continue;
+    const int64_t pix = int64_t(pb.py[i]) * pb.W + pb.px[i];
+    if (in.c0 == 0) {
+      final_T[pb.pix0 + pix] = nan;
+      ncontrib[pb.pix0 + pix] = 0;
+    }
+    for (int ch = 0; ch < cc; ++ch) out[int64_t(in.C) * pb.pix0 + (in.c0 + ch) * hw + pix] = nan;
+  }
+  if (in.l2_part && threadIdx.x == 0)
+    in.l2_part[int64_t(in.c0 / 64) * in.l2_slots + (blockIdx.x & 7) * in.per_xcd + (blockIdx.x >> 3)] = nan;
+}
+
 template <int CMAX, int PPL, bool kWalk>
 __global__ __launch_bounds__(kBlendLanes) __attribute__((amdgpu_waves_per_eu(blend_min_waves(CMAX)))) void
 blend_fwd_kernel(BlendIn in, float *__restrict__ out, float *__restrict__ final_T, int32_t *__restrict__ ncontrib,
@@ -359,6 +381,10 @@ blend_fwd_kernel(BlendIn in, float *__restrict__ out, float *__restrict__ final_
   if (!pb.active) return;
   const int lane = threadIdx.x;
   const int cc = in.C - in.c0 < CMAX ? in.C - in.c0 : CMAX;  // channels of this pass
+  if (expired_state() != 0u) {  // uniform: lists left undefined by an expired wait
+    blend_fwd_undefined<PPL>(in, pb, cc, out, final_T, ncontrib);
+    return;
+  }
   const int2 lst = tile_list<kWalk>(in, pb);
   int pos = lst.x;
   const int end = lst.y;
@@ -545,6 +571,9 @@ blend_bwd_kernel(
   // (power, alpha, T) goes through explicitly rounded code.
 #pragma clang fp contract(fast)
   static_assert(TG == 1 || (TG == 4 && PPL == 4 && kWalk), "tile groups: cell lists, 4 pixels per lane");
+  // lists left undefined by an expired wait are not read: the gradients keep
+  // their zero fill (the forward wrote NaN)
+  if (expired_state() != 0u) return;  // uniform over the workgroup
   constexpr int NVA = 6 + CMAX;                     // accumulated values per Gaussian
   constexpr int kAcc = TG > 1 ? kGroupAcc : 1;
   __shared__ BlendStage<CMAX> stage_all[TG];

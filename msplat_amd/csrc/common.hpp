@@ -277,6 +277,32 @@ inline void launch_fill_zero_words(std::initializer_list<std::pair<void *, int64
   flush();
 }
 
+// ---- expired waits ----------------------------------------------------------
+// Every look-back wait is bounded.  When one expires, its pass still finishes
+// and stays inside its outputs (a partial prefix only lowers the offsets), but
+// the order it wrote is undefined, so the expiry has to reach the caller:
+//  * the pass sets the error word of its workspace;
+//  * the last launch of the binning ORs the error words into wait_expired,
+//    one word per device that stays set until the host clears it;
+//  * while wait_expired is set, the cell-list blending forward writes NaN
+//    images and transmittances, so undefined lists never yield a
+//    plausible-looking image;
+//  * the next binning's counting pass copies wait_expired into the second
+//    word of its count, which the host reads with the count, and raises.
+// force_expiry (set by msplat_debug_force_expiry, for tests only) makes every
+// look-back give up at once without reading its predecessors, so the whole
+// path can be exercised while every kernel still runs to its end.
+__device__ uint32_t wait_expired;
+__device__ uint32_t force_expiry;
+
+MSPLAT_DEV bool expiry_forced() {
+  return __hip_atomic_load(&force_expiry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+MSPLAT_DEV uint32_t expired_state() {
+  return __hip_atomic_load(&wait_expired, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+
 inline unsigned grid_1d(int64_t n, int threads) {
   return unsigned((n + threads - 1) / threads);
 }

@@ -132,9 +132,10 @@ struct PrepLayout {
   size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, area_s, rect_s, bstatus, total;
 };
 
-// Workgroups of the binning's single-pass scans over n rows (at least one
-// look-back word is always carved).
-int64_t bin_blocks(int64_t n) { return n > 0 ? (n + sortk::kEmitThreads - 1) / sortk::kEmitThreads : 1; }
+// Workgroups (of sortk::kEmitThreads threads) of the cell-list emission over
+// k entries, and so its look-back words; at least one word is always carved.
+// The status buffer is sized for that kernel's workgroup only.
+int64_t emit_blocks(int64_t k) { return k > 0 ? (k + sortk::kEmitThreads - 1) / sortk::kEmitThreads : 1; }
 
 // (W, H are part of the size query so that the layout may depend on the
 // tile grid; it currently does not.)
@@ -187,7 +188,7 @@ EmitLayout emit_layout(int64_t k, int W, int H) {
   L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
   L.ghist = c.take(size_t(sortk::ghist_words()) * 4);
   L.status = c.take(size_t(sortk::status_words(k)) * 4);
-  L.estatus = c.take(size_t(bin_blocks(k)) * 8);  // look-back words of the cell-list emission
+  L.estatus = c.take(size_t(emit_blocks(k)) * 8);  // look-back words of the cell-list emission
   L.total = c.off;
   return L;
 }
@@ -297,6 +298,10 @@ sortk::RadixBuffers prep_radix(void *ws, const PrepLayout &L) {
   return sortk::RadixBuffers{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
                              at<uint32_t>(ws, L.status)};
 }
+sortk::RadixBuffers emit_radix(void *ws, const EmitLayout &L) {
+  return sortk::RadixBuffers{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
+                             at<uint32_t>(ws, L.status)};
+}
 
 // Stable order of the prepared (depth bits, id) pairs by depth -- four 8-bit
 // LSD passes, ids end in L.va; only the first *dlen pairs when dlen is given
@@ -326,24 +331,26 @@ const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t
   const int ncls = vw.cell0[vw.count];
   const PrepLayout P = prep_layout(n, 1, 1);
   const uint32_t *order = at<uint32_t>(prepared, P.va);
+  // the counting pass's and depth sort's error word: the emission reads
+  // nothing of an order it marks undefined
+  const uint32_t *upstream = sortk::radix_error_word(prep_radix(const_cast<void *>(prepared), P));
   const int64_t *offs = at<int64_t>(prepared, P.offs);
   uint32_t *ca = at<uint32_t>(ws, L.ca), *ga = at<uint32_t>(ws, L.ga);
   uint32_t *cb = at<uint32_t>(ws, L.cb), *gb = at<uint32_t>(ws, L.gb);
   int64_t *owner = at<int64_t>(ws, L.owner);
-  const sortk::RadixBuffers rb{at<uint32_t>(ws, L.hist), at<int64_t>(ws, L.sums), at<uint32_t>(ws, L.ghist),
-                               at<uint32_t>(ws, L.status)};
+  const sortk::RadixBuffers rb = emit_radix(ws, L);
   if (direct) {  // cell lists: one thread per listed row, offsets by a single-pass scan
     Timed timed_("emit", s);
     const int64_t rows = n < k ? n : k;  // every listed row has a cell: at most k rows hold slots below k
     sortk::emit_direct_kernel<<<grid_1d(rows, sortk::kEmitThreads), sortk::kEmitThreads, 0, s>>>(
         order, at<int32_t>(prepared, P.area_s), at<uint2>(prepared, P.rect_s), n, at<int64_t>(prepared, P.nlisted),
-        k, dlen, vw, S, ca, ga, at<uint64_t>(ws, L.estatus), bin_blocks(k), sortk::radix_error_word(rb));
+        k, dlen, vw, S, ca, ga, at<uint64_t>(ws, L.estatus), emit_blocks(k), sortk::radix_error_word(rb), upstream);
   } else {  // equal output chunks per workgroup (tile rectangles of thousands of tiles)
     sortk::emit_owner_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
-        at<int32_t>(prepared, P.area), order, offs, n, sortk::emit_chunks(k), owner);
+        at<int32_t>(prepared, P.area), order, offs, n, sortk::emit_chunks(k), owner, upstream);
     Timed timed_("emit", s);
     sortk::emit_chunk_kernel<<<dim3(unsigned(sortk::emit_chunks(k))), sortk::kThreads, 0, s>>>(
-        uv, depth, radius, order, offs, owner, n, k, dlen, vw, S, ca, ga);
+        uv, depth, radius, order, offs, owner, n, k, dlen, vw, S, ca, ga, upstream);
   }
   const int bits = bit_length(uint32_t(ncls));  // classes 0..ncls
   const int npass = (bits + 7) / 8;
@@ -372,6 +379,9 @@ std::string make_views(int B, const int *W, const int *H, int S, const int64_t *
   for (int b = 0; b < B; ++b) {
     const int64_t f = first ? first[b] : 0, c = count ? count[b] : n;
     if (f < 0 || c < 0) return "negative Gaussian range of view " + std::to_string(b);
+    if (f > n || c > n - f)
+      return "Gaussian range [" + std::to_string(f) + ", " + std::to_string(f) + " + " + std::to_string(c) +
+             ") of view " + std::to_string(b) + " exceeds the " + std::to_string(n) + " Gaussians";
     v.vbase[b] = rows;
     v.gbase[b] = f;
     rows += c;
@@ -606,7 +616,17 @@ int msplat_sq_diff_grad(const float *a, const float *t, int64_t n, const float *
   return launched("sq_diff_grad");
 }
 
-int msplat_version(void) { return 100; }
+int msplat_version(void) { return 200; }
+
+int msplat_wait_status(int64_t *d_out, int clear, void *stream) {
+  sortk::wait_status_kernel<<<1, 1, 0, as_stream(stream)>>>(d_out, clear);
+  return launched("wait_status");
+}
+
+int msplat_debug_force_expiry(int on, void *stream) {
+  sortk::set_force_expiry_kernel<<<1, 1, 0, as_stream(stream)>>>(on ? 1u : 0u);
+  return launched("debug_force_expiry");
+}
 
 const char *msplat_last_error(void) { return g_error.c_str(); }
 
@@ -914,7 +934,8 @@ int msplat_sort_prepare(const float *uv, const float *depth, const int32_t *radi
   REQUIRE(workspace, "sort_prepare: null workspace");
   hipStream_t s = as_stream(stream);
   int32_t *flag = at<int32_t>(workspace, L.flag);
-  launch_fill_zero(flag, 1, s);
+  uint32_t *error = sortk::radix_error_word(prep_radix(workspace, L));
+  launch_fill_zero_words({{flag, 1}, {error, 1}}, s);
   if (n > 0) {
     REQUIRE(uv && depth && radius, "sort_prepare: null pointer");
     sortk::prepare_kernel<<<grid_1d(n, sortk::kThreads), sortk::kThreads, 0, s>>>(
@@ -922,7 +943,7 @@ int msplat_sort_prepare(const float *uv, const float *depth, const int32_t *radi
         at<uint32_t>(workspace, L.ka), at<uint32_t>(workspace, L.va), flag);
     depth_order(workspace, L, n, d_count, s);
   }
-  sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, d_count, n);
+  sortk::finish_count_kernel<<<1, 1, 0, s>>>(flag, error, d_count, n);
   return launched("sort_prepare");
 }
 
@@ -946,16 +967,21 @@ int msplat_sort_emit(const float *uv, const float *depth, const int32_t *radius,
     return fail(MSPLAT_ERR_WORKSPACE, "sort_emit: workspace %zu < %zu bytes", workspace_bytes, L.total);
   hipStream_t s = as_stream(stream);
   const int gx = tiles_x(W), gy = tiles_y(H), T = gx * gy;
-  launch_fill_zero(tile_range, 2 * int64_t(T), s);
+  REQUIRE(k == 0 || (uv && depth && radius && prepared && workspace && idx_sorted), "sort_emit: null pointer");
+  uint32_t *error = k > 0 ? sortk::radix_error_word(emit_radix(workspace, L)) : nullptr;
+  // step 1's error word (its depth sort): set when the caller went on after a count of -2
+  const uint32_t *prep_error =
+      k > 0 ? sortk::radix_error_word(prep_radix(const_cast<void *>(prepared), prep_layout(n, W, H))) : nullptr;
+  launch_fill_zero_words({{tile_range, 2 * int64_t(T)}, {error, 1}}, s);
   if (k == 0) return launched("sort_emit");
-  REQUIRE(uv && depth && radius && prepared && workspace && idx_sorted, "sort_emit: null pointer");
   const uint32_t *sk = emit_by_class(uv, depth, radius, n, one_view(W, H, 1, n), 1, prepared, k, nullptr, workspace,
                                      L, reinterpret_cast<uint32_t *>(idx_sorted), s);
   sortk::range_vec_kernel<<<grid_1d((k + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, k, nullptr,
-                                                                                           tile_range);
+                                                                                           tile_range, T, error,
+                                                                                           prep_error);
   if (key_sorted)
     sortk::sorted_keys_kernel<<<grid_1d(k, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, idx_sorted, depth, k,
-                                                                                      key_sorted);
+                                                                                      n, key_sorted);
   return launched("sort_emit");
 }
 
@@ -1021,7 +1047,7 @@ int bin_views_prepare_stages(int stages, const float *uv, const float *depth, co
     // word, the look-back words, and the counts (which stay 0 when m = 0)
     uint64_t *bstatus = at<uint64_t>(workspace, L.bstatus);
     launch_fill_zero_words(
-        {{rb.ghist, sortk::ghist_words()}, {bstatus, 2 * sortk::prep_blocks(m)}, {nlisted, 2}, {d_count, 2}}, s);
+        {{rb.ghist, sortk::ghist_words()}, {bstatus, 2 * sortk::prep_blocks(m)}, {nlisted, 2}, {d_count, 4}}, s);
     if (m > 0)
       sortk::bin_prepare_kernel<<<unsigned(sortk::prep_blocks(m)), sortk::kThreads, 0, s>>>(
           uv, depth, radius, m, vw, cell, reinterpret_cast<uint2 *>(rect), ka, va, bstatus, nlisted, d_count,
@@ -1089,13 +1115,21 @@ int msplat_bin_views_emit(const float *uv, const float *depth, const int32_t *ra
   // emission's look-back words
   launch_fill_zero_words({{cell_range, 2 * int64_t(vw.cell0[B])},
                           {count > 0 ? at<uint32_t>(workspace, L.ghist) : nullptr, sortk::ghist_words()},
-                          {count > 0 ? at<uint64_t>(workspace, L.estatus) : nullptr, 2 * bin_blocks(count)}},
+                          {count > 0 ? at<uint64_t>(workspace, L.estatus) : nullptr, 2 * emit_blocks(count)}},
                          s);
-  if (count == 0) return launched("bin_emit");
+  // the counting pass's and the depth sort's error word (step 1 zeroed it)
+  const PrepLayout P = prep_layout(vw.vbase[B], 1, 1);
+  const uint32_t *prep_error =
+      prepared ? sortk::radix_error_word(prep_radix(const_cast<void *>(prepared), P)) : nullptr;
+  if (count == 0) {
+    // no entries: the range kernel only folds the error words
+    sortk::range_vec_kernel<<<1, sortk::kThreads, 0, s>>>(nullptr, 0, nullptr, nullptr, 0, prep_error, nullptr);
+    return launched("bin_emit");
+  }
   const uint32_t *sk = emit_by_class(uv, depth, radius, vw.vbase[B], vw, cell, prepared, count, d_count, workspace,
                                      L, reinterpret_cast<uint32_t *>(cell_ids), s, true);
-  sortk::range_vec_kernel<<<grid_1d((count + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(sk, count, d_count,
-                                                                                               cell_range);
+  sortk::range_vec_kernel<<<grid_1d((count + 3) / 4, sortk::kThreads), sortk::kThreads, 0, s>>>(
+      sk, count, d_count, cell_range, vw.cell0[B], prep_error, sortk::radix_error_word(emit_radix(workspace, L)));
   return launched("bin_emit");
 }
 

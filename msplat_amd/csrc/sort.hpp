@@ -53,10 +53,14 @@ MSPLAT_DEV int64_t dev_len(const int64_t *dlen, int64_t cap) {
 
 // ---- device-wide exclusive scan (reduce, scan of partials, apply) ------------
 
-// Value at position i of the scanned sequence: in[perm[i]] or in[i].
+// Value at position i of the scanned sequence: in[perm[i]] (in[i] without
+// perm); a permuted index past the cap elements of `in` (only an order left
+// undefined by an expired wait holds one) reads 0.
 template <typename In>
-MSPLAT_DEV int64_t scan_load(const In *in, const uint32_t *perm, int64_t i) {
-  return static_cast<int64_t>(perm ? in[perm[i]] : in[i]);
+MSPLAT_DEV int64_t scan_load(const In *in, const uint32_t *perm, int64_t i, int64_t cap) {
+  if (!perm) return static_cast<int64_t>(in[i]);
+  const uint32_t p = perm[i];
+  return int64_t(p) < cap ? static_cast<int64_t>(in[p]) : 0;
 }
 
 MSPLAT_DEV int64_t wave_incl_scan(int64_t v) {
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const In *__restr
   int64_t s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j)
-    if (base + j < len) s += scan_load(in, perm, base + j);
+    if (base + j < len) s += scan_load(in, perm, base + j, cap);
   int64_t total;
   block_excl_scan(s, &total);
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(kThreads) void scan_apply_kernel(const In *__restri
   int64_t s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    vals[j] = (base + j < len) ? scan_load(in, perm, base + j) : 0;
+    vals[j] = (base + j < len) ? scan_load(in, perm, base + j, cap) : 0;
     s += vals[j];
   }
   int64_t total;
@@ -225,6 +229,19 @@ constexpr int kMaxPasses = 4;
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kValueMask = (1u << 30) - 1u;
 constexpr int64_t kLookbackSpins = int64_t(1) << 16;
 constexpr int kLookbackWidth = 8;
+
+// ---- expired waits: see wait_expired in common.hpp ------------------------------
+// One thread: out[0] = wait_expired, then clear it when asked.
+__global__ void wait_status_kernel(int64_t *__restrict__ out, int clear) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (out) out[0] = int64_t(atomicOr(&wait_expired, 0u));
+    if (clear) atomicAnd(&wait_expired, 0u);
+  }
+}
+
+__global__ void set_force_expiry_kernel(uint32_t on) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) atomicExch(&force_expiry, on);
+}
 
 template <bool kOnesweep, int kScatterThreads>
 __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
@@ -331,6 +348,10 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
         // Look back kLookbackWidth tiles per round: their words are loaded
         // together (the loads' latencies overlap), then consumed in order.
         bool found = false;
+        if (expiry_forced()) {  // tests: give up without looking
+          atomicOr(error, 1u);
+          found = true;
+        }
         for (int64_t t = tile - 1; t >= 0 && !found; t -= kLookbackWidth) {
           uint32_t w[kLookbackWidth];
 #pragma unroll
@@ -394,7 +415,7 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
     kout[pos] = k;
     vout[pos] = v;
     if (gather.oarea) {  // uniform: per-value records, moved into the sorted order
-      const uint2 r = gather.rect[v];
+      const uint2 r = v < cap ? gather.rect[v] : uint2{0u, 0u};  // v >= cap: an order left undefined
       gather.orect[pos] = r;
       gather.oarea[pos] = rect_cells(r, gather.S);
     }
@@ -563,6 +584,7 @@ MSPLAT_DEV bool lookback_pair(const uint64_t *status, int64_t b, int64_t &hi, in
   const int lane = __lane_id();
   hi = 0;
   lo = 0;
+  if (expiry_forced()) return false;  // tests: give up without looking
   int64_t spins = 0;
   for (int64_t p = b - 1;; p -= kWave) {
     const int64_t q = p - lane;  // before workgroup 0: an inclusive zero
@@ -677,7 +699,8 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
       if (blockIdx.x == gridDim.x - 1) {
         *nlisted = pl + nl;
         const int64_t e = pe + ne;
-        *count = ok ? (e >= kLbField ? kLbField + 1 : e) : -1;
+        count[0] = ok ? (e >= kLbField ? kLbField + 1 : e) : -1;
+        count[1] = int64_t(expired_state());  // an earlier binning's expired wait
       }
     }
   }
@@ -706,12 +729,16 @@ __global__ __launch_bounds__(kEmitThreads) void emit_direct_kernel(
     const uint32_t *__restrict__ order, const int32_t *__restrict__ area_s, const uint2 *__restrict__ rect_s,
     int64_t n, const int64_t *__restrict__ nrows, int64_t cap, const int64_t *__restrict__ dlen, Views v, int S,
     uint32_t *__restrict__ cls, uint32_t *__restrict__ gid, uint64_t *__restrict__ status, int64_t nstatus,
-    uint32_t *__restrict__ error) {
+    uint32_t *__restrict__ error, const uint32_t *__restrict__ upstream) {
   // Row i of the depth order: its Gaussian row order[i], cell count area_s[i]
   // and tile rectangle rect_s[i] (both carried into depth order by the depth
   // sort's last pass, so every read here is coalesced).  Listed rows have
   // positive depth, so every entry has a cell class.
   __shared__ int64_t s_base;
+  // An expired wait of the counting pass or the depth sort (upstream) left the
+  // depth order undefined: nothing here may be read.  The lists stay
+  // unwritten; the range kernel flags them.
+  if (*upstream) return;  // uniform: the word is final before this launch
   const int64_t i = int64_t(blockIdx.x) * kEmitThreads + threadIdx.x;
   const int64_t rows = dev_len(nrows, n);  // rows past the listed ones have nothing to emit
   if (int64_t(blockIdx.x) * kEmitThreads >= rows || int64_t(blockIdx.x) >= nstatus) return;  // uniform
@@ -777,9 +804,16 @@ __global__ __launch_bounds__(kEmitThreads) void emit_direct_kernel(
 // (negative depth) has no tile.  Each thread reads 4 consecutive classes with
 // one 16-byte load and compares them with their neighbours.  The caller
 // zero-fills tile_range (unset tiles stay (0, 0)).
+// It is a sort's last launch: its first thread also folds the sort's error
+// words e0, e1 (either may be NULL) into wait_expired.  Classes above
+// nranges are not written (lists left undefined by an expired wait may hold
+// any class).
 __global__ __launch_bounds__(kThreads) void range_vec_kernel(const uint32_t *__restrict__ cls, int64_t cap,
                                                              const int64_t *__restrict__ dlen,
-                                                             int32_t *__restrict__ range) {
+                                                             int32_t *__restrict__ range, int64_t nranges,
+                                                             const uint32_t *__restrict__ e0,
+                                                             const uint32_t *__restrict__ e1) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ((e0 && *e0) || (e1 && *e1))) atomicOr(&wait_expired, 1u);
   const int64_t k = dev_len(dlen, cap);
   const int64_t i0 = (int64_t(blockIdx.x) * kThreads + threadIdx.x) * 4;
   if (i0 >= k) return;
@@ -799,17 +833,21 @@ __global__ __launch_bounds__(kThreads) void range_vec_kernel(const uint32_t *__r
 #pragma unroll
   for (int j = 1; j <= 4; ++j) {
     const uint32_t cj = c[j];
-    if (cj == 0u || cj == 0xffffffffu) continue;
+    if (cj == 0u || int64_t(cj) > nranges) continue;
     const int64_t i = i0 + j - 1;
     if (c[j - 1] != cj) range[2 * (int64_t(cj) - 1)] = int32_t(i);
     if (c[j + 1] != cj) range[2 * (int64_t(cj) - 1) + 1] = int32_t(i + 1);
   }
 }
 
-__global__ void finish_count_kernel(const int32_t *__restrict__ flag, int64_t *__restrict__ count, int64_t n) {
+// The sort's count: 0 for no Gaussians, -1 when `tiles` disagrees with the
+// rectangles, -2 when a wait of the depth sort expired.
+__global__ void finish_count_kernel(const int32_t *__restrict__ flag, const uint32_t *__restrict__ error,
+                                    int64_t *__restrict__ count, int64_t n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     if (n == 0) *count = 0;
     if (*flag) *count = -1;
+    if (*error) *count = -2;
   }
 }
 
@@ -826,9 +864,10 @@ MSPLAT_HD int64_t emit_chunks(int64_t k) { return (k + kEmitChunk - 1) / kEmitCh
 __global__ __launch_bounds__(kThreads) void emit_owner_kernel(const int32_t *__restrict__ area,
                                                               const uint32_t *__restrict__ order,
                                                               const int64_t *__restrict__ offs, int64_t n,
-                                                              int64_t nchunks, int64_t *__restrict__ owner) {
+                                                              int64_t nchunks, int64_t *__restrict__ owner,
+                                                              const uint32_t *__restrict__ upstream) {
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || *upstream) return;  // upstream: the depth sort's error word (its order is then undefined)
   const int a = area[order[i]];
   if (a == 0) return;
   const int64_t o = offs[i];
@@ -846,9 +885,9 @@ __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
     const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius,
     const uint32_t *__restrict__ order, const int64_t *__restrict__ offs, const int64_t *__restrict__ owner,
     int64_t n, int64_t cap, const int64_t *__restrict__ dlen, Views v, int S,
-    uint32_t *__restrict__ cls, uint32_t *__restrict__ gid) {
+    uint32_t *__restrict__ cls, uint32_t *__restrict__ gid, const uint32_t *__restrict__ upstream) {
   const int64_t k = dev_len(dlen, cap);  // slots past it (or past cap) are not written
-  if (int64_t(blockIdx.x) * kEmitChunk >= k) return;
+  if (int64_t(blockIdx.x) * kEmitChunk >= k || *upstream) return;  // upstream: as emit_owner_kernel
   __shared__ int32_t s_rel[kThreads + 1];
   __shared__ int32_t s_w[kThreads], s_ncx[kThreads];
   __shared__ uint32_t s_g[kThreads], s_c[kThreads];  // id, class of the box's first cell (0: no tile)
@@ -913,11 +952,12 @@ __global__ __launch_bounds__(kThreads) void emit_chunk_kernel(
 __global__ __launch_bounds__(kThreads) void sorted_keys_kernel(const uint32_t *__restrict__ cls,
                                                                const int32_t *__restrict__ idx,
                                                                const float *__restrict__ depth, int64_t k,
-                                                               int64_t *__restrict__ keys) {
+                                                               int64_t n, int64_t *__restrict__ keys) {
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= k) return;
   const uint32_t c = cls[i];
-  const uint32_t b = fbits(depth[idx[i]]);
+  const uint32_t g = uint32_t(idx[i]);  // >= n only in lists left undefined by an expired wait
+  const uint32_t b = int64_t(g) < n ? fbits(depth[g]) : 0u;
   keys[i] = c == 0 ? int64_t(int32_t(b)) : ((int64_t(c) - 1) << 32) | int64_t(b);
 }
 

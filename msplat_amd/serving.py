@@ -37,7 +37,7 @@ import time
 
 import torch
 
-from . import tuning, views
+from . import binning, tuning, views
 
 
 @dataclasses.dataclass
@@ -235,7 +235,9 @@ class RenderServer:
                 continue
             self.used -= b.nbytes
             if b.capacity is not None:
-                k = int(b.count.item())  # the batch has finished: no wait
+                k, expired = (int(v) for v in b.count.tolist())  # the batch has finished: no wait
+                if expired:  # an earlier batch's binning left undefined lists (its images are NaN)
+                    binning.check_waits(self.device)
                 if k > b.capacity:  # truncated lists: raise the estimate, render again exactly
                     self.entries_per_row = max(self.entries_per_row, 1.25 * k / max(b.rows, 1))
                     self.rerenders += 1

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import torch
 
-from . import _lib
+from . import _lib, binning
 
 TILE = 16
 
@@ -48,7 +48,10 @@ def sort_gaussian_keys(uv, depth, W: int, H: int, radius, tiles=None, want_keys:
         count = torch.empty(1, dtype=torch.int64, device=dev)
         _lib.call("msplat_sort_prepare", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), _lib.ptr(t), n, int(W), int(H),
                   _lib.ptr(prep), prep_bytes, _lib.ptr(count), stream)
-        k = int(count.item())  # the only host synchronisation of the sort
+        k = int(count.item())  # the first host synchronisation of the sort
+        if k == -2:
+            binning.check_waits(dev)  # clears the device's wait state
+            raise RuntimeError(binning.WAIT_EXPIRED)
         if k < 0:
             raise RuntimeError("sort_gaussian: tiles disagree with the tile boxes of (uv, radius)")
         idx_sorted = torch.empty(k, dtype=torch.int32, device=dev)
@@ -58,6 +61,9 @@ def sort_gaussian_keys(uv, depth, W: int, H: int, radius, tiles=None, want_keys:
         ws = torch.empty(max(emit_bytes, 1), dtype=torch.uint8, device=dev)
         _lib.call("msplat_sort_emit", _lib.ptr(u), _lib.ptr(d), _lib.ptr(r), n, int(W), int(H), _lib.ptr(prep), k,
                   _lib.ptr(ws), emit_bytes, _lib.ptr(idx_sorted), _lib.ptr(tile_range), _lib.ptr(keys), stream)
+        # the second (as the reference's two .item() calls): the class sort's
+        # bounded waits must have ended in time, or the lists are undefined
+        binning.check_waits(dev)
     return idx_sorted, tile_range, keys
 
 

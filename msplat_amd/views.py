@@ -120,7 +120,7 @@ def bin_views(uv, depth, radius, n: int, sizes, cell: int = 8, capacity: int | N
         rect = torch.empty((R, 2), dtype=torch.int32, device=dev)
         prep_bytes = int(_lib.query("msplat_bin_views_prepare_workspace_bytes", R))
         prep = torch.empty(max(prep_bytes, 1), dtype=torch.uint8, device=dev)
-        cnt = torch.empty(1, dtype=torch.int64, device=dev)
+        cnt = torch.empty(2, dtype=torch.int64, device=dev)
         k = binning.prepare_views(u, d, r, n, B, W, H, F, N, cell, rect, prep, prep_bytes, cnt, capacity)
         binning._observe(cnt, capacity, k)
         cell_ids = torch.empty(k, dtype=torch.int32, device=dev)

@@ -53,6 +53,35 @@ def test_version(lib):
     assert lib.msplat_version() >= 100
 
 
+def _views_range_call(L, fn, first, count, n=10):
+    """A one-view call of `fn` with Gaussian range [first, first + count) of n (no pointers: it must fail first)."""
+    W, H = (ctypes.c_int * 1)(64), (ctypes.c_int * 1)(48)
+    F, N = (ctypes.c_int64 * 1)(first), (ctypes.c_int64 * 1)(count)
+    if fn == "bin_views_count":
+        return L.msplat_bin_views_count(None, None, None, n, 1, W, H, F, N, 8, None, None, 0, None, None)
+    if fn == "preprocess_views_forward":
+        return L.msplat_preprocess_views_forward(None, None, None, n, 1, None, None, W, H, F, N,
+                                                 ctypes.c_float(0.0), ctypes.c_float(1.3), None, None, None, None,
+                                                 None, None)
+    return L.msplat_rasterize_views_forward(None, None, None, None, n, 3, 0, 1, W, H, F, N, None, None, None, 8,
+                                            ctypes.c_float(0.0), 0, 1, None, None, None, None, None)
+
+
+@pytest.mark.parametrize("fn", ["bin_views_count", "preprocess_views_forward", "rasterize_views_forward"])
+@pytest.mark.parametrize("first,count", [(5, 6), (11, 0), (0, 11), (3, (1 << 63) - 2)])
+def test_view_range_past_the_gaussians(lib, fn, first, count):
+    """A view whose Gaussian range runs past the n Gaussians is an argument error, raised before any launch."""
+    status = _views_range_call(lib, fn, first, count)
+    assert status == -1
+    assert "exceeds the 10 Gaussians" in lib.msplat_last_error().decode()
+
+
+@pytest.mark.parametrize("fn", ["bin_views_count", "preprocess_views_forward", "rasterize_views_forward"])
+def test_view_range_at_the_end_passes_the_range_check(lib, fn):
+    status = _views_range_call(lib, fn, 4, 6)  # [4, 10): in range, then fails on the null pointers
+    assert "exceeds" not in lib.msplat_last_error().decode() or status == 0
+
+
 @pytest.mark.parametrize("call,needle", [
     (lambda L: L.msplat_project_point_forward(None, -1, None, None, 8, 8, 0.0, 1.3, None, None, None), "n = -1"),
     (lambda L: L.msplat_compute_sh_forward(None, None, None, 4, 3, 17, None, None), "D = 17"),

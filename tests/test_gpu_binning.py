@@ -75,10 +75,10 @@ def test_bins_capacity_mode(cell):
     k = exact.cell_ids.numel()
     assert k > 100 and not exact.overflowed()
     roomy = binning.bin_gaussians(uv, depth, 192, 128, radius, cell, capacity=k + 777)
-    assert not roomy.overflowed() and int(roomy.count.item()) == k
+    assert not roomy.overflowed() and int(roomy.count[0].item()) == k
     assert torch.equal(roomy.cell_ids[:k], exact.cell_ids) and torch.equal(roomy.cell_range, exact.cell_range)
     short = binning.bin_gaussians(uv, depth, 192, 128, radius, cell, capacity=k // 3)
-    assert short.overflowed() and int(short.count.item()) == k
+    assert short.overflowed() and int(short.count[0].item()) == k
     assert int(short.cell_range.max()) <= k // 3
 
 
