@@ -12,6 +12,38 @@ import numpy as np
 from . import blend, geometry, sort
 
 
+def stage_geometry(xyz, scale, rotate, intr, extr, W, H, dtype=np.float32):
+    """Projection, covariance and EWA (msplat/__init__.py:71-82) with the fp32
+    discrete decisions (culling, radius, tile boxes) and, for dtype=np.float64,
+    the continuous quantities recomputed in fp64.  Returns a dict of uv,
+    depth, visible, cov3d, conic, radius, tiles."""
+    uv, depth = geometry.project_forward(xyz, intr, extr, W, H, dtype=np.float32)
+    visible = depth != 0
+    cov3d = geometry.cov3d_forward(scale, rotate, visible, dtype=np.float32)
+    conic, radius, tiles = geometry.ewa_forward(xyz, cov3d, intr, extr, uv, W, H, visible, dtype=np.float32)
+    out = dict(uv=uv, depth=depth, visible=visible, cov3d=cov3d, conic=conic, radius=radius, tiles=tiles)
+    if np.dtype(dtype) != np.float32:
+        uv64, d64 = geometry.project_forward(xyz, intr, extr, W, H, nearest=0.0, extent=0.0, dtype=dtype)
+        out["uv"] = np.where(visible[:, None], uv64, 0.0)
+        out["depth"] = np.where(visible, d64, 0.0)
+        out["cov3d"] = geometry.cov3d_forward(scale, rotate, visible, dtype=dtype)
+        craw = geometry.ewa_forward(xyz, out["cov3d"], intr, extr, out["uv"], W, H, visible, dtype=dtype, raw=True)
+        out["conic"] = np.where((radius > 0)[:, None], craw, 0.0)
+        out["uv32"], out["depth32"] = uv, depth
+    return out
+
+
+def stage_geometry_backward(xyz, scale, rotate, intr, extr, geo, duv, dconic, ddepth=None, dtype=np.float32):
+    """Adjoint of stage_geometry: (dxyz, dscale, drotate, dintr, dextr), the
+    chain autograd runs through ewa_project, compute_cov3d and project_point
+    (ddepth: the depth's own gradient, e.g. from a depth feature channel)."""
+    dxyz_e, dcov, dintr_e, dextr_e = geometry.ewa_backward(xyz, geo["cov3d"], intr, extr, geo["radius"], dconic,
+                                                           dtype=dtype)
+    dscale, drot = geometry.cov3d_backward(scale, rotate, dcov, geo["visible"], dtype=dtype)
+    dxyz_p, dintr_p, dextr_p = geometry.project_backward(xyz, intr, extr, geo["depth"], duv, ddepth, dtype=dtype)
+    return dxyz_p + dxyz_e, dscale, drot, dintr_p + dintr_e, dextr_p + dextr_e
+
+
 def rasterize_forward(xyz, scale, rotate, opacity, feature, intr, extr, W, H, bg, dtype=np.float32):
     """Returns the rendered image [C,H,W] and a dict of intermediates.
 

@@ -113,3 +113,21 @@ def cell_lists(uv, depth, radius, W, H, cell):
     rect = np.stack([x0, y0, x1, y1], 1).astype(np.int64)
     rect[~live] = 0
     return rect, ids.astype(np.int32), rng
+
+
+def band_lists(uv, depth, radius, W, H, row0, row1):
+    """sort_gaussian (sort_gaussian.cu:17-71, sort_gaussian.py:42-52)
+    restricted to the tiles of tile rows [row0, row1): every such tile gets
+    the list (ids and order) the full sort gives it, every other tile an
+    empty range.  Only the Gaussians whose tile box meets the band are keyed,
+    so a band of a large frame costs a fraction of the full sort."""
+    gx, gy = grid(W, H)
+    uv = np.asarray(uv, np.float32)
+    r = np.asarray(radius).reshape(-1).astype(np.int64)
+    _, y0, _, y1 = tile_rect(uv[:, 0], uv[:, 1], r, gx, gy, np.float32)
+    sub = np.nonzero((r > 0) & (y0 < row1) & (y1 > row0))[0]
+    keys, ids, _ = gaussian_keys(uv[sub], np.asarray(depth, np.float32).reshape(-1)[sub], r[sub], W, H)
+    tiles = keys >> 32  # -1 for negative depths (sign extension): in no band
+    keep = (tiles >= row0 * gx) & (tiles < row1 * gx)
+    ks, idx = sort_keys(keys[keep], sub[ids[keep]].astype(np.int32))  # sub ascending: ties keep Gaussian order
+    return idx, tile_range(ks, W, H)

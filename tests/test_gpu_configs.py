@@ -1,0 +1,293 @@
+"""Parity at each benchmark workload's own size (the small-scene tests cover
+the same code on inputs the oracle finishes in milliseconds).
+
+  C0  N = 10 000, 256 x 256, C = 3: forward and backward over the whole image
+      against the fp64 oracle.
+  C1  N = 1 000 000, 1920 x 1080, SH degree 3 + depth (C = 4), L2 loss:
+      rasterization_sh_l2 against the composition rasterization_sh + l2_loss;
+      the image over 4 tile rows against the fp64 oracle; every Gaussian's
+      gradient of a loss on those rows against the oracle's adjoint; fp64
+      finite differences for a fixed 1 000-Gaussian subset of the full loss.
+  C2  N = 500 000, 800 x 800, SH degree 10 + 32 extra channels (C = 35):
+      image and gradients over a tile band against the fp64 oracle; the SH
+      values of a 10 000-row sample bit-exact against the fp32 oracle.
+  C3  N = 2 000 000, 16 views of 400x300 .. 1920x1080: one many-view launch
+      equals 16 single-view runs (images bit for bit, gradients 1e-4).
+  C4  the four resident scenes of 0.1, 0.5, 1.5 and 3 M Gaussians: every
+      request's image is the same alone and batched in the mix.
+
+Tolerances: images 1e-4 max-abs against fp64, gradients 1e-3 relative (max
+over the array), as the north star states; the fp32 error against fp64 is
+printed."""
+
+import concurrent.futures as cf
+import os
+import random
+
+import numpy as np
+import pytest
+import torch
+
+import msplat_amd as ms
+from msplat_amd import scenes, serving, views
+from oracle import pipeline
+from oracle import sh as osh
+from tests.band import WindowFD, sh_scene_backward, sh_scene_forward
+from tests.helpers import np_, rel_err
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def _npd(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _leaves(tensors):
+    return [t.to(DEV).clone().requires_grad_() for t in tensors]
+
+
+def test_c0_full_image_and_gradients():
+    cfg = scenes.config_small()
+    g, cam = cfg["gaussians"], cfg["camera"]
+    W, H = cam.W, cam.H
+    print(f"C0: n={g['xyz'].shape[0]} {W}x{H}", flush=True)
+    leaves = _leaves([g["xyz"], g["scale"], g["rotate"], g["opacity"], g["feature"], cam.intr, cam.extr])
+    out = ms.rasterization(*leaves, W, H, 0.0)
+    gout = np.random.default_rng(0).normal(size=out.shape).astype(np.float32)
+    (out * torch.from_numpy(gout).to(DEV)).sum().backward()
+    npg = {k: np_(v) for k, v in g.items()}
+    args = (npg["xyz"], npg["scale"], npg["rotate"], npg["opacity"], npg["feature"], np_(cam.intr), np_(cam.extr),
+            W, H, 0.0)
+    o64, inter = pipeline.rasterize_forward(*args, dtype=np.float64)
+    err = float(np.abs(np_(out) - o64).max())
+    print(f"C0 image max-abs error vs fp64 oracle: {err:.3g}", flush=True)
+    assert err < 1e-4
+    ref = pipeline.rasterize_backward(*args, inter, gout, dtype=np.float64)
+    for a, b, w in zip(leaves, (ref["dxyz"], ref["dscale"], ref["drotate"], ref["dopacity"].reshape(-1, 1),
+                                ref["dfeature"], ref["dintr"], ref["dextr"]), ("xyz", "scale", "rotate", "opacity",
+                                                                            "feature", "intr", "extr")):
+        e = rel_err(np_(a.grad).reshape(-1)[:b.size] if w == "extr" else np_(a.grad), b)
+        print(f"C0 d{w} relative error {e:.3g}", flush=True)
+        assert e < 1e-3, w
+
+
+@pytest.fixture(scope="module")
+def c1():
+    cfg = scenes.config_sh3_1080p()
+    g, cam = cfg["gaussians"], cfg["camera"]
+    return g, cam, cfg["target"]
+
+
+def _sh_leaves(g, cam):
+    P = {k: g[k].to(DEV).clone().requires_grad_() for k in ("xyz", "scale", "rotate", "opacity", "shs")}
+    return P, cam.intr.to(DEV), cam.extr.to(DEV), cam.eye.to(DEV)
+
+
+def test_c1_fused_loss_equals_composition(c1):
+    g, cam, target = c1
+    W, H = cam.W, cam.H
+    print(f"C1: n={g['xyz'].shape[0]} {W}x{H} fused loss vs composition", flush=True)
+    A, intr, extr, campos = _sh_leaves(g, cam)
+    B, _, _, _ = _sh_leaves(g, cam)
+    t = target.to(DEV)
+    img_a, loss_a = ms.rasterization_sh_l2(A["xyz"], A["scale"], A["rotate"], A["opacity"], A["shs"], intr, extr,
+                                           campos, W, H, 0.0, t, depth_channel=True)
+    img_b = ms.rasterization_sh(B["xyz"], B["scale"], B["rotate"], B["opacity"], B["shs"], intr, extr, campos, W, H,
+                                0.0, depth_channel=True)
+    loss_b = ms.l2_loss(img_b, t)
+    assert torch.equal(img_a, img_b)
+    ref64 = float(((img_b.detach().double() - t.double()) ** 2).sum())
+    assert abs(float(loss_a) - ref64) <= 1e-6 * ref64
+    loss_a.backward()
+    loss_b.backward()
+    for k in ("xyz", "scale", "rotate", "opacity", "shs"):
+        e = rel_err(np_(A[k].grad), np_(B[k].grad))
+        assert e < 1e-5, (k, e)
+
+
+C1_ROWS = (30, 38)  # 8 tile rows through the middle of the 68
+
+
+def test_c1_band_image_and_gradients_vs_oracle(c1):
+    """The image over C1_ROWS against the fp64 oracle, and every Gaussian's
+    gradient of sum over those rows of (image - target)^2 against the oracle's
+    fp64 adjoint (the same loss: dL/dimage is zero outside the band)."""
+    g, cam, target = c1
+    W, H = cam.W, cam.H
+    P, intr, extr, campos = _sh_leaves(g, cam)
+    img = ms.rasterization_sh(P["xyz"], P["scale"], P["rotate"], P["opacity"], P["shs"], intr, extr, campos, W, H,
+                              0.0, depth_channel=True)
+    y0, y1 = 16 * C1_ROWS[0], 16 * C1_ROWS[1]
+    mask = torch.zeros(1, H, W, device=DEV)
+    mask[:, y0:y1] = 1.0
+    dout = (2.0 * (img.detach() - target.to(DEV)) * mask).contiguous()
+    (img * dout).sum().backward()
+    print("C1: oracle over the band ...", flush=True)
+    npg = {k: _npd(v) for k, v in g.items()}
+    args = (npg["xyz"], npg["scale"], npg["rotate"], npg["opacity"], npg["shs"], np_(cam.intr), np_(cam.extr),
+            _npd(cam.eye), W, H, 0.0)
+    o64, st = sh_scene_forward(*args, depth_channel=True, rows=C1_ROWS)
+    err = float(np.abs(np_(img)[:, y0:y1] - o64[:, y0:y1]).max())
+    print(f"C1 band image max-abs error vs fp64 oracle: {err:.3g} ({len(st['idx'])} band intersections)", flush=True)
+    assert err < 1e-4
+    ref = sh_scene_backward(*args, st, np_(dout).astype(np.float64), depth_channel=True)
+    for k, rk in (("xyz", "dxyz"), ("scale", "dscale"), ("rotate", "drotate"), ("opacity", "dopacity"),
+                  ("shs", "dshs")):
+        e = rel_err(np_(P[k].grad).reshape(ref[rk].shape), ref[rk])
+        print(f"C1 band d{k} relative error {e:.3g}", flush=True)
+        assert e < 1e-3, k
+
+
+def test_c1_finite_differences_on_1000_gaussians(c1):
+    """fp64 central differences of the full L2 loss (rasterization_sh_l2's) along
+    a random direction in each of 1 000 Gaussians' parameters (xyz, scale,
+    rotate, opacity, the DC SH coefficients), against the GPU gradient.  The
+    subset: Gaussians whose tile box lies inside C1_ROWS and that reach the
+    image (non-zero gradient), drawn with a fixed seed.  A Gaussian whose two
+    step sizes disagree (a blending threshold crossed within the step) is
+    left out; at most 5 % may be."""
+    g, cam, target = c1
+    W, H = cam.W, cam.H
+    P, intr, extr, campos = _sh_leaves(g, cam)
+    _, loss = ms.rasterization_sh_l2(P["xyz"], P["scale"], P["rotate"], P["opacity"], P["shs"], intr, extr, campos,
+                                     W, H, 0.0, target.to(DEV), depth_channel=True)
+    loss.backward()
+    grad = {k: _npd(P[k].grad) for k in P}
+    npg = {k: _npd(v) for k, v in g.items()}
+    intr_n, extr_n, campos_n = np_(cam.intr), np_(cam.extr), _npd(cam.eye)
+    args = (npg["xyz"], npg["scale"], npg["rotate"], npg["opacity"], npg["shs"], intr_n, extr_n, campos_n, W, H, 0.0)
+    _, st = sh_scene_forward(*args, depth_channel=True, rows=C1_ROWS)
+    fd = WindowFD(st["geo"], npg["opacity"], st["feat"], st["idx"], st["rng"], W, H, _npd(target), 0.0, intr_n,
+                  extr_n, campos=campos_n, shs=npg["shs"], depth_channel=True)
+    geo = st["geo"]
+    gx, gy = (W + 15) // 16, (H + 15) // 16
+    from oracle import geometry
+    x0, y0, x1, y1 = geometry.tile_rect(geo["uv32"][:, 0], geo["uv32"][:, 1], geo["radius"], gx, gy)
+    inside = (geo["radius"] > 0) & (y0 >= C1_ROWS[0]) & (y1 <= C1_ROWS[1]) & (np.abs(grad["opacity"][:, 0]) > 0)
+    pool = np.nonzero(inside)[0]
+    assert len(pool) >= 1000, len(pool)
+    rng = np.random.default_rng(1000)
+    subset = np.sort(rng.choice(pool, 1000, replace=False))
+    dirs = rng.normal(size=(1000, 14))
+    print(f"C1: FD over {len(subset)} Gaussians (pool {len(pool)}) ...", flush=True)
+
+    def one(j):
+        gi = subset[j]
+        p = fd.params(gi, npg["xyz"], npg["scale"], npg["rotate"], npg["opacity"])
+        h = np.r_[np.full(3, 1e-6), 1e-6 * npg["scale"][gi], np.full(4, 1e-6), 1e-6, np.full(3, 1e-6)]
+        v = h * dirs[j]
+        an = (grad["xyz"][gi] @ v[0:3] + grad["scale"][gi] @ v[3:6] + grad["rotate"][gi] @ v[6:10]
+              + grad["opacity"][gi, 0] * v[10] + grad["shs"][gi, :, 0] @ v[11:14])
+        d1, d4 = fd.directional(gi, p, v)
+        return an, d1, d4
+
+    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        res = np.array(list(ex.map(one, range(len(subset)))))
+    an, d1, d4 = res[:, 0], res[:, 1], res[:, 2]
+    smooth = np.abs(d1 - d4) <= 1e-4 * np.maximum(np.abs(d1), 1e-12)
+    print(f"C1 FD: {int(smooth.sum())} of {len(subset)} smooth", flush=True)
+    assert smooth.sum() >= 950
+    a, f = an[smooth], d1[smooth]
+    e = float(np.abs(a - f).max() / np.abs(f).max())
+    per = np.abs(a - f) / np.maximum(np.abs(f), 1e-30)
+    print(f"C1 FD: max-normalised error {e:.3g}, median per-Gaussian relative error {np.median(per):.3g}", flush=True)
+    assert e < 1e-3
+    assert float(np.median(per)) < 1e-3
+
+
+C2_ROWS = (24, 27)  # 3 tile rows through the middle of the 50
+
+
+def test_c2_band_image_gradients_and_sh():
+    cfg = scenes.config_sh10_wide()
+    g, cam = cfg["gaussians"], cfg["camera"]
+    W, H = cam.W, cam.H
+    n = g["xyz"].shape[0]
+    print(f"C2: n={n} {W}x{H} SH-10 + {g['extra'].shape[1]} extra channels", flush=True)
+    P = {k: g[k].to(DEV).clone().requires_grad_() for k in ("xyz", "scale", "rotate", "opacity", "shs", "extra")}
+    intr, extr, campos = cam.intr.to(DEV), cam.extr.to(DEV), cam.eye.to(DEV)
+    img = ms.rasterization_sh(P["xyz"], P["scale"], P["rotate"], P["opacity"], P["shs"], intr, extr, campos, W, H,
+                              0.0, extra=P["extra"])
+    assert img.shape == (35, H, W)
+    y0, y1 = 16 * C2_ROWS[0], 16 * C2_ROWS[1]
+    gen = torch.Generator().manual_seed(2)
+    dout = torch.zeros(img.shape)
+    dout[:, y0:y1] = torch.randn(35, y1 - y0, W, generator=gen)
+    dout = dout.to(DEV)
+    (img * dout).sum().backward()
+    # SH values of a 10 000-row sample, bit-exact against the fp32 oracle
+    rows = torch.from_numpy(np.random.default_rng(3).choice(n, 10000, replace=False))
+    val = ms.compute_sh_points(g["shs"][rows].to(DEV), g["xyz"][rows].to(DEV), campos)
+    ref32 = osh.sh_points_forward(np_(g["shs"][rows]), np_(g["xyz"][rows]), np_(cam.eye), None, dtype=np.float32)
+    assert np.array_equal(np_(val), ref32)
+    print("C2: oracle over the band ...", flush=True)
+    npg = {k: _npd(v) for k, v in g.items()}
+    args = (npg["xyz"], npg["scale"], npg["rotate"], npg["opacity"], npg["shs"], np_(cam.intr), np_(cam.extr),
+            _npd(cam.eye), W, H, 0.0)
+    o64, st = sh_scene_forward(*args, extra=npg["extra"], rows=C2_ROWS)
+    err = float(np.abs(np_(img)[:, y0:y1] - o64[:, y0:y1]).max())
+    print(f"C2 band image max-abs error vs fp64 oracle: {err:.3g}", flush=True)
+    assert err < 1e-4
+    ref = sh_scene_backward(*args, st, np_(dout).astype(np.float64), extra=npg["extra"])
+    for k, rk in (("xyz", "dxyz"), ("scale", "dscale"), ("rotate", "drotate"), ("opacity", "dopacity"),
+                  ("shs", "dshs"), ("extra", "dextra")):
+        e = rel_err(np_(P[k].grad).reshape(ref[rk].shape), ref[rk])
+        print(f"C2 band d{k} relative error {e:.3g}", flush=True)
+        assert e < 1e-3, k
+
+
+def test_c3_sixteen_views_equal_single_views():
+    cfg = scenes.config_multiview()
+    g, cams = cfg["gaussians"], cfg["cameras"]
+    sizes = [(c.W, c.H) for c in cams]
+    print(f"C3: n={g['xyz'].shape[0]}, {len(cams)} views", flush=True)
+    intrs = torch.stack([c.intr for c in cams]).to(DEV)
+    extrs = torch.stack([c.extr for c in cams]).to(DEV)
+
+    def leaves():
+        return [t.to(DEV).clone().requires_grad_() for t in (g["xyz"], g["scale"], g["rotate"], g["opacity"],
+                                                             g["feature"], extrs)]
+
+    la = leaves()
+    imgs = views.rasterization_views(*la[:5], intrs, la[5], sizes, 0.0)
+    gouts = [torch.randn(im.shape, generator=torch.Generator().manual_seed(b)).to(DEV) for b, im in enumerate(imgs)]
+    sum((im * go).sum() for im, go in zip(imgs, gouts)).backward()
+    lb = leaves()
+    for b, (w, h) in enumerate(sizes):
+        img = ms.rasterization(*lb[:5], intrs[b], lb[5][b], w, h, 0.0)
+        assert torch.equal(img, imgs[b]), f"view {b} image"
+        (img * gouts[b]).sum().backward()
+    for a, r, w in zip(la, lb, ("xyz", "scale", "rotate", "opacity", "feature", "extr")):
+        e = rel_err(np_(a.grad), np_(r.grad))
+        print(f"C3 d{w}: batched vs single-view relative difference {e:.3g}", flush=True)
+        assert e < 1e-4, w
+
+
+def test_c4_four_scenes_alone_and_in_mix():
+    scenes_ = scenes.config_serving()
+    st = serving.SceneStore(DEV, channels=3)
+    for name, sg in scenes_.items():
+        st.add(name, sg["xyz"], sg["scale"], sg["rotate"], sg["opacity"], sg["feature"])
+    print(f"C4: scenes {sorted(st.scenes)}", flush=True)
+    server = serving.RenderServer(st, hbm_budget=32 << 30, max_views=16, streams=3)
+    rng = random.Random(4)
+    reqs = []
+    for i in range(24):
+        W, H = rng.choice(scenes.SERVING_SIZES)
+        cam = scenes.serving_camera(rng, W, H)
+        reqs.append(serving.Request(rng.choice(sorted(st.scenes)), cam.intr, cam.extr, W, H, bg=0.0))
+    for i, r in enumerate(reqs):  # staggered: some arrive while others are on the GPU
+        server.submit(r)
+        if i % 4 == 3:
+            server.step()
+    server.drain()
+    assert len(server.done) == len(reqs)
+    assert len({r.batch for r in reqs}) >= 2
+    for r in reqs:
+        s = st.scenes[r.scene]
+        sl = slice(s.first, s.first + s.count)
+        with torch.no_grad():
+            alone = ms.rasterization(st.xyz[sl], st.scale[sl], st.rotate[sl], st.opacity[sl], st.feature[sl],
+                                     r.intr.to(DEV), r.extr.to(DEV), r.W, r.H, r.bg)
+        assert torch.equal(r.image, alone), f"request on {r.scene} {r.W}x{r.H} (batch {r.batch})"

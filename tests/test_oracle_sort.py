@@ -7,6 +7,7 @@ import struct
 
 import numpy as np
 import pytest
+import torch
 
 from oracle import sort as osort
 from tests.conftest import GOLDEN
@@ -133,3 +134,30 @@ def test_cell_lists_single_cell_is_depth_sort():
     assert crng.shape == (1, 2) and tuple(crng[0]) == (0, n)
     expect = sorted(range(n), key=lambda g: (float(depth[g]), g))
     assert ids.tolist() == expect
+
+
+@pytest.mark.parametrize("seed,W,H,rows", [(3, 200, 136, (3, 6)), (4, 97, 61, (0, 1)), (5, 160, 112, (5, 7))])
+def test_band_lists_equal_full_sort_on_the_band(seed, W, H, rows):
+    """band_lists gives every tile of the band its full-sort list (ids and
+    order: ties, negative depths and boxes crossing the band included) and
+    every other tile nothing."""
+    from msplat_amd import scenes
+    from oracle import pipeline
+    gen = torch.Generator().manual_seed(seed)
+    g = scenes.make_gaussians(gen, 4000, 1.5)
+    cam = scenes.make_camera(gen, W, H, 4.0, 1.2 * W)
+    geo = pipeline.stage_geometry(g["xyz"].numpy(), g["scale"].numpy(), g["rotate"].numpy(), cam.intr.numpy(),
+                                  cam.extr.numpy(), W, H)
+    depth = geo["depth"].copy()
+    depth[::50] = depth[1::50][: len(depth[::50])]  # ties
+    depth[::37] = -depth[::37]  # negative depths: keyed into tile -1
+    assert (depth < 0).any()
+    idx, rng, _ = osort.sort_gaussian(geo["uv"], depth, geo["radius"], W, H)
+    bi, br = osort.band_lists(geo["uv"], depth, geo["radius"], W, H, *rows)
+    gx = (W + 15) // 16
+    for t in range(rng.shape[0]):
+        band = bi[br[t, 0]:br[t, 1]]
+        if rows[0] <= t // gx < rows[1]:
+            assert np.array_equal(idx[rng[t, 0]:rng[t, 1]], band), t
+        else:
+            assert len(band) == 0
