@@ -470,6 +470,45 @@ int msplat_sq_diff_sum(const float *a, const float *t, int64_t n,
 int msplat_sq_diff_grad(const float *a, const float *t, int64_t n,
                         const float *dL, float *grad, void *stream);
 
+/* ---- training step next to the rasterizer --------------------------------
+ *
+ * msplat_adam_step: one Adam step (torch.optim.Adam, no weight decay, no
+ * amsgrad) over `count` fp32 arrays in one launch per 8 arrays, replacing the
+ * optimizer of tutorials/gs_2d.py:32-36,73-90 and tutorials/gs_3d.py:165-172
+ * (torch.optim.Adam with one learning rate per Gaussian attribute).  Array k
+ * has numel[k] elements, learning rate lr[k] and is at step step[k] (1 on
+ * the first update); params, exp_avg and exp_avg_sq are updated in place. */
+int msplat_adam_step(int count, float *const *params,
+                     const float *const *grads, float *const *exp_avg,
+                     float *const *exp_avg_sq, const int64_t *numel,
+                     const float *lr, const int64_t *step, float beta1,
+                     float beta2, float eps, void *stream);
+
+/* msplat_densify_accumulate: the consumer of dL/dndc, the gradient the
+ * rasterizer hands back through `ndc` (msplat/alpha_blending.py:107-110,
+ * msplat/__init__.py:33,90).  For every Gaussian seen by the view (radius >
+ * 0, or a non-zero gradient when radius is NULL): accum += |dL/dndc|,
+ * denom += 1.  dndc is [n,2]. */
+int msplat_densify_accumulate(const float *dndc, const int32_t *radius,
+                              int64_t n, float *accum, float *denom,
+                              void *stream);
+
+/* msplat_prune_clone: prune and refill at a fixed count, as
+ * tutorials/gs_3d.py:193-225 does with torch ops: the M Gaussians with
+ * opacity[g] < threshold (ascending index order) receive the rows of the M
+ * Gaussians of largest score (a stable descending order; score = the
+ * largest of the row's three `scale` entries when `score` is NULL), in every
+ * one of the `narrays` row-major fp32 arrays (widths[k] floats per row).  A
+ * pruned Gaussian that is also a donor keeps its row.  *d_count receives M
+ * (-2 when a bounded wait of the sort expired: nothing is copied).  No host
+ * synchronisation. */
+size_t msplat_prune_clone_workspace_bytes(int64_t n);
+int msplat_prune_clone(const float *opacity, float threshold,
+                       const float *score, const float *scale, int64_t n,
+                       int narrays, float *const *arrays, const int *widths,
+                       void *workspace, size_t workspace_bytes,
+                       int64_t *d_count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

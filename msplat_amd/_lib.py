@@ -84,6 +84,10 @@ SIGNATURES = {
         I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, SZ, P, P, P, P, P, P]),
     "msplat_rasterize_backward_l2": (
         I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "msplat_adam_step": (I32, [I32, P, P, P, P, P, P, P, F32, F32, F32, P]),
+    "msplat_densify_accumulate": (I32, [P, P, I64, P, P, P]),
+    "msplat_prune_clone_workspace_bytes": (SZ, [I64]),
+    "msplat_prune_clone": (I32, [P, F32, P, P, I64, I32, P, P, P, SZ, P, P]),
 }
 
 _lib = None

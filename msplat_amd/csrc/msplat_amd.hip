@@ -6,6 +6,7 @@
 #include "../../include/msplat_amd.h"
 
 #include <atomic>
+#include <cmath>
 #include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include "preprocess.hpp"
 #include "sh.hpp"
 #include "sort.hpp"
+#include "train.hpp"
 
 using namespace msplat;
 
@@ -614,6 +616,137 @@ int msplat_sq_diff_grad(const float *a, const float *t, int64_t n, const float *
   const unsigned blocks = unsigned(want < 8192 ? want : 8192);
   sq_diff_grad_kernel<<<blocks, kLossThreads, 0, as_stream(stream)>>>(a, t, n, dL, grad);
   return launched("sq_diff_grad");
+}
+
+// ---- training step: Adam, densification statistics, prune and refill --------
+
+int msplat_adam_step(int count, float *const *params, const float *const *grads, float *const *exp_avg,
+                     float *const *exp_avg_sq, const int64_t *numel, const float *lr, const int64_t *step,
+                     float beta1, float beta2, float eps, void *stream) {
+  REQUIRE(count >= 0, "adam_step: count = %d", count);
+  if (count == 0) return MSPLAT_OK;
+  REQUIRE(params && grads && exp_avg && exp_avg_sq && numel && lr && step, "adam_step: null array");
+  REQUIRE(beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f, "adam_step: betas (%g, %g) outside [0, 1)",
+          double(beta1), double(beta2));
+  hipStream_t s = as_stream(stream);
+  for (int first = 0; first < count; first += train::kAdamGroups) {
+    train::AdamArgs a{};
+    // torch passes 1 - beta as a Python double rounded to fp32
+    a.w1 = float(1.0 - double(beta1));
+    a.b2 = beta2;
+    a.w2 = float(1.0 - double(beta2));
+    a.eps = eps;
+    int64_t blocks = 0;
+    for (int k = first; k < count && k < first + train::kAdamGroups; ++k) {
+      REQUIRE(numel[k] >= 0, "adam_step: array %d has %lld elements", k, (long long)numel[k]);
+      REQUIRE(step[k] >= 1, "adam_step: array %d at step %lld (the first step is 1)", k, (long long)step[k]);
+      if (numel[k] == 0) continue;
+      REQUIRE(params[k] && grads[k] && exp_avg[k] && exp_avg_sq[k], "adam_step: null pointer in array %d", k);
+      train::AdamGroup &G = a.grp[a.count++];
+      G.p = params[k];
+      G.g = grads[k];
+      G.m = exp_avg[k];
+      G.v = exp_avg_sq[k];
+      G.n = numel[k];
+      G.blk0 = blocks;
+      const double t = double(step[k]);
+      const double bc1 = 1.0 - std::pow(double(beta1), t), bc2 = 1.0 - std::pow(double(beta2), t);
+      G.neg_step = float(-(double(lr[k]) / bc1));
+      G.bc2_sqrt = float(std::sqrt(bc2));
+      G.vec = ((reinterpret_cast<uintptr_t>(G.p) | reinterpret_cast<uintptr_t>(G.g) |
+                reinterpret_cast<uintptr_t>(G.m) | reinterpret_cast<uintptr_t>(G.v)) & 15) == 0;
+      blocks += (G.n + train::kAdamBlockElems - 1) / train::kAdamBlockElems;
+    }
+    REQUIRE(blocks < (int64_t(1) << 31), "adam_step: %lld workgroups", (long long)blocks);
+    if (blocks > 0) train::adam_kernel<<<unsigned(blocks), train::kAdamThreads, 0, s>>>(a);
+  }
+  return launched("adam_step");
+}
+
+int msplat_densify_accumulate(const float *dndc, const int32_t *radius, int64_t n, float *accum, float *denom,
+                              void *stream) {
+  REQUIRE(n >= 0, "densify_accumulate: n = %lld", (long long)n);
+  if (n == 0) return MSPLAT_OK;
+  REQUIRE(dndc && accum && denom, "densify_accumulate: null pointer");
+  REQUIRE((reinterpret_cast<uintptr_t>(dndc) & 7) == 0, "densify_accumulate: dndc must be 8-byte aligned");
+  train::densify_accumulate_kernel<<<grid_1d(n, 256), 256, 0, as_stream(stream)>>>(
+      reinterpret_cast<const float2 *>(dndc), radius, n, accum, denom);
+  return launched("densify_accumulate");
+}
+
+namespace {
+struct PruneLayout {
+  size_t flag, pos, ka, va, kb, vb, rank, prune, total, sums, hist, rsums, ghist, status, end;
+};
+PruneLayout prune_layout(int64_t n) {
+  Carver c;
+  PruneLayout L;
+  const size_t un = size_t(n);
+  L.flag = c.take(un * 4);
+  L.pos = c.take(un * 4);
+  L.ka = c.take(un * 4);
+  L.va = c.take(un * 4);
+  L.kb = c.take(un * 4);
+  L.vb = c.take(un * 4);
+  L.rank = c.take(un * 4);
+  L.prune = c.take(un * 4);
+  L.total = c.take(8);
+  L.sums = c.take(size_t(sortk::scan_tiles(n) > 0 ? sortk::scan_tiles(n) : 1) * 8);
+  const int64_t nt = sortk::radix_tiles(n);
+  L.hist = c.take(size_t(256 * nt) * 4);
+  const int64_t ns = sortk::scan_tiles(256 * nt);
+  L.rsums = c.take(size_t(ns > 0 ? ns : 1) * 8);
+  L.ghist = c.take(size_t(sortk::ghist_words()) * 4);
+  L.status = c.take(size_t(sortk::status_words(n)) * 4);
+  L.end = c.off;
+  return L;
+}
+}  // namespace
+
+size_t msplat_prune_clone_workspace_bytes(int64_t n) { return n < 0 ? 0 : prune_layout(n).end; }
+
+int msplat_prune_clone(const float *opacity, float threshold, const float *score, const float *scale, int64_t n,
+                       int narrays, float *const *arrays, const int *widths, void *workspace,
+                       size_t workspace_bytes, int64_t *d_count, void *stream) {
+  REQUIRE(n >= 0 && n <= int64_t(INT32_MAX), "prune_clone: n = %lld", (long long)n);
+  REQUIRE(narrays >= 0 && narrays <= train::kDensifyArrays, "prune_clone: %d arrays (at most %d)", narrays,
+          train::kDensifyArrays);
+  REQUIRE(d_count, "prune_clone: null count pointer");
+  const PruneLayout L = prune_layout(n);
+  if (workspace_bytes < L.end)
+    return fail(MSPLAT_ERR_WORKSPACE, "prune_clone: workspace %zu < %zu bytes", workspace_bytes, L.end);
+  hipStream_t s = as_stream(stream);
+  if (n == 0) {
+    launch_fill_zero_words({{d_count, 2}}, s);
+    return launched("prune_clone");
+  }
+  REQUIRE(workspace && opacity && (score || scale), "prune_clone: null pointer");
+  REQUIRE(narrays == 0 || (arrays && widths), "prune_clone: null array table");
+  train::DensifyRows rows{};
+  for (int k = 0; k < narrays; ++k) {
+    REQUIRE(arrays[k] && widths[k] >= 1, "prune_clone: array %d (width %d)", k, widths[k]);
+    rows.a[k] = arrays[k];
+    rows.w[k] = widths[k];
+  }
+  rows.count = narrays;
+  uint32_t *flag = at<uint32_t>(workspace, L.flag), *pos = at<uint32_t>(workspace, L.pos);
+  uint32_t *ka = at<uint32_t>(workspace, L.ka), *va = at<uint32_t>(workspace, L.va);
+  uint32_t *kb = at<uint32_t>(workspace, L.kb), *vb = at<uint32_t>(workspace, L.vb);
+  uint32_t *rank = at<uint32_t>(workspace, L.rank), *prune = at<uint32_t>(workspace, L.prune);
+  int64_t *total = at<int64_t>(workspace, L.total);
+  const sortk::RadixBuffers rb{at<uint32_t>(workspace, L.hist), at<int64_t>(workspace, L.rsums),
+                               at<uint32_t>(workspace, L.ghist), at<uint32_t>(workspace, L.status)};
+  launch_fill_zero_words({{rb.ghist, sortk::ghist_words()}}, s);
+  const unsigned g = grid_1d(n, 256);
+  train::densify_prepare_kernel<<<g, 256, 0, s>>>(opacity, threshold, score, scale, n, flag, ka, va);
+  sortk::exclusive_scan<uint32_t, uint32_t>(flag, nullptr, n, at<int64_t>(workspace, L.sums), pos, total, s);
+  const int r = sortk::radix_sort(ka, va, kb, vb, n, 32, 4, rb, s, nullptr, nullptr, {}, true);
+  const uint32_t *order = r ? vb : va;
+  train::densify_lists_kernel<<<g, 256, 0, s>>>(flag, pos, order, n, rank, prune);
+  train::densify_count_kernel<<<1, 1, 0, s>>>(total, sortk::radix_error_word(rb), d_count);
+  const int64_t cw = (n + 3) / 4 < train::kCopyBlocks ? (n + 3) / 4 : train::kCopyBlocks;  // 4 waves each
+  if (narrays > 0) train::densify_copy_kernel<<<unsigned(cw), 256, 0, s>>>(prune, order, rank, n, d_count, rows);
+  return launched("prune_clone");
 }
 
 int msplat_version(void) { return 200; }

@@ -72,6 +72,25 @@ class SceneStore:
         self.scenes[name] = s
         return s
 
+    def add_ply(self, name: str, path: str, activate: bool = True) -> Scene:
+        """Load a scene file (msplat_amd.scene_io's PLY layout) and keep it
+        resident.  Its ``feat_*`` channels are rendered; a file with SH
+        coefficients of degree 0 only renders their SH value (the constant
+        basis function times the coefficient); higher degrees are view
+        dependent and are refused."""
+        from . import scene_io
+
+        s = scene_io.load_ply(path, device="cpu", activate=activate)
+        feat = s.get("feature")
+        if feat is None and "shs" in s:
+            if s["sh_degree"] != 0:
+                raise ValueError(f"scene {name}: SH degree {s['sh_degree']} colours depend on the view; store "
+                                 "rendered feature channels (feat_*) for serving")
+            feat = s["shs"][:, :, 0] * 0.28209479177387814
+        if feat is None:
+            raise ValueError(f"scene {name}: the file holds no feature channels")
+        return self.add(name, s["xyz"], s["scale"], s["rotate"], s["opacity"], feat)
+
     def nbytes(self) -> int:
         return sum(t.numel() * 4 for t in (self.xyz, self.scale, self.rotate, self.opacity, self.feature))
 
@@ -143,6 +162,20 @@ class RenderServer:
         self._cam_dev = [(torch.empty((max_views, 4), device=dev), torch.empty((max_views, 12), device=dev))
                          for _ in self.streams]
         self._cam_copied: list[torch.cuda.Event | None] = [None] * len(self.streams)
+
+    def set_max_views(self, max_views: int) -> None:
+        """Requests per launch from now on (the camera staging grows as needed)."""
+        if not 1 <= max_views <= views.MAX_VIEWS:
+            raise ValueError(f"max_views must be in 1..{views.MAX_VIEWS}")
+        self.drain()
+        if max_views > self._cam_host[0][0].shape[0]:
+            dev = self.store.device
+            self._cam_host = [(torch.empty((max_views, 4), pin_memory=True),
+                               torch.empty((max_views, 12), pin_memory=True)) for _ in self.streams]
+            self._cam_dev = [(torch.empty((max_views, 4), device=dev), torch.empty((max_views, 12), device=dev))
+                             for _ in self.streams]
+            self._cam_copied = [None] * len(self.streams)
+        self.max_views = max_views
 
     def submit(self, req: Request) -> None:
         if req.scene not in self.store.scenes:
@@ -237,7 +270,7 @@ class RenderServer:
             if b.capacity is not None:
                 k, expired = (int(v) for v in b.count.tolist())  # the batch has finished: no wait
                 if expired:  # an earlier batch's binning left undefined lists (its images are NaN)
-                    binning.check_waits(self.device)
+                    binning.check_waits(self.store.device)
                 if k > b.capacity:  # truncated lists: raise the estimate, render again exactly
                     self.entries_per_row = max(self.entries_per_row, 1.25 * k / max(b.rows, 1))
                     self.rerenders += 1
@@ -300,6 +333,7 @@ def run_poisson(server: RenderServer, make_request, rate: float, duration: float
     while t < duration:
         arrivals.append((t, make_request(rng)))
         t += rng.expovariate(rate)
+    b0, r0 = server.next_batch, server.rerenders
     t0 = time.perf_counter()
     nxt = 0
     while True:
@@ -320,5 +354,6 @@ def run_poisson(server: RenderServer, make_request, rate: float, duration: float
                 throughput_req_s=round(len(server.done) / (t1 - t0), 2),
                 throughput_mpix_s=round(pix / (t1 - t0) / 1e6, 2),
                 p50_ms=round(percentile(lat, 0.5), 3), p99_ms=round(percentile(lat, 0.99), 3),
-                max_ms=round(max(lat), 3) if lat else None, batches=server.next_batch,
-                rerendered_batches=server.rerenders)
+                max_ms=round(max(lat), 3) if lat else None, batches=server.next_batch - b0,
+                requests_per_batch=round(len(server.done) / max(server.next_batch - b0, 1), 2),
+                rerendered_batches=server.rerenders - r0)

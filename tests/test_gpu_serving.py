@@ -83,3 +83,29 @@ def test_capacity_overflow_renders_again():
     assert len(server.done) == len(reqs) and server.used == 0
     for r in reqs:
         assert torch.equal(r.image, _alone(st, r)), f"request on scene {r.scene} {r.W}x{r.H} (batch {r.batch})"
+
+
+def test_scene_loaded_from_ply(tmp_path):
+    """A scene written to disk (scene_io's PLY layout) and loaded into a second
+    store renders every request exactly as the in-memory scene does."""
+    from msplat_amd import scene_io
+
+    st = _store()
+    disk = serving.SceneStore(DEV, channels=3)
+    for name in sorted(st.scenes):
+        s = st.scenes[name]
+        sl = slice(s.first, s.first + s.count)
+        path = str(tmp_path / f"{name}.ply")
+        scene_io.save_ply(path, st.xyz[sl], st.scale[sl], st.rotate[sl], st.opacity[sl], feature=st.feature[sl],
+                          activated=False)
+        disk.add_ply(name, path, activate=False)
+    for k in ("xyz", "scale", "rotate", "opacity", "feature"):
+        assert torch.equal(getattr(disk, k), getattr(st, k)), k
+    rng = random.Random(3)
+    server = serving.RenderServer(disk, hbm_budget=4096 << 20, max_views=4)
+    reqs = [_request(rng, disk) for _ in range(6)]
+    for r in reqs:
+        server.submit(r)
+    server.drain()
+    for r in reqs:
+        assert torch.equal(r.image, _alone(st, r))
