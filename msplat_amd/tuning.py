@@ -5,6 +5,8 @@
 ``ppl_bwd``  pixels per lane of the blending adjoint;
 ``cull``     skip, per tile, Gaussians that give none of its pixels alpha >= 1/255.
 
+and, for a batch of cameras, how many views share one launch per stage.
+
 Results do not depend on these (images, final_T and ncontrib are identical
 for every choice; gradients differ only by float summation order).  The
 defaults (cell 8, 2 pixels per lane forward, 4 in the adjoint) were the
@@ -65,6 +67,25 @@ def record_views(sizes, C: int, cfg: BlendConfig) -> None:
         _table[views_key(sizes, C)] = cfg
 
 
+# Views per launch of a batch of cameras (msplat_amd.views): None renders the
+# whole batch in one launch per stage; v splits it into consecutive groups of
+# at most v views, one launch per stage each.
+_per_launch: dict[tuple, int] = {}
+
+
+def views_per_launch_for(sizes, C: int) -> int | None:
+    with _lock:
+        return _per_launch.get(views_key(sizes, C))
+
+
+def record_views_per_launch(sizes, C: int, v: int | None) -> None:
+    with _lock:
+        if v is None:
+            _per_launch.pop(views_key(sizes, C), None)
+        else:
+            _per_launch[views_key(sizes, C)] = int(v)
+
+
 def record(W: int, H: int, C: int, cfg: BlendConfig) -> None:
     with _lock:
         _table[shape_key(W, H, C)] = cfg
@@ -73,6 +94,7 @@ def record(W: int, H: int, C: int, cfg: BlendConfig) -> None:
 def clear() -> None:
     with _lock:
         _table.clear()
+        _per_launch.clear()
 
 
 class override:

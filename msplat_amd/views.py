@@ -186,17 +186,33 @@ def split_views(flat, C: int, sizes):
     return [p.view(C, h, w) for p, (w, h) in zip(parts, sizes)]
 
 
-def rasterization_views(xyz, scale, rotate, opacity, feature, intrs, extrs, sizes, bg: float):
+def rasterization_views(xyz, scale, rotate, opacity, feature, intrs, extrs, sizes, bg: float,
+                        views_per_launch: int | None = None):
     """Render B cameras in one launch per stage; returns a list of ``[C, H_b, W_b]`` images.
 
     feature: [P, C] shared by the views, or [B, P, C] per view (e.g. SH colours
     evaluated per camera).  intrs [B, 4]; extrs [B, 3, 4] or [B, 4, 4];
     sizes: B pairs (W, H).  Differentiable with respect to every tensor input.
+    ``views_per_launch`` (default: the autotuner's choice for these sizes, else
+    all) splits the batch into consecutive groups of that many views, one
+    launch per stage each; images do not depend on the grouping.
     """
     sizes = [(int(w), int(h)) for w, h in sizes]
     C = feature.shape[-1]
-    flat = _RasterizeViews.apply(xyz, scale, rotate, opacity, feature, intrs, extrs, tuple(sizes), bg)
-    return split_views(flat, C, sizes)
+    B = len(sizes)
+    v = views_per_launch if views_per_launch is not None else tuning.views_per_launch_for(sizes, C)
+    if v is None or v >= B:
+        flat = _RasterizeViews.apply(xyz, scale, rotate, opacity, feature, intrs, extrs, tuple(sizes), bg)
+        return split_views(flat, C, sizes)
+    if v < 1:
+        raise RuntimeError(f"rasterization_views: views_per_launch = {v}")
+    out = []
+    for b0 in range(0, B, v):
+        sl = slice(b0, min(B, b0 + v))
+        feat = feature[sl] if feature.dim() == 3 else feature
+        flat = _RasterizeViews.apply(xyz, scale, rotate, opacity, feat, intrs[sl], extrs[sl], tuple(sizes[sl]), bg)
+        out += split_views(flat, C, sizes[sl])
+    return out
 
 
 class _RasterizeViews(torch.autograd.Function):

@@ -29,6 +29,12 @@ def _load():
             b = getattr(lib, "blend_backward" + sfx)
             b.restype = None
             b.argtypes = [P, P, P, P, I64, I32, P, P, F32, I32, I32, P, P, P, P, P, P, P]
+            fm = getattr(lib, "blend_forward_mt" + sfx)
+            fm.restype = I64
+            fm.argtypes = f.argtypes + [I32]
+            bm = getattr(lib, "blend_backward_mt" + sfx)
+            bm.restype = None
+            bm.argtypes = b.argtypes + [I32]
         _lib = lib
     return _lib
 
@@ -47,8 +53,9 @@ def _prep(uv, conic, opacity, feature, idx, rng, dt):
     return uv, conic, opacity, feature, idx, rng
 
 
-def blend_forward(uv, conic, opacity, feature, idx_sorted, tile_range, bg, W, H, dtype=np.float32):
-    """Returns (rendered [C,H,W], final_T [H,W], ncontrib [H,W] int32, pairs evaluated)."""
+def blend_forward(uv, conic, opacity, feature, idx_sorted, tile_range, bg, W, H, dtype=np.float32, threads=1):
+    """Returns (rendered [C,H,W], final_T [H,W], ncontrib [H,W] int32, pairs evaluated).
+    ``threads`` > 1 shares tile rows among OpenMP threads (same outputs)."""
     lib = _load()
     dt = np.dtype(dtype)
     uv, conic, opacity, feature, idx, rng = _prep(uv, conic, opacity, feature, idx_sorted, tile_range, dt)
@@ -56,15 +63,22 @@ def blend_forward(uv, conic, opacity, feature, idx_sorted, tile_range, bg, W, H,
     out = np.zeros((C, H, W), dt)
     fT = np.zeros((H, W), dt)
     nc = np.zeros((H, W), np.int32)
-    fn = lib.blend_forward_f32 if dt == np.float32 else lib.blend_forward_f64
-    ev = fn(_p(uv), _p(conic), _p(opacity), _p(feature), n, C, _p(idx), _p(rng), float(bg), W, H, _p(out), _p(fT),
+    args = (_p(uv), _p(conic), _p(opacity), _p(feature), n, C, _p(idx), _p(rng), float(bg), W, H, _p(out), _p(fT),
             _p(nc))
+    if threads > 1:
+        fn = lib.blend_forward_mt_f32 if dt == np.float32 else lib.blend_forward_mt_f64
+        ev = fn(*args, int(threads))
+    else:
+        fn = lib.blend_forward_f32 if dt == np.float32 else lib.blend_forward_f64
+        ev = fn(*args)
     return out, fT, nc, int(ev)
 
 
 def blend_backward(uv, conic, opacity, feature, idx_sorted, tile_range, bg, W, H, final_T, ncontrib, dout,
-                   dtype=np.float32):
-    """Returns dL/duv [N,2], dL/dconic [N,3], dL/dopacity [N], dL/dfeature [N,C]."""
+                   dtype=np.float32, threads=1):
+    """Returns dL/duv [N,2], dL/dconic [N,3], dL/dopacity [N], dL/dfeature [N,C].
+    ``threads`` > 1 shares tile rows among OpenMP threads, each with gradient
+    arrays of its own summed afterwards (another summation order)."""
     lib = _load()
     dt = np.dtype(dtype)
     uv, conic, opacity, feature, idx, rng = _prep(uv, conic, opacity, feature, idx_sorted, tile_range, dt)
@@ -76,7 +90,12 @@ def blend_backward(uv, conic, opacity, feature, idx_sorted, tile_range, bg, W, H
     dco = np.zeros((n, 3), dt)
     dop = np.zeros(n, dt)
     dfe = np.zeros((n, C), dt)
-    fn = lib.blend_backward_f32 if dt == np.float32 else lib.blend_backward_f64
-    fn(_p(uv), _p(conic), _p(opacity), _p(feature), n, C, _p(idx), _p(rng), float(bg), W, H, _p(fT), _p(nc), _p(g),
-       _p(duv), _p(dco), _p(dop), _p(dfe))
+    args = (_p(uv), _p(conic), _p(opacity), _p(feature), n, C, _p(idx), _p(rng), float(bg), W, H, _p(fT), _p(nc),
+            _p(g), _p(duv), _p(dco), _p(dop), _p(dfe))
+    if threads > 1:
+        fn = lib.blend_backward_mt_f32 if dt == np.float32 else lib.blend_backward_mt_f64
+        fn(*args, int(threads))
+    else:
+        fn = lib.blend_backward_f32 if dt == np.float32 else lib.blend_backward_f64
+        fn(*args)
     return duv, dco, dop, dfe

@@ -16,16 +16,16 @@ static REAL FN(gpower)(REAL a, REAL b, REAL c, REAL dx, REAL dy) {
   return (REAL)-0.5 * ((a * dx) * dx + (c * dy) * dy) - (b * dx) * dy;
 }
 
-/* Forward over the whole image.  Writes out [C,H,W], final_T [H,W],
- * ncontrib [H,W]; returns the number of (pixel, Gaussian) pairs evaluated. */
-long long FN(blend_forward)(const IN *uv, const IN *conic, const IN *opac, const IN *feat,
-                            long long n, int C, const int *idx, const int *range, float bg, int W, int H,
-                            REAL *out, REAL *final_T, int *ncontrib) {
-  (void)n;
+/* Forward over pixel rows [py0, py1).  Writes out [C,H,W], final_T [H,W],
+ * ncontrib [H,W] there; returns the number of (pixel, Gaussian) pairs
+ * evaluated. */
+static long long FN(blend_forward_rows)(const IN *uv, const IN *conic, const IN *opac, const IN *feat, int C,
+                                        const int *idx, const int *range, float bg, int W, int H, int py0, int py1,
+                                        REAL *out, REAL *final_T, int *ncontrib) {
   const int gx = (W + 15) / 16;
   long long evaluated = 0;
   REAL *F = (REAL *)calloc(C > 0 ? C : 1, sizeof(REAL));
-  for (int py = 0; py < H; ++py) {
+  for (int py = py0; py < py1; ++py) {
     for (int px = 0; px < W; ++px) {
       const int tile = (py / 16) * gx + (px / 16);
       const int start = range[2 * tile], end = range[2 * tile + 1];
@@ -58,21 +58,42 @@ long long FN(blend_forward)(const IN *uv, const IN *conic, const IN *opac, const
   return evaluated;
 }
 
-/* Adjoint; gradient outputs are zeroed here and accumulated per pixel in
- * row-major pixel order. */
-void FN(blend_backward)(const IN *uv, const IN *conic, const IN *opac, const IN *feat, long long n,
-                        int C, const int *idx, const int *range, float bg, int W, int H, const REAL *final_T,
-                        const int *ncontrib, const REAL *dout, REAL *duv, REAL *dconic, REAL *dopac,
-                        REAL *dfeat) {
+/* Forward over the whole image (one thread). */
+long long FN(blend_forward)(const IN *uv, const IN *conic, const IN *opac, const IN *feat,
+                            long long n, int C, const int *idx, const int *range, float bg, int W, int H,
+                            REAL *out, REAL *final_T, int *ncontrib) {
+  (void)n;
+  return FN(blend_forward_rows)(uv, conic, opac, feat, C, idx, range, bg, W, H, 0, H, out, final_T, ncontrib);
+}
+
+/* The same with tile rows shared among `threads` OpenMP threads (pixels are
+ * independent, so the outputs equal the one-thread call's). */
+long long FN(blend_forward_mt)(const IN *uv, const IN *conic, const IN *opac, const IN *feat,
+                               long long n, int C, const int *idx, const int *range, float bg, int W, int H,
+                               REAL *out, REAL *final_T, int *ncontrib, int threads) {
+  (void)n;
+  const int gy = (H + 15) / 16;
+  long long evaluated = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) reduction(+ : evaluated)
+  for (int ty = 0; ty < gy; ++ty) {
+    const int py1 = 16 * ty + 16 < H ? 16 * ty + 16 : H;
+    evaluated += FN(blend_forward_rows)(uv, conic, opac, feat, C, idx, range, bg, W, H, 16 * ty, py1, out, final_T,
+                                        ncontrib);
+  }
+  return evaluated;
+}
+
+/* Adjoint over pixel rows [py0, py1), accumulated into the gradient arrays
+ * per pixel in row-major pixel order. */
+static void FN(blend_backward_rows)(const IN *uv, const IN *conic, const IN *opac, const IN *feat, int C,
+                                    const int *idx, const int *range, float bg, int W, int H, int py0, int py1,
+                                    const REAL *final_T, const int *ncontrib, const REAL *dout, REAL *duv,
+                                    REAL *dconic, REAL *dopac, REAL *dfeat) {
   const int gx = (W + 15) / 16;
-  memset(duv, 0, sizeof(REAL) * 2 * n);
-  memset(dconic, 0, sizeof(REAL) * 3 * n);
-  memset(dopac, 0, sizeof(REAL) * n);
-  memset(dfeat, 0, sizeof(REAL) * n * (C > 0 ? C : 0));
   REAL *g = (REAL *)calloc(C > 0 ? C : 1, sizeof(REAL));
   REAL *accum = (REAL *)calloc(C > 0 ? C : 1, sizeof(REAL));
   REAL *last_f = (REAL *)calloc(C > 0 ? C : 1, sizeof(REAL));
-  for (int py = 0; py < H; ++py) {
+  for (int py = py0; py < py1; ++py) {
     for (int px = 0; px < W; ++px) {
       const int tile = (py / 16) * gx + (px / 16);
       const int start = range[2 * tile];
@@ -125,6 +146,62 @@ void FN(blend_backward)(const IN *uv, const IN *conic, const IN *opac, const IN 
   free(g);
   free(accum);
   free(last_f);
+}
+
+/* Adjoint over the whole image (one thread); gradient outputs are zeroed
+ * here. */
+void FN(blend_backward)(const IN *uv, const IN *conic, const IN *opac, const IN *feat, long long n,
+                        int C, const int *idx, const int *range, float bg, int W, int H, const REAL *final_T,
+                        const int *ncontrib, const REAL *dout, REAL *duv, REAL *dconic, REAL *dopac,
+                        REAL *dfeat) {
+  memset(duv, 0, sizeof(REAL) * 2 * n);
+  memset(dconic, 0, sizeof(REAL) * 3 * n);
+  memset(dopac, 0, sizeof(REAL) * n);
+  memset(dfeat, 0, sizeof(REAL) * n * (C > 0 ? C : 0));
+  FN(blend_backward_rows)(uv, conic, opac, feat, C, idx, range, bg, W, H, 0, H, final_T, ncontrib, dout, duv,
+                          dconic, dopac, dfeat);
+}
+
+/* The same with tile rows shared among `threads` OpenMP threads, each adding
+ * into gradient arrays of its own, which are then summed in thread order
+ * (a different summation order than the one-thread call). */
+void FN(blend_backward_mt)(const IN *uv, const IN *conic, const IN *opac, const IN *feat, long long n,
+                           int C, const int *idx, const int *range, float bg, int W, int H, const REAL *final_T,
+                           const int *ncontrib, const REAL *dout, REAL *duv, REAL *dconic, REAL *dopac,
+                           REAL *dfeat, int threads) {
+  const long long per = n * (6 + (C > 0 ? C : 0)); /* duv 2, dconic 3, dopac 1, dfeat C per Gaussian */
+  REAL *priv = (REAL *)calloc((size_t)(per > 0 ? per : 1) * (size_t)threads, sizeof(REAL));
+  const int gy = (H + 15) / 16;
+#pragma omp parallel num_threads(threads)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    REAL *b = priv + (size_t)t * (size_t)per;
+#pragma omp for schedule(dynamic, 1)
+    for (int ty = 0; ty < gy; ++ty) {
+      const int py1 = 16 * ty + 16 < H ? 16 * ty + 16 : H;
+      FN(blend_backward_rows)(uv, conic, opac, feat, C, idx, range, bg, W, H, 16 * ty, py1, final_T, ncontrib,
+                              dout, b, b + 2 * n, b + 5 * n, b + 6 * n);
+    }
+  }
+  memset(duv, 0, sizeof(REAL) * 2 * n);
+  memset(dconic, 0, sizeof(REAL) * 3 * n);
+  memset(dopac, 0, sizeof(REAL) * n);
+  memset(dfeat, 0, sizeof(REAL) * n * (C > 0 ? C : 0));
+  for (int t = 0; t < threads; ++t) {
+    const REAL *b = priv + (size_t)t * (size_t)per;
+#pragma omp parallel for num_threads(threads)
+    for (long long i = 0; i < n; ++i) {
+      duv[2 * i] += b[2 * i];
+      duv[2 * i + 1] += b[2 * i + 1];
+      for (int k = 0; k < 3; ++k) dconic[3 * i + k] += b[2 * n + 3 * i + k];
+      dopac[i] += b[5 * n + i];
+      for (int c = 0; c < C; ++c) dfeat[i * C + c] += b[6 * n + i * C + c];
+    }
+  }
+  free(priv);
 }
 
 #undef FN

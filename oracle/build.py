@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "liboracle_blend.so")
 
 
 def build(verbose: bool = False) -> str:
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-std=c11",
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-std=c11", "-fopenmp",
            SRC, "-o", OUT + ".tmp", "-lm"]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -98,9 +98,9 @@ def test_rasterize_equals_tile_lists(C):
     ref_out, ref_T, ref_nc = blend_forward(uv, conic, op, feat, idx, trng, 0.2, W, H)
     gout = torch.randn(ref_out.shape, generator=torch.Generator().manual_seed(C)).to(DEV)
     ref_g = blend_backward(uv, conic, op, feat, idx, trng, 0.2, W, H, ref_T, ref_nc, gout)
-    for cell in (1, 8):
+    for cell in (1, 8, 16):
         b = binning.bin_gaussians(uv, depth, W, H, radius, cell)
-        for ppl in (1, 2, 4):
+        for ppl in ((1, 2, 4) if cell != 16 else (4,)):  # cell 16, 4 pixels per lane: the benchmarked launch
             for cull in (False, True):
                 print(f"C={C} cell={cell} ppl={ppl} cull={cull}", flush=True)
                 out, T, nc, walk = binning.rasterize_forward(uv, conic, op, feat, b, 0.2, W, H, ppl, cull)
@@ -139,3 +139,29 @@ def test_walk_record():
             assert walk[t, 0] == 0
         else:
             assert walk[t, 0] == pos[tile_last[t] - 1] + 1
+
+
+@pytest.mark.parametrize("C", [3, 4])
+def test_grouped_adjoint_long_lists(C):
+    """A dense, faint scene: tiles walk hundreds of cell-list positions before
+    saturating, beyond the 256 positions the grouped adjoint (2 x 2 tiles per
+    workgroup, cells of an even number of tiles, 4 pixels per lane) keeps in
+    LDS, so its global-memory path for the positions past them runs."""
+    W, H = 96, 80
+    d, uv, depth, conic, radius, tiles = _projected(40 + C, 20000, W, H, ext=1.0, C=C)
+    op = (d["opacity"] * 0.08).contiguous()
+    feat = d["feature"]
+    idx, trng = sort_gaussian(uv, depth, W, H, radius, tiles)
+    ref_out, ref_T, ref_nc = blend_forward(uv, conic, op, feat, idx, trng, 0.0, W, H)
+    assert int(ref_nc.max()) > 600, int(ref_nc.max())
+    gout = torch.randn(ref_out.shape, generator=torch.Generator().manual_seed(C)).to(DEV)
+    ref_g = blend_backward(uv, conic, op, feat, idx, trng, 0.0, W, H, ref_T, ref_nc, gout)
+    for cell in (2, 16):
+        b = binning.bin_gaussians(uv, depth, W, H, radius, cell)
+        for cull in (False, True):
+            print(f"long lists C={C} cell={cell} cull={cull}", flush=True)
+            out, T, nc, walk = binning.rasterize_forward(uv, conic, op, feat, b, 0.0, W, H, 4, cull)
+            assert torch.equal(out, ref_out) and torch.equal(T, ref_T)
+            g = binning.rasterize_backward(uv, conic, op, feat, b, 0.0, W, H, T, nc, walk, gout, 4, cull)
+            for a, r, w in zip(g, ref_g, ("duv", "dconic", "dopacity", "dfeature")):
+                assert rel_err(np_(a), np_(r)) < 1e-5, f"{w} cell={cell} cull={cull}"

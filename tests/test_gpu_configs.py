@@ -164,8 +164,11 @@ def test_c1_finite_differences_on_1000_gaussians(c1):
     gx, gy = (W + 15) // 16, (H + 15) // 16
     from oracle import geometry
     x0, y0, x1, y1 = geometry.tile_rect(geo["uv32"][:, 0], geo["uv32"][:, 1], geo["radius"], gx, gy)
-    inside = (geo["radius"] > 0) & (y0 >= C1_ROWS[0]) & (y1 <= C1_ROWS[1]) & (np.abs(grad["opacity"][:, 0]) > 0)
-    pool = np.nonzero(inside)[0]
+    box = (geo["radius"] > 0) & (y0 >= C1_ROWS[0]) & (y1 <= C1_ROWS[1])
+    reach = np.abs(grad["opacity"][:, 0]) > 0
+    print(f"C1 FD: {int(box.sum())} boxes inside the band, {int(reach.sum())} Gaussians with an opacity gradient, "
+          f"{int((np.abs(grad['xyz']).sum(1) > 0).sum())} with a position gradient", flush=True)
+    pool = np.nonzero(box & reach)[0]
     assert len(pool) >= 1000, len(pool)
     rng = np.random.default_rng(1000)
     subset = np.sort(rng.choice(pool, 1000, replace=False))

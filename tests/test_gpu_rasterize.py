@@ -39,7 +39,8 @@ def test_rasterization_vs_oracle(seed, n, W, H, ext):
     g, cam = small_scene(seed, n=n, W=W, H=H, half_extent=ext)
     leaves = _leaves(g, cam)
     bg = 0.1
-    out = ms.rasterization(*leaves, W, H, bg)
+    ndc = torch.zeros(n, 2, device=DEV, requires_grad=True)
+    out = ms.rasterization(*leaves, W, H, bg, ndc)
     npg = {k: np_(v) for k, v in g.items()}
     o64, inter = pipeline.rasterize_forward(npg["xyz"], npg["scale"], npg["rotate"], npg["opacity"],
                                             npg["feature"], np_(cam.intr), np_(cam.extr), W, H, bg,
@@ -55,6 +56,9 @@ def test_rasterization_vs_oracle(seed, n, W, H, ext):
               (rot.grad, ref["drotate"], "drotate"), (op.grad, ref["dopacity"].reshape(-1, 1), "dopacity"),
               (feat.grad, ref["dfeature"], "dfeature"), (intr.grad, ref["dintr"], "dintr"),
               (extr.grad.reshape(-1), ref["dextr"], "dextr")]
+    # dL/dndc (msplat/alpha_blending.py:107-110): pixel u = (ndc_x + 1) W/2 - 1/2, v likewise with H, so
+    # dL/dndc = dL/duv * (W/2, H/2), with dL/duv the fp64 oracle's blending adjoint (pinned by finite differences)
+    checks.append((ndc.grad, ref["duv"] * np.array([0.5 * W, 0.5 * H]), "dndc"))
     for a, b, w in checks:
         e = rel_err(np_(a), b)
         assert e < 1e-3, f"{w}: relative error {e}"

@@ -110,3 +110,27 @@ def test_backward_fd():
     fd("c", conic, dco, 1e-8)
     fd("o", opac, dop)
     fd("f", feat, dfe)
+
+
+def test_threaded_oracle_equals_one_thread():
+    """The CPU baseline's OpenMP variants: forward bit-exact with the
+    one-thread oracle; adjoint equal up to the order of its per-thread sums."""
+    from tests.helpers import small_scene
+    from oracle import pipeline
+
+    g, cam = small_scene(5, n=3000, W=96, H=80)
+    npg = {k: v.numpy() for k, v in g.items()}
+    args = (npg["xyz"], npg["scale"], npg["rotate"], npg["opacity"], npg["feature"], cam.intr.numpy(),
+            cam.extr.numpy(), 96, 80, 0.2)
+    _, inter = pipeline.rasterize_forward(*args)
+    blend_args = (inter["uv"], inter["conic"], npg["opacity"], npg["feature"], inter["idx_sorted"],
+                  inter["tile_range"], 0.2, 96, 80)
+    a = ob.blend_forward(*blend_args)
+    b = ob.blend_forward(*blend_args, threads=4)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    dout = np.random.default_rng(0).normal(size=a[0].shape).astype(np.float32)
+    ga = ob.blend_backward(*blend_args, a[1], a[2], dout)
+    gb = ob.blend_backward(*blend_args, a[1], a[2], dout, threads=4)
+    for x, y in zip(ga, gb):
+        assert np.abs(x - y).max() <= 1e-5 * max(np.abs(x).max(), 1e-12)
