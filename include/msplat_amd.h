@@ -509,6 +509,24 @@ int msplat_prune_clone(const float *opacity, float threshold,
                        void *workspace, size_t workspace_bytes,
                        int64_t *d_count, void *stream);
 
+/* ---- the fused-loss forward without a workspace (no reference counterpart:
+ * for callers that cannot allocate, such as code under HIP graph capture or
+ * serving loops that forbid hipMalloc).  Same as msplat_rasterize_forward_l2
+ * (same kernels, same results), with the loss partials in a scratch array of
+ * msplat_noalloc_scratch_bytes() (4 MiB) that the library holds on every
+ * GPU it runs on until the process ends; an image needing more
+ * (msplat_rasterize_l2_workspace_bytes) fails with MSPLAT_ERR_CAPACITY
+ * before anything is launched.  Calls on one GPU share the array: calls that
+ * overlap on one GPU race.  Allocates nothing. */
+size_t msplat_noalloc_scratch_bytes(void);
+int msplat_rasterize_forward_l2_noalloc(
+    const float *uv, const float *conic, const float *opacity,
+    const float *feature, int64_t n, int C, const uint32_t *rect,
+    const int32_t *cell_ids, const int32_t *cell_range, int cell, float bg,
+    int W, int H, int ppl, int cull, const float *target, float *rendered,
+    float *final_T, int32_t *ncontrib, int32_t *walk, float *loss,
+    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,6 +88,9 @@ SIGNATURES = {
     "msplat_densify_accumulate": (I32, [P, P, I64, P, P, P]),
     "msplat_prune_clone_workspace_bytes": (SZ, [I64]),
     "msplat_prune_clone": (I32, [P, F32, P, P, I64, I32, P, P, P, SZ, P, P]),
+    "msplat_noalloc_scratch_bytes": (SZ, []),
+    "msplat_rasterize_forward_l2_noalloc": (
+        I32, [P, P, P, P, I64, I32, P, P, P, I32, F32, I32, I32, I32, I32, P, P, P, P, P, P, P]),
 }
 
 _lib = None

@@ -1422,6 +1422,75 @@ int msplat_rasterize_forward_l2(const float *uv, const float *conic, const float
 
 }  // extern "C"
 
+// ---- the fused-loss forward without a workspace argument -------------------
+// For callers that cannot allocate (graph capture, serving loops that forbid
+// hipMalloc): the loss partials live in this file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): the entry point's first call on a device looks up that device's
+// copy with hipGetSymbolAddress and keeps the address.  Every call on a GPU
+// shares its copy; the forward zeroes what must start at zero on each call,
+// and calls that overlap on one GPU race.
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocDevices] = {};
+
+// This device's copy of `scratch` (looked up once per device), or null with
+// the error recorded.
+void *scratch_ptr(int *status) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kNoAllocDevices) {
+    *status = fail(MSPLAT_ERR_LAUNCH, "no-allocation scratch: hipGetDevice: %s (device %d)", hipGetErrorString(e),
+                   dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *status = fail(MSPLAT_ERR_LAUNCH, "no-allocation scratch: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+size_t msplat_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+int msplat_rasterize_forward_l2_noalloc(const float *uv, const float *conic, const float *opacity,
+                                        const float *feature, int64_t n, int C, const uint32_t *rect,
+                                        const int32_t *cell_ids, const int32_t *cell_range, int cell, float bg, int W,
+                                        int H, int ppl, int cull, const float *target, float *rendered,
+                                        float *final_T, int32_t *ncontrib, int32_t *walk, float *loss,
+                                        void *stream) {
+  REQUIRE(target && loss, "rasterize_forward_l2_noalloc: null target or loss");
+  REQUIRE(valid_image(W, H) && C >= 0, "rasterize_forward_l2_noalloc: image %dx%d, C = %d", W, H, C);
+  const size_t need = msplat_rasterize_l2_workspace_bytes(W, H, C);
+  if (need > kNoAllocScratchBytes)
+    return fail(MSPLAT_ERR_CAPACITY,
+                "rasterize_forward_l2_noalloc: %dx%d with %d channels needs %zu bytes of scratch, more than the "
+                "%zu this entry point holds; use msplat_rasterize_forward_l2 with a workspace",
+                W, H, C, need, kNoAllocScratchBytes);
+  int status = MSPLAT_OK;
+  void *ws = scratch_ptr(&status);
+  if (!ws) return status;
+  return rasterize_forward_impl(uv, conic, opacity, feature, n, C, 0, 1, &W, &H, nullptr, nullptr, rect, cell_ids,
+                                cell_range, cell, bg, ppl, cull, rendered, final_T, ncontrib, walk, target,
+                                static_cast<float *>(ws), loss, stream);
+}
+
+}  // extern "C"
+
 namespace {
 // The cell-list adjoint of every entry point; rendered / target / dL_dloss:
 // the fused loss's gradient (optional; dL_drendered may then be NULL).
