@@ -98,18 +98,22 @@ def build_step(cfg, dev, features: str = "fused", fused_loss: bool = True, loss_
 def algorithmic_work(name, w):
     """(bound, work per launch, unit) of a library kernel for this workload.
 
-    w: n Gaussians, nvis of them binned, count cell entries, C rendered
+    w: n Gaussians, nproj of them projected (radius > 0), ngrad of them with a
+    non-zero colour gradient, nvis binned, count cell entries, C rendered
     channels, Csh / D SH channels / coefficients, E = sum of ncontrib.
     Bytes are the algorithmic minimum each launch must move; FLOPs follow the
     per-(pixel, Gaussian) operation counts of the blending forward (16 + 2C)
     and adjoint (30 + 6C)."""
     n, C, Csh, D = w["n"], w["C"], w["Csh"], w["D"]
+    npj, ng = w["nproj"], w["ngrad"]
     table = {
         "preprocess_fwd": ("hbm", 72 * n),          # xyz, scale, rot in; uv, depth, conic, radius, tiles out
         "preprocess_bwd": ("hbm", 108 * n),         # + depth, radius, dL/duv, dL/dconic in; 3 grads out
-        "sh_fwd": ("hbm", n * (4 * Csh * D + 12 + 4 * Csh)),
-        # shs in, dL/dshs out, xyz in, the geometry's xyz gradient in and out (the SH part added), dL/dvalue in
-        "sh_bwd": ("hbm", n * (8 * Csh * D + 36 + 4 * Csh)),
+        # every row: its mask byte in, its colours out; projected rows: centre and coefficients in
+        "sh_fwd": ("hbm", n * (1 + 4 * Csh) + npj * (12 + 4 * Csh * D)),
+        # every row: mask byte in, dL/dshs out; projected rows: centre and dL/dvalue in, the geometry's xyz
+        # gradient in and out (the SH part added); rows with a non-zero gradient: coefficients in
+        "sh_bwd": ("hbm", n * (1 + 4 * Csh * D) + npj * (12 + 4 * Csh + 24) + ng * 4 * Csh * D),
         # the binned Gaussians only: one key read for all digit histograms, 4 x (pair read + pair write)
         "depth_sort": ("hbm", w["nvis"] * (4 + 4 * 16)),
         "emit": ("hbm", w["count"] * 8 + w["nvis"] * 28),
@@ -142,62 +146,104 @@ def roofline(name, st, w):
                  frac=round(ach / HBM_PEAK_GBS, 4))
     else:
         ach = work / sec / 1e12
-        r = dict(kernel=name, bound="mfma", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+        r = dict(kernel=name, bound="valu", achieved=round(ach, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                  frac=round(ach / FP32_PEAK_TFLOPS, 4),
-                 pipe="fp32 vector ALU: blending is not a contraction, no matrix instructions are used")
+                 pipe=("fp32 vector ALU (blending is not a contraction: no matrix instructions); peak = the "
+                       "published fp32 figure, which counts packed fp32, disabled in this build (78.6 unpacked)"))
     r["avg_launch_ms"] = round(st["avg_ms"], 4)
     r["work_per_launch"] = work
     r["traffic"] = read_traffic(name)
     return r
 
 
-def cpu_baseline(cfg, rows: int = 64) -> dict:
-    """Time the CPU oracle (numpy + C, one thread) on a bounded sample of the same frame."""
+def _host_cores() -> int:
+    """Threads for the CPU baseline: this process's CPU share (the affinity
+    mask), at most OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, cores: int | None = None) -> dict:
+    """Time the CPU oracle on the whole frame of the same workload, on `cores`
+    threads: projection, SH, covariance, EWA and their adjoints over row
+    chunks in a thread pool (numpy releases the interpreter lock), the tile
+    sort in numpy, and blending forward + backward over every tile in the C
+    oracle with OpenMP."""
+    import concurrent.futures as cf
+
     import numpy as np
 
     from oracle import blend, geometry, sh, sort
 
+    cores = cores or _host_cores()
     g = {k: v.numpy() for k, v in cfg["gaussians"].items()}
     cam = cfg["camera"]
     W, H = cam.W, cam.H
     intr, extr, eye = cam.intr.numpy(), cam.extr.numpy(), cam.eye.numpy()
+    n = len(g["xyz"])
+    chunks = [slice(i, min(n, i + (n + cores - 1) // cores)) for i in range(0, n, (n + cores - 1) // cores)]
+    pool = cf.ThreadPoolExecutor(cores)
+
+    def per_chunk(fn):
+        parts = list(pool.map(fn, chunks))
+        return [np.concatenate([p[j] for p in parts]) for j in range(len(parts[0]))]
+
+    def fwd(sl):
+        xyz = g["xyz"][sl]
+        uv, depth = geometry.project_forward(xyz, intr, extr, W, H)
+        dirs = xyz - eye[None]
+        dirs = dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+        rgb = sh.sh_forward(g["shs"][sl], dirs)
+        vis = depth != 0
+        cov = geometry.cov3d_forward(g["scale"][sl], g["rotate"][sl], vis)
+        conic, radius, tiles = geometry.ewa_forward(xyz, cov, intr, extr, uv, W, H, vis)
+        return uv, depth, dirs, rgb, vis, cov, conic, radius, tiles
+
     t0 = time.perf_counter()
-    uv, depth = geometry.project_forward(g["xyz"], intr, extr, W, H)
-    dirs = g["xyz"] - eye[None]
-    dirs = dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
-    rgb = sh.sh_forward(g["shs"], dirs)
+    uv, depth, dirs, rgb, vis, cov, conic, radius, tiles = per_chunk(fwd)
     feats = np.concatenate([rgb, depth[:, None]], 1).astype(np.float32)
-    vis = depth != 0
-    cov = geometry.cov3d_forward(g["scale"], g["rotate"], vis)
-    conic, radius, tiles = geometry.ewa_forward(g["xyz"], cov, intr, extr, uv, W, H, vis)
     idx, rng, _ = sort.sort_gaussian(uv, depth, radius, W, H)
     t_geo_fwd = time.perf_counter() - t0
-    # blending: a band of tile rows through the middle of the frame, forward +
-    # backward, scaled to the whole frame
-    gy, gx = (H + 15) // 16, (W + 15) // 16
-    band = max(1, rows // 16)
-    r0 = max(0, gy // 2 - band // 2)
-    keep = np.zeros((gy, gx, 2), np.int32)
-    keep[r0:r0 + band] = rng.reshape(gy, gx, 2)[r0:r0 + band]
     t1 = time.perf_counter()
-    out, fT, nc, _ = blend.blend_forward(uv, conic, g["opacity"], feats, idx, keep.reshape(-1, 2), 0.0, W, H)
+    out, fT, nc, _ = blend.blend_forward(uv, conic, g["opacity"], feats, idx, rng, 0.0, W, H, threads=cores)
     dout = np.ones_like(out)
-    duv, dco, dop, dfe = blend.blend_backward(uv, conic, g["opacity"], feats, idx, keep.reshape(-1, 2), 0.0, W, H,
-                                              fT, nc, dout)
-    t_blend_band = time.perf_counter() - t1
+    duv, dco, dop, dfe = blend.blend_backward(uv, conic, g["opacity"], feats, idx, rng, 0.0, W, H, fT, nc, dout,
+                                              threads=cores)
+    t_blend = time.perf_counter() - t1
     t2 = time.perf_counter()
-    _, dcov, _, _ = geometry.ewa_backward(g["xyz"], cov, intr, extr, radius, dco)
-    geometry.cov3d_backward(g["scale"], g["rotate"], dcov, vis)
-    geometry.project_backward(g["xyz"], intr, extr, depth, duv)
-    sh.sh_backward(g["shs"], dirs, dfe[:, :3])
+
+    def bwd(sl):
+        xyz = g["xyz"][sl]
+        _, dcov, _, _ = geometry.ewa_backward(xyz, cov[sl], intr, extr, radius[sl], dco[sl])
+        ds, dr = geometry.cov3d_backward(g["scale"][sl], g["rotate"][sl], dcov, vis[sl])
+        dxyz = geometry.project_backward(xyz, intr, extr, depth[sl], duv[sl])[0]
+        dsh = sh.sh_backward(g["shs"][sl], dirs[sl], dfe[sl, :3])[0]
+        return ds, dr, dxyz, dsh
+
+    per_chunk(bwd)
     t_geo_bwd = time.perf_counter() - t2
-    scale = gy / band
-    frame_s = t_geo_fwd + t_geo_bwd + t_blend_band * scale
-    return dict(value=round(W * H / frame_s / 1e6, 5), unit="Mpix/s", cores=1, kind="port",
-                sample=(f"same frame (N={len(g['xyz'])}, {W}x{H}, C=4): projection, SH, covariance, EWA, tile sort "
-                        f"and their adjoints timed in full ({t_geo_fwd + t_geo_bwd:.1f} s); blending forward + "
-                        f"backward (C oracle, one thread) timed on tile rows {r0}..{r0 + band - 1} of {gy} "
-                        f"({t_blend_band:.1f} s) and scaled x{scale:.1f}"),
+    pool.shutdown()
+    frame_s = t_geo_fwd + t_geo_bwd + t_blend
+    return dict(value=round(W * H / frame_s / 1e6, 5), unit="Mpix/s", cores=cores, kind="port",
+                cpu_model=_cpu_model(),
+                sample=(f"one whole frame of the same workload (N={n}, {W}x{H}, C=4, fwd+bwd) on {cores} threads: "
+                        f"projection, SH, covariance, EWA and adjoints in numpy over {len(chunks)} row chunks "
+                        f"({t_geo_fwd + t_geo_bwd:.1f} s with the numpy tile sort), blending forward + backward "
+                        f"in the C oracle with OpenMP over all {((W + 15) // 16) * ((H + 15) // 16)} tiles "
+                        f"({t_blend:.1f} s)"),
                 seconds=time.perf_counter() - t0)
 
 
@@ -344,9 +390,10 @@ def bench_batched_1080p(dev, views: int = 8, steps: int = 3, warmup: int = 1) ->
                 max_abs_diff_vs_plain=diff)
 
 
-def bench_serving(dev, rate: float, duration: float) -> dict:
+def bench_serving(dev, rates, duration: float) -> dict:
     """4 resident scenes (100k, 500k, 1.5M, 3M Gaussians, C = 3), Poisson requests for a uniformly chosen
-    scene with a random camera at 640x480, 1280x720 or 1920x1080, forward only."""
+    scene with a random camera at 640x480, 1280x720 or 1920x1080, forward only; a sweep of arrival rates
+    that stops once the server saturates (completed requests per second fall below 90 % of the rate)."""
     from msplat_amd import scenes, serving
 
     store = serving.SceneStore(dev, channels=3)
@@ -360,12 +407,24 @@ def bench_serving(dev, rate: float, duration: float) -> dict:
         return serving.Request(rng.choice(names), cam.intr, cam.extr, W, H)
 
     server = serving.RenderServer(store, hbm_budget=32 << 30, max_views=16, streams=3)
-    _quiet_gc()
-    stats = serving.run_poisson(server, make, rate=rate, duration=duration, seed=5)
-    log(f"serving: {stats}")
-    stats["workload"] = ("4 resident scenes (100k, 500k, 1.5M, 3M), Poisson arrivals, 640x480 / 1280x720 / "
-                         "1920x1080, forward C=3, HBM budget 32 GiB, <= 16 views per launch")
-    return stats
+    sweep = []
+    for i, rate in enumerate(rates):
+        _quiet_gc()
+        st = serving.run_poisson(server, make, rate=rate, duration=duration, seed=5 + i)
+        log(f"serving at {rate:g} req/s: {st}")
+        sweep.append(st)
+        if st["throughput_req_s"] < 0.9 * rate:
+            break
+    under = [st for st in sweep if st["p99_ms"] < 10.0 and st["throughput_req_s"] >= 0.9 * st["rate_req_s"]]
+    best = max(under, key=lambda st: st["rate_req_s"]) if under else None
+    return dict(workload=("4 resident scenes (100k, 500k, 1.5M, 3M), Poisson arrivals, 640x480 / 1280x720 / "
+                          "1920x1080, forward C=3, HBM budget 32 GiB, <= 16 views per launch"),
+                sweep=sweep,
+                highest_rate_p99_under_10ms=None if best is None else dict(
+                    rate_req_s=best["rate_req_s"], p50_ms=best["p50_ms"], p99_ms=best["p99_ms"],
+                    throughput_mpix_s=best["throughput_mpix_s"], requests_per_batch=best["requests_per_batch"]),
+                saturated_at_req_s=sweep[-1]["rate_req_s"] if sweep[-1]["throughput_req_s"] < 0.9 * rates[
+                    len(sweep) - 1] else None)
 
 
 def main():
@@ -378,7 +437,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--blend-config", default=None,
                     help="cell,ppl_fwd,ppl_bwd: use these blending launch parameters (no autotune)")
-    ap.add_argument("--cpu-rows", type=int, default=64, help="pixel rows of the CPU blending sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: this process's share)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a PMC calibration kernel)")
     ap.add_argument("--no-multiview", action="store_true")
@@ -390,8 +449,9 @@ def main():
                     help="fused: rasterization_sh renders SH colours + depth in one op; ops: compute_sh_points + "
                          "project_point + cat + rasterization; torch: view directions in torch + compute_sh + ...")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
-    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-rates", default="200,500,1000,2000,4000,8000,16000",
+                    help="request rates of the serving sweep (per second; stops at saturation)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds per rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the rasterizer shards by view "
@@ -401,7 +461,7 @@ def main():
     import torch
 
     from msplat_amd import _lib, autotune, scenes, tuning
-    from msplat_amd.binning import bin_gaussians, rasterize_forward
+    from msplat_amd.binning import bin_gaussians, rasterize_backward, rasterize_forward
     from msplat_amd.rasterization import preprocess_forward
 
     dev = torch.device("cuda:0")
@@ -440,9 +500,15 @@ def main():
         bins = bin_gaussians(uv, depth, W, H, radius, cfg_used.cell)
         nvis = int(torch.unique(bins.cell_ids).numel())
         feats = torch.zeros(args.n, C, device=dev)
-        _, _, ncontrib, _ = rasterize_forward(uv, conic, P["opacity"], feats, bins, 0.0, W, H, cfg_used.ppl_fwd)
+        _, fT, ncontrib, walk = rasterize_forward(uv, conic, P["opacity"], feats, bins, 0.0, W, H, cfg_used.ppl_fwd)
         E = int(ncontrib.sum().item())
-    w = dict(n=args.n, nvis=nvis, count=int(bins.cell_ids.numel()), C=C, Csh=3, D=16, E=E)
+        dfeat = rasterize_backward(uv, conic, P["opacity"], feats, bins, 0.0, W, H, fT, ncontrib, walk,
+                                   torch.ones(C, H, W, device=dev), cfg_used.ppl_bwd)[3]
+        ngrad = int((dfeat[:, :3] != 0).any(dim=1).sum().item())
+        nproj = int((radius > 0).sum().item())
+        del dfeat
+    w = dict(n=args.n, nvis=nvis, nproj=nproj, ngrad=ngrad, count=int(bins.cell_ids.numel()), C=C, Csh=3, D=16,
+             E=E)
 
     step()  # first call: allocations, library load
     torch.cuda.synchronize()
@@ -512,12 +578,13 @@ def main():
 
     multiview = None if args.no_multiview else bench_multiview(dev)
     batched = None if args.no_multiview else bench_batched_1080p(dev)
-    serving_stats = None if args.no_serving else bench_serving(dev, args.serve_rate, args.serve_seconds)
+    serving_stats = None if args.no_serving else bench_serving(
+        dev, [float(r) for r in args.serve_rates.split(",")], args.serve_seconds)
 
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(cfg, args.cpu_rows)
+        cpu = cpu_baseline(cfg, args.cpu_threads or None)
         log(f"cpu baseline: {cpu['value']} Mpix/s in {cpu.pop('seconds'):.1f} s")
 
     line = dict(metric="fwd+bwd Mpix/s at N=1M,1080p", value=round(value, 3), unit="Mpix/s", n_gpus=1,
