@@ -44,8 +44,8 @@ FP32_PEAK_TFLOPS = 157.3
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "rasterize_1080p_pmc_traffic.txt")
 
 # Kernels the library times (msplat_timing_read names).
-KERNELS = ("preprocess_fwd", "preprocess_bwd", "sh_fwd", "sh_bwd", "depth_sort", "emit", "class_sort",
-           "rasterize_fwd", "rasterize_bwd")
+KERNELS = ("preprocess_fwd", "preprocess_bwd", "sh_fwd", "sh_bwd", "bin_prepare", "depth_sort", "emit",
+           "class_sort", "rasterize_fwd", "rasterize_bwd")
 
 
 def log(*a):
@@ -116,6 +116,8 @@ def algorithmic_work(name, w):
         "sh_bwd": ("hbm", n * (1 + 4 * Csh * D) + npj * (12 + 4 * Csh + 24) + ng * 4 * Csh * D),
         # the binned Gaussians only: one key read for all digit histograms, 4 x (pair read + pair write)
         "depth_sort": ("hbm", w["nvis"] * (4 + 4 * 16)),
+        # uv, depth, radius in; packed rectangle out; the listed rows' depth keys and ids out
+        "bin_prepare": ("hbm", w["n"] * (8 + 4 + 4 + 8) + w["nvis"] * 8),
         "emit": ("hbm", w["count"] * 8 + w["nvis"] * 28),
         "class_sort": ("hbm", w["count"] * (4 + 16)),   # histogram read + one pass (cells fit 8 bits)
         "rasterize_fwd": ("flop", w["E"] * (16 + 2 * C)),

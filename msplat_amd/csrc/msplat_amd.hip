@@ -131,7 +131,8 @@ struct Carver {
 };
 
 struct PrepLayout {
-  size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, area_s, rect_s, bstatus, total;
+  size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, area_s, rect_s, bstatus, ctotal,
+      total;
 };
 
 // Workgroups (of sortk::kEmitThreads threads) of the cell-list emission over
@@ -163,6 +164,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
   L.area_s = c.take(un * 4);  // cell counts and tile rectangles in depth order (cell binning)
   L.rect_s = c.take(un * 8);
   L.bstatus = c.take(size_t(sortk::prep_blocks(n)) * 8);  // look-back words of the counting pass
+  L.ctotal = c.take(size_t(sortk::kCtSlices) * sortk::kFusedCells * 4);  // entries per cell (fused emission)
   (void)W;
   (void)H;
   L.total = c.off;
@@ -170,7 +172,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
 }
 
 struct EmitLayout {
-  size_t ca, ga, cb, gb, owner, hist, sums, ghist, status, estatus, total;
+  size_t ca, ga, cb, gb, owner, hist, sums, ghist, status, estatus, cstatus, total;
 };
 
 EmitLayout emit_layout(int64_t k, int W, int H) {
@@ -191,6 +193,8 @@ EmitLayout emit_layout(int64_t k, int W, int H) {
   L.ghist = c.take(size_t(sortk::ghist_words()) * 4);
   L.status = c.take(size_t(sortk::status_words(k)) * 4);
   L.estatus = c.take(size_t(emit_blocks(k)) * 8);  // look-back words of the cell-list emission
+  // the fused emission: per-cell look-back words of each workgroup, then its ticket
+  L.cstatus = c.take((size_t(emit_blocks(k)) * sortk::kFusedCells + 1) * 4);
   L.total = c.off;
   return L;
 }
@@ -360,6 +364,18 @@ const uint32_t *emit_by_class(const float *uv, const float *depth, const int32_t
   // cell lists: the emission's launch zeroed ghist
   return sortk::radix_sort(ca, ga, cb, gb, k, bits, npass, rb, s, dlen, ids, {}, direct) ? cb : ca;
 }
+
+// The emission writes straight into the cell lists when there are at most
+// kFusedCells cells (the default; MSPLAT_FUSED_EMIT=0 keeps the emission +
+// class sort + range launches).
+bool fused_emit_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("MSPLAT_FUSED_EMIT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+bool fused_emit(const Views &vw) { return fused_emit_enabled() && vw.cell0[vw.count] <= sortk::kFusedCells; }
 
 // Cells of S x S tiles on the tile grid of a W x H image.
 int cells_x(int W, int S) { return (tiles_x(W) + S - 1) / S; }
@@ -1179,12 +1195,18 @@ int bin_views_prepare_stages(int stages, const float *uv, const float *depth, co
     // one launch zeroes the depth sort's digit totals, tickets and error
     // word, the look-back words, and the counts (which stay 0 when m = 0)
     uint64_t *bstatus = at<uint64_t>(workspace, L.bstatus);
-    launch_fill_zero_words(
-        {{rb.ghist, sortk::ghist_words()}, {bstatus, 2 * sortk::prep_blocks(m)}, {nlisted, 2}, {d_count, 4}}, s);
+    uint32_t *ctotal = fused_emit(vw) ? at<uint32_t>(workspace, L.ctotal) : nullptr;
+    launch_fill_zero_words({{rb.ghist, sortk::ghist_words()},
+                            {bstatus, 2 * sortk::prep_blocks(m)},
+                            {nlisted, 2},
+                            {d_count, 4},
+                            {ctotal, int64_t(sortk::kCtSlices) * sortk::kFusedCells}},
+                           s);
+    Timed timed_("bin_prepare", s);
     if (m > 0)
       sortk::bin_prepare_kernel<<<unsigned(sortk::prep_blocks(m)), sortk::kThreads, 0, s>>>(
           uv, depth, radius, m, vw, cell, reinterpret_cast<uint2 *>(rect), ka, va, bstatus, nlisted, d_count,
-          sortk::radix_error_word(rb));
+          sortk::radix_error_word(rb), ctotal);
   }
   if ((stages & 2) && m > 0) {
     // the listed rows (first *nlisted pairs) by depth; the last pass carries
@@ -1244,16 +1266,32 @@ int msplat_bin_views_emit(const float *uv, const float *depth, const int32_t *ra
     return fail(MSPLAT_ERR_WORKSPACE, "bin_emit: workspace %zu < %zu bytes", workspace_bytes, L.total);
   REQUIRE(count == 0 || (uv && depth && radius && prepared && workspace && cell_ids), "bin_emit: null pointer");
   hipStream_t s = as_stream(stream);
+  // the counting pass's and the depth sort's error word (step 1 zeroed it)
+  const PrepLayout P = prep_layout(vw.vbase[B], 1, 1);
+  const uint32_t *prep_error =
+      prepared ? sortk::radix_error_word(prep_radix(const_cast<void *>(prepared), P)) : nullptr;
+  if (count > 0 && count < (int64_t(1) << 30) && fused_emit(vw)) {
+    // straight into the lists: one launch zeroes the ranges, the look-back
+    // words and ticket, and the error word; one emits
+    uint32_t *cst = at<uint32_t>(workspace, L.cstatus);
+    const int64_t nblk = emit_blocks(count);
+    uint32_t *error = sortk::radix_error_word(emit_radix(workspace, L));
+    launch_fill_zero_words(
+        {{cell_range, 2 * int64_t(vw.cell0[B])}, {cst, nblk * sortk::kFusedCells + 1}, {error, 1}}, s);
+    const int64_t rows = vw.vbase[B] < count ? vw.vbase[B] : count;  // every listed row has a cell
+    Timed timed_("emit", s);
+    sortk::emit_cells_kernel<<<grid_1d(rows, sortk::kEmitThreads), sortk::kEmitThreads, 0, s>>>(
+        at<uint32_t>(prepared, P.va), at<int32_t>(prepared, P.area_s), at<uint2>(prepared, P.rect_s), vw.vbase[B],
+        at<int64_t>(prepared, P.nlisted), count, vw, cell, vw.cell0[B], at<uint32_t>(prepared, P.ctotal),
+        cst + nblk * sortk::kFusedCells, cst, cell_ids, cell_range, error, prep_error);
+    return launched("bin_emit");
+  }
   // one launch zeroes the ranges, the class sort's digit totals and the
   // emission's look-back words
   launch_fill_zero_words({{cell_range, 2 * int64_t(vw.cell0[B])},
                           {count > 0 ? at<uint32_t>(workspace, L.ghist) : nullptr, sortk::ghist_words()},
                           {count > 0 ? at<uint64_t>(workspace, L.estatus) : nullptr, 2 * emit_blocks(count)}},
                          s);
-  // the counting pass's and the depth sort's error word (step 1 zeroed it)
-  const PrepLayout P = prep_layout(vw.vbase[B], 1, 1);
-  const uint32_t *prep_error =
-      prepared ? sortk::radix_error_word(prep_radix(const_cast<void *>(prepared), P)) : nullptr;
   if (count == 0) {
     // no entries: the range kernel only folds the error words
     sortk::range_vec_kernel<<<1, sortk::kThreads, 0, s>>>(nullptr, 0, nullptr, nullptr, 0, prep_error, nullptr);

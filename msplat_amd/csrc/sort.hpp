@@ -243,6 +243,55 @@ __global__ void set_force_expiry_kernel(uint32_t on) {
   if (threadIdx.x == 0 && blockIdx.x == 0) atomicExch(&force_expiry, on);
 }
 
+// Decoupled look-back for one digit (or class) d of tile `tile`: publishes
+// the tile's count `tot` in status[tile * 256 + d] and returns the digit's
+// total over all earlier tiles, reading kLookbackWidth earlier tiles' words
+// per round (their loads' latencies overlap) until one carries an inclusive
+// prefix.  A tile publishes before it looks back and looks back only at
+// earlier tiles, which are running or done, so every wait ends; waits are
+// also bounded, and an expired one sets *error (the prefix is then short).
+MSPLAT_DEV uint32_t digit_lookback(uint32_t *status, int64_t tile, int d, uint32_t tot, uint32_t *error) {
+  uint32_t *st = status + tile * 256;
+  uint32_t prefix = 0;
+  if (tile == 0) {
+    __hip_atomic_store(st + d, kFlagIncl | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return 0u;
+  }
+  __hip_atomic_store(st + d, kFlagAgg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  bool found = false;
+  if (expiry_forced()) {  // tests: give up without looking
+    atomicOr(error, 1u);
+    found = true;
+  }
+  for (int64_t t = tile - 1; t >= 0 && !found; t -= kLookbackWidth) {
+    uint32_t w[kLookbackWidth];
+#pragma unroll
+    for (int q = 0; q < kLookbackWidth; ++q)
+      w[q] = t - q >= 0 ? __hip_atomic_load(status + (t - q) * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : kFlagIncl;
+#pragma unroll
+    for (int q = 0; q < kLookbackWidth; ++q) {
+      if (found) break;
+      if (t - q < 0) {  // before the first tile: nothing more to add
+        found = true;
+        break;
+      }
+      int64_t spins = 0;
+      while ((w[q] & ~kValueMask) == 0u && ++spins < kLookbackSpins)
+        w[q] = __hip_atomic_load(status + (t - q) * 256 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((w[q] & ~kValueMask) == 0u) {  // never published: give up, flag it
+        atomicOr(error, 1u);
+        found = true;
+        break;
+      }
+      prefix += w[q] & kValueMask;
+      if ((w[q] & ~kValueMask) == kFlagIncl) found = true;
+    }
+  }
+  __hip_atomic_store(st + d, kFlagIncl | (prefix + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return prefix;
+}
+
 template <bool kOnesweep, int kScatterThreads>
 __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t cap,
@@ -338,50 +387,7 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
     // this tile's first slot per digit = digits below it in the whole pass
     // (offsets holds the pass's digit totals) + this digit in earlier tiles
     const uint32_t doff = uint32_t(block_excl_scan<kScatterThreads>(tid < ndig ? int64_t(offsets[tid]) : 0, &dummy));
-    if (tid < ndig) {
-      uint32_t *st = status + tile * 256;
-      uint32_t prefix = 0;
-      if (tile == 0) {
-        __hip_atomic_store(st + tid, kFlagIncl | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        __hip_atomic_store(st + tid, kFlagAgg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // Look back kLookbackWidth tiles per round: their words are loaded
-        // together (the loads' latencies overlap), then consumed in order.
-        bool found = false;
-        if (expiry_forced()) {  // tests: give up without looking
-          atomicOr(error, 1u);
-          found = true;
-        }
-        for (int64_t t = tile - 1; t >= 0 && !found; t -= kLookbackWidth) {
-          uint32_t w[kLookbackWidth];
-#pragma unroll
-          for (int q = 0; q < kLookbackWidth; ++q)
-            w[q] = t - q >= 0 ? __hip_atomic_load(status + (t - q) * 256 + tid, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT)
-                              : kFlagIncl;
-#pragma unroll
-          for (int q = 0; q < kLookbackWidth; ++q) {
-            if (found) break;
-            if (t - q < 0) {  // before the first tile: nothing more to add
-              found = true;
-              break;
-            }
-            int64_t spins = 0;
-            while ((w[q] & ~kValueMask) == 0u && ++spins < kLookbackSpins)
-              w[q] = __hip_atomic_load(status + (t - q) * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((w[q] & ~kValueMask) == 0u) {  // never published: give up, flag it
-              atomicOr(error, 1u);
-              found = true;
-              break;
-            }
-            prefix += w[q] & kValueMask;
-            if ((w[q] & ~kValueMask) == kFlagIncl) found = true;
-          }
-        }
-        __hip_atomic_store(st + tid, kFlagIncl | (prefix + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      s_gstart[tid] = doff + prefix;
-    }
+    if (tid < ndig) s_gstart[tid] = doff + digit_lookback(status, tile, tid, tot, error);
   }
   const uint32_t lstart = uint32_t(block_excl_scan<kScatterThreads>(tid < ndig ? int64_t(tot) : 0, &dummy));
   if (tid < ndig) {
@@ -639,19 +645,34 @@ MSPLAT_HD int64_t prep_blocks(int64_t n) {
 // loads of all its rounds are in flight together, and a few hundred
 // workgroups (all resident at once) keep the look-back short.  Ranks follow
 // row order: round, then wave, then lane.
+// With ctotal (at most kFusedCells cells), it also adds every cell's entry
+// count to ctotal[k * kFusedCells + cell] (zero before the launch), for the
+// fused emission: kCtSlices slices, workgroup b adding to slice b mod
+// kCtSlices, so that the workgroups' atomics on one cell spread over several
+// addresses (the emission adds the slices up).
+constexpr int kFusedCells = 256;
+constexpr int kCtSlices = 8;
+
 __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
     const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius, int64_t n,
     Views v, int S, uint2 *__restrict__ rect, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
     uint64_t *__restrict__ status, int64_t *__restrict__ nlisted, int64_t *__restrict__ count,
-    uint32_t *__restrict__ error) {
+    uint32_t *__restrict__ error, uint32_t *__restrict__ ctotal) {
+  static_assert(kThreads == kFusedCells, "one thread per cell adds the cell counts");
   constexpr int kWaves = kThreads / kWave;
   __shared__ int64_t s_base;
   __shared__ uint32_t s_cnt[kPrepRows][kWaves];
   __shared__ int64_t s_area[kWaves];
+  __shared__ uint32_t s_ct[kFusedCells];
+  if (ctotal) {  // uniform
+    s_ct[threadIdx.x] = 0u;
+    __syncthreads();
+  }
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const int64_t base = int64_t(blockIdx.x) * kThreads * kPrepRows + threadIdx.x;
   int a[kPrepRows];
   uint32_t db[kPrepRows];
+  uint32_t cr[kPrepRows];  // fused emission: the row's cells (first cell, width - 1, height - 1, ncx - 1)
   uint64_t m[kPrepRows];
   int64_t my_area = 0;
 #pragma unroll
@@ -659,6 +680,7 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
     const int64_t i = base + int64_t(it) * kThreads;
     a[it] = 0;
     db[it] = 0u;
+    cr[it] = 0u;
     if (i < n) {
       const int b = view_of_row(v, i);
       const int r = radius[i];
@@ -666,13 +688,30 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
       TileRect t{0, 0, 0, 0};
       if (r > 0 && !(db[it] >> 31)) {
         t = tile_rect(uv[2 * i], uv[2 * i + 1], r, v.gx[b], v.gy[b]);
-        if (t.area() > 0) a[it] = rect_cells(pack_rect(t), S);
+        if (t.area() > 0) {
+          a[it] = rect_cells(pack_rect(t), S);
+          if (ctotal) {  // the rectangle of cells, packed; counted after the loop
+            const int cx0 = t.x0 / S, cx1 = (t.x1 + S - 1) / S, cy0 = t.y0 / S, cy1 = (t.y1 + S - 1) / S;
+            cr[it] = uint32_t(v.cell0[b] + cy0 * v.ncx[b] + cx0) | (uint32_t(cx1 - cx0 - 1) << 8) |
+                     (uint32_t(cy1 - cy0 - 1) << 16) | (uint32_t(v.ncx[b] - 1) << 24);
+          }
+        }
       }
       rect[i] = pack_rect(t);
     }
     m[it] = __ballot(a[it] > 0);
     my_area += a[it];
     if (lane == 0) s_cnt[it][wave] = uint32_t(__popcll(m[it]));
+  }
+  if (ctotal) {  // uniform: the rows' cells, after the loop so that its loads stay in flight together
+#pragma unroll
+    for (int it = 0; it < kPrepRows; ++it) {
+      if (a[it] <= 0) continue;
+      const int c0 = int(cr[it] & 0xffu), w = int((cr[it] >> 8) & 0xffu) + 1, h = int((cr[it] >> 16) & 0xffu) + 1;
+      const int ncx = int(cr[it] >> 24) + 1;
+      for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) atomicAdd(&s_ct[c0 + y * ncx + x], 1u);
+    }
   }
   my_area = wave_sum_i64(my_area);
   if (lane == 0) s_area[wave] = my_area;
@@ -713,6 +752,8 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
       vals[j] = uint32_t(base + int64_t(it) * kThreads);
     }
   }
+  if (ctotal && s_ct[threadIdx.x])  // s_ct is final: the barrier above
+    atomicAdd(&ctotal[(blockIdx.x % kCtSlices) * kFusedCells + threadIdx.x], s_ct[threadIdx.x]);
 }
 
 // Cell binning, emission (one launch): row i of the depth order writes its
@@ -796,6 +837,131 @@ __global__ __launch_bounds__(kEmitThreads) void emit_direct_kernel(
       const int q = e / sw, r = e - q * sw;
       cls[so + e] = sb ? sb + uint32_t(q * sn + r) : 0u;
       gid[so + e] = sg;
+    }
+  }
+}
+
+// Cell binning, emission straight into the cell lists (one launch), when
+// there are at most kFusedCells cells: row i of the depth order writes its
+// Gaussian into each of its cells' lists at
+//   start[c] + (entries of c from earlier workgroups) + (from earlier waves
+//   of this workgroup) + (from earlier lanes of this wave),
+// the depth-ordered (stable) position the emission + class sort would give it
+// (a row holds a cell once).  A wave records which of its lanes hold each
+// cell as a 64-bit mask in LDS; the workgroup's per-cell counts are summed
+// over its waves and looked back over the earlier workgroups one cell per
+// thread (workgroups take tiles in ticket order, so every predecessor is
+// running or done); start[c] is the exclusive scan of the totals the
+// counting pass added up.  The first workgroup writes the cell ranges
+// (clipped to the capacity; entries at or past it are not written).
+__global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
+    const uint32_t *__restrict__ order, const int32_t *__restrict__ area_s, const uint2 *__restrict__ rect_s,
+    int64_t n, const int64_t *__restrict__ nrows, int64_t cap, Views v, int S, int ncls,
+    const uint32_t *__restrict__ ctotal, uint32_t *__restrict__ ticket, uint32_t *__restrict__ status,
+    int32_t *__restrict__ cell_ids, int32_t *__restrict__ cell_range, uint32_t *__restrict__ error,
+    const uint32_t *__restrict__ upstream) {
+  constexpr int kWaves = kEmitThreads / kWave;
+  constexpr int kSmall = 16;  // larger boxes are handled by the whole wave
+  __shared__ uint64_t s_mask[kWaves][kFusedCells];
+  __shared__ uint32_t s_wbase[kWaves][kFusedCells];
+  __shared__ uint32_t s_gbase[kFusedCells];
+  __shared__ uint32_t s_tile;
+  // An expired wait of the counting pass or the depth sort left the depth
+  // order undefined: nothing is read, the lists stay empty (zero ranges),
+  // and the device's expired-wait word makes the blending write NaN.
+  if (*upstream) {  // uniform: the word is final before this launch
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&wait_expired, 1u);
+    return;
+  }
+  const int64_t rows = dev_len(nrows, n);
+  if (int64_t(blockIdx.x) * kEmitThreads >= rows) return;  // uniform; no ticket taken
+  const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  for (int j = tid; j < kWaves * kFusedCells; j += kEmitThreads) s_mask[j / kFusedCells][j % kFusedCells] = 0ull;
+  __syncthreads();
+  const int64_t blk = s_tile;
+  const int64_t i = blk * kEmitThreads + tid;
+  const int a = i < rows ? area_s[i] : 0;
+  uint32_t g = 0u;
+  int w = 1, ncx = 1, base = 0;  // entry e of the row is cell base + (e / w) ncx + e % w
+  if (a > 0) {
+    g = order[i];
+    const int b = view_of_row(v, g);
+    const uint2 rp = rect_s[i];
+    const int x0 = int(rp.x & 0xffffu), y0 = int(rp.x >> 16), x1 = int(rp.y & 0xffffu);
+    const int cx0 = x0 / S;
+    w = imax(1, (x1 + S - 1) / S - cx0);
+    ncx = v.ncx[b];
+    base = v.cell0[b] + (y0 / S) * ncx + cx0;
+  }
+  auto cell_of = [](int e, int bs, int ws, int ns) {
+    const int q = e / ws;
+    return bs + q * ns + (e - q * ws);
+  };
+  // which lanes of the wave hold each cell
+  if (a > 0 && a <= kSmall)
+    for (int e = 0; e < a; ++e) {
+      const int c = cell_of(e, base, w, ncx);
+      if (c < ncls) atomicOr(&s_mask[wave][c], 1ull << lane);
+    }
+  const uint64_t big = __ballot(a > kSmall);
+  for (uint64_t rem = big; rem;) {
+    const int src = __ffsll((unsigned long long)rem) - 1;
+    rem &= rem - 1;
+    const int sa = __shfl(a, src, kWave), sw = __shfl(w, src, kWave), sn = __shfl(ncx, src, kWave);
+    const int sb = __shfl(base, src, kWave);
+    for (int e = lane; e < sa; e += kWave) {
+      const int c = cell_of(e, sb, sw, sn);
+      if (c < ncls) atomicOr(&s_mask[wave][c], 1ull << src);
+    }
+  }
+  __syncthreads();
+  // per cell: the waves' offsets and the workgroup's count
+  uint32_t tot = 0u;
+  if (tid < ncls) {
+#pragma unroll
+    for (int wv = 0; wv < kWaves; ++wv) {
+      s_wbase[wv][tid] = tot;
+      tot += uint32_t(__popcll(s_mask[wv][tid]));
+    }
+  }
+  int64_t dummy;
+  int64_t ct = 0;  // entries of cell tid, over the counting pass's slices
+  if (tid < ncls)
+#pragma unroll
+    for (int k = 0; k < kCtSlices; ++k) ct += ctotal[k * kFusedCells + tid];
+  const int64_t start = block_excl_scan<kEmitThreads>(ct, &dummy);
+  if (tid < ncls) {
+    s_gbase[tid] = uint32_t(start) + digit_lookback(status, blk, tid, tot, error);
+    if (blk == 0) {
+      const int64_t e = start + ct;
+      cell_range[2 * tid] = int32_t(start < cap ? start : cap);
+      cell_range[2 * tid + 1] = int32_t(e < cap ? e : cap);
+    }
+  }
+  __syncthreads();
+  // an expired look-back (this workgroup's, or an earlier one's): the lists are undefined
+  if (tid == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicOr(&wait_expired, 1u);
+  const uint64_t below = (1ull << lane) - 1ull;
+  if (a > 0 && a <= kSmall)
+    for (int e = 0; e < a; ++e) {
+      const int c = cell_of(e, base, w, ncx);
+      if (c >= ncls) continue;
+      const int64_t pos = int64_t(s_gbase[c]) + s_wbase[wave][c] + __popcll(s_mask[wave][c] & below);
+      if (pos < cap) cell_ids[pos] = int32_t(g);
+    }
+  for (uint64_t rem = big; rem;) {
+    const int src = __ffsll((unsigned long long)rem) - 1;
+    rem &= rem - 1;
+    const int sa = __shfl(a, src, kWave), sw = __shfl(w, src, kWave), sn = __shfl(ncx, src, kWave);
+    const int sb = __shfl(base, src, kWave);
+    const uint32_t sg = uint32_t(__shfl(int(g), src, kWave));
+    const uint64_t sbelow = (1ull << src) - 1ull;
+    for (int e = lane; e < sa; e += kWave) {
+      const int c = cell_of(e, sb, sw, sn);
+      if (c >= ncls) continue;
+      const int64_t pos = int64_t(s_gbase[c]) + s_wbase[wave][c] + __popcll(s_mask[wave][c] & sbelow);
+      if (pos < cap) cell_ids[pos] = int32_t(sg);
     }
   }
 }

@@ -83,7 +83,7 @@ KERNEL_PATTERNS = {
     "preprocess_fwd": r"preprocess_fwd_\w*kernel",
     "preprocess_bwd": r"preprocess_bwd_\w*kernel",
     "bin_prepare": r"bin_prepare_kernel",
-    "emit": r"emit_(chunk|direct)_kernel",
+    "emit": r"emit_(chunk|direct|cells)_kernel",
     # both radix sorts (depth order and cell class) launch the same kernels
     "radix_scatter": r"radix_scatter_tile_kernel",
     "radix_hist": r"(onesweep|radix)_hist_kernel",

@@ -40,8 +40,11 @@ def _check_bins(uv, depth, radius, W, H, cell):
 
 
 @pytest.mark.parametrize("seed,n,W,H,cell", [(0, 3000, 128, 96, 1), (1, 5000, 200, 120, 3),
-                                              (2, 4000, 256, 256, 8), (3, 2000, 100, 70, 64)])
+                                              (2, 4000, 256, 256, 8), (3, 2000, 100, 70, 64),
+                                              (5, 6000, 640, 480, 1)])
 def test_bins_bitexact(seed, n, W, H, cell):
+    """Up to 256 cells the emission writes the lists directly (per-cell look-back);
+    640 x 480 with cells of one tile has 1200 cells: emission + class sort."""
     _, uv, depth, _, radius, _ = _projected(seed, n, W, H)
     _check_bins(uv, depth, radius, W, H, cell)
 
