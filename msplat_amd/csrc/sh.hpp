@@ -183,6 +183,8 @@ struct ShDirArray {
     grad[3 * i + 2] = gz;
   }
   MSPLAT_DEV void store_zero(int64_t i) const { store_grad(i, 0.f, 0.f, 0.f); }
+  // a Gaussian whose upstream gradient is zero in every channel
+  MSPLAT_DEV void store_unreached(int64_t i) const { store_zero(i); }
 };
 
 // View directions from the Gaussian centres and one camera centre,
@@ -214,8 +216,15 @@ struct ShDirFromPoints {
     put(i, (gx - x * dot) / r, (gy - y * dot) / r, (gz - z * dot) / r);
   }
   MSPLAT_DEV void store_zero(int64_t i) const { put(i, 0.f, 0.f, 0.f); }
+  // A Gaussian whose upstream gradient is zero in every channel: its
+  // direction gradient is zero (for finite directions), which added to a
+  // given gradient leaves it unchanged (a -0 stays -0), so it is not read
+  // or written.
+  MSPLAT_DEV void store_unreached(int64_t i) const {
+    if (!add) put(i, 0.f, 0.f, 0.f);
+  }
   MSPLAT_DEV void put(int64_t i, float gx, float gy, float gz) const {
-    if (add) {  // uniform; a zero is added too (-0 + 0 = +0, as a separate addition gives)
+    if (add) {  // uniform
       gx = grad[3 * i] + gx;
       gy = grad[3 * i + 1] + gy;
       gz = grad[3 * i + 2] + gz;
@@ -463,9 +472,15 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
       sh_store_row<D>(dshs + t * D, r);
     }
   };
+  // A zero upstream gradient (most rows: Gaussians nobody sees) gives a zero
+  // dL/dshs row (Y_k 0) and adds nothing to dL/ddir: neither the direction
+  // nor the coefficients are read and the basis is not evaluated (for a
+  // finite direction and coefficients the results are the same).
+  const bool vis_i = active && !(vis && !vis[i]);
+  const float gc = vis_i ? dval[i * ld + c] : 0.f;
   if (active) {
     const int64_t t = i * C + c;
-    if (vis && !vis[i]) {
+    if (gc == 0.f) {
       float z[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) z[k] = 0.f;
@@ -473,17 +488,8 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
     } else {
       float x, y, zz;
       dir.load(i, x, y, zz);
-      const float gc = dval[i * ld + c];
-      // a zero upstream gradient (most rows: Gaussians nobody sees) adds
-      // nothing to dL/ddir, so its coefficients are not read (for finite
-      // coefficients the sum is unchanged; dL/dshs is still Y_k * 0)
       float r[D];
-      if (gc != 0.f) {
-        sh_load_row<D>(shs + t * D, r);
-      } else {
-#pragma unroll
-        for (int k = 0; k < D; ++k) r[k] = 0.f;
-      }
+      sh_load_row<D>(shs + t * D, r);
       float Y[D];
       sh_basis<L>(x, y, zz, [&](int k, float v, float dx, float dy, float dz) {
         Y[k] = v * gc;
@@ -510,18 +516,20 @@ __global__ __launch_bounds__(256) void sh_bwd_gc_kernel(int64_t n, int C, const 
   }
   // channel partials of one Gaussian sit in lanes j*C .. j*C + C - 1
   float sx = gx, sy = gy, sz = gz;
+  bool reached = gc != 0.f;
   for (int cc = 1; cc < C; ++cc) {
     const int src = imin(lane + cc, kWave - 1);
     const float ox = __shfl(gx, src, kWave), oy = __shfl(gy, src, kWave), oz = __shfl(gz, src, kWave);
     sx += ox;
     sy += oy;
     sz += oz;
+    reached = reached || (__shfl(gc, src, kWave) != 0.f && lane + cc < kWave);
   }
   if (active && c == 0) {
-    if (vis && !vis[i])
-      dir.store_zero(i);
-    else
+    if (reached)
       dir.store_grad(i, sx, sy, sz);
+    else  // invisible, or no channel has an upstream gradient
+      dir.store_unreached(i);
   }
 }
 
