@@ -129,6 +129,11 @@ MSPLAT_DEV int view_of_row(const Views &v, int64_t i) {
 
 MSPLAT_DEV int lane_id() { return __lane_id(); }
 
+// Workgroup barrier for LDS traffic only: each wave waits for its own LDS
+// operations, then for the other waves; its global loads stay in flight
+// across it (__syncthreads' fence would wait for them too).
+MSPLAT_DEV void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 MSPLAT_DEV float wave_sum(float v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);

@@ -305,12 +305,6 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
   // Exactly ceil(len / tile) workgroups take tickets, so every ticket names a
   // tile with data, and no later tile waits on a workgroup that left.
   if (tile * kRadixTile >= len) return;
-  if constexpr (kOnesweep) {
-    __shared__ uint32_t s_ticket;
-    if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1u);
-    __syncthreads();
-    tile = s_ticket;
-  }
   constexpr int kScatterItems = kRadixTile / kScatterThreads;  // keys per lane
   constexpr int kWaves = kScatterThreads / kWave;
   constexpr int kPerWave = kRadixTile / kWaves;  // contiguous keys per wave
@@ -319,12 +313,18 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
   __shared__ uint32_t s_run[kWaves][256];  // per-wave running digit counts, then wave offsets
   __shared__ uint32_t s_lstart[256];       // tile-local first slot of each digit
   __shared__ uint32_t s_gstart[256];       // global first slot of each digit for this tile
+  __shared__ uint32_t s_ticket;
   const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const uint32_t mask = (1u << bits) - 1u;
   const int ndig = 1 << bits;
+  // onesweep: the pass's digit totals do not depend on the tile; their load
+  // stays in flight until the look-back (the barriers before it order LDS only)
+  const uint32_t gtot = (kOnesweep && tid < ndig) ? offsets[tid] : 0u;
+  if (kOnesweep && tid == 0) s_ticket = atomicAdd(ticket, 1u);
   for (int j = tid; j < kWaves * 256; j += kScatterThreads) s_run[j / 256][j % 256] = 0u;
   if (!kOnesweep && tid < ndig) s_gstart[tid] = offsets[int64_t(tid) * ntiles + tile];
-  __syncthreads();
+  lds_barrier();
+  if constexpr (kOnesweep) tile = s_ticket;
 
   const int64_t base = tile * kRadixTile + int64_t(wave) * kPerWave;
   uint32_t key[kScatterItems], val[kScatterItems], rnk[kScatterItems];
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
   if constexpr (kOnesweep) {
     // this tile's first slot per digit = digits below it in the whole pass
     // (offsets holds the pass's digit totals) + this digit in earlier tiles
-    const uint32_t doff = uint32_t(block_excl_scan<kScatterThreads>(tid < ndig ? int64_t(offsets[tid]) : 0, &dummy));
+    const uint32_t doff = uint32_t(block_excl_scan<kScatterThreads>(int64_t(gtot), &dummy));
     if (tid < ndig) s_gstart[tid] = doff + digit_lookback(status, tile, tid, tot, error);
   }
   const uint32_t lstart = uint32_t(block_excl_scan<kScatterThreads>(tid < ndig ? int64_t(tot) : 0, &dummy));
@@ -457,12 +457,22 @@ __global__ __launch_bounds__(kThreads) void onesweep_hist_kernel(const uint32_t 
   __syncthreads();
   const int64_t len = dev_len(dlen, cap);
   const int64_t stride = int64_t(gridDim.x) * kThreads;
-  for (int64_t i = int64_t(blockIdx.x) * kThreads + tid; i < len; i += stride) {
-    const uint32_t k = keys[i];
-    for (int p = 0; p < npass; ++p) {
-      const int sh = p * db;
-      const int pb = bits - sh < db ? bits - sh : db;
-      atomicAdd(&h[p][(k >> sh) & ((1u << pb) - 1u)], 1u);
+  // chunks of kRadixTile keys, all of a chunk's loads in flight together
+  for (int64_t c0 = int64_t(blockIdx.x) * kRadixTile; c0 < len; c0 += int64_t(gridDim.x) * kRadixTile) {
+    uint32_t k[kItems];
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+      const int64_t i = c0 + it * kThreads + tid;
+      k[it] = i < len ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+      if (c0 + it * kThreads + tid >= len) break;
+      for (int p = 0; p < npass; ++p) {
+        const int sh = p * db;
+        const int pb = bits - sh < db ? bits - sh : db;
+        atomicAdd(&h[p][(k[it] >> sh) & ((1u << pb) - 1u)], 1u);
+      }
     }
   }
   __syncthreads();

@@ -61,6 +61,8 @@ def active() -> "StageTimer | None":
 #   steps DIR PATTERN         per step (a step ends with a kernel whose name matches
 #                             PATTERN): wall time on the device, busy time (sum of kernel
 #                             durations), idle share and launches
+#   counters DIR              per-launch counter averages of one `rocprofv3 --pmc ...`
+#                             run and the utilisation figures they give
 #   pmc FETCH_DIR WRITE_DIR MIB
 #                             HBM bytes per launch of the library's kernels from two
 #                             `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of
@@ -160,9 +162,46 @@ def summarize_pmc(fetch_dir, write_dir, mib):
     return "\n".join(lines)
 
 
+# MI355X: 256 CUs (4 SIMDs each) in 8 XCDs.  GRBM_GUI_ACTIVE, summed over the
+# dispatch's dimensions, counts each XCD's busy cycles, so / 8 is the
+# kernel's own cycle count.
+CUS, SIMDS, XCDS = 256, 1024, 8
+
+
+def _derived(c):
+    """Utilisation figures from one kernel's per-launch counter averages
+    (only those whose counters the run holds)."""
+    out = []
+    cyc = c.get("GRBM_GUI_ACTIVE", 0.0) / XCDS
+    if cyc > 0:
+        out.append(("cycles", cyc))
+        if "SQ_ACTIVE_INST_VALU" in c:  # 4 cycles per wave64 VALU issue on one SIMD
+            out.append(("valu_busy", c["SQ_ACTIVE_INST_VALU"] * 4 / SIMDS / cyc))
+        if "SQ_LDS_IDX_ACTIVE" in c:  # cycles the CU's LDS is busy with indexed ops
+            out.append(("lds_util", c["SQ_LDS_IDX_ACTIVE"] / CUS / cyc))
+    if c.get("SQ_LDS_IDX_ACTIVE", 0) > 0 and "SQ_LDS_BANK_CONFLICT" in c:
+        out.append(("lds_bank_conflict", c["SQ_LDS_BANK_CONFLICT"] / c["SQ_LDS_IDX_ACTIVE"]))
+    if c.get("SQ_WAVE_CYCLES", 0) > 0:
+        for k, lab in (("SQ_WAIT_ANY", "wait_any"), ("SQ_WAIT_INST_LDS", "wait_lds"),
+                       ("SQ_ACTIVE_INST_ANY", "issue_any")):
+            if k in c:
+                out.append((lab, c[k] / c["SQ_WAVE_CYCLES"]))
+    return out
+
+
 def summarize_counters(d):
     """Per-launch averages of every counter of one PMC run, for the kernels of
-    KERNEL_PATTERNS (counters summed over a dispatch's dimensions first)."""
+    KERNEL_PATTERNS (counters summed over a dispatch's dimensions first), then
+    the utilisation figures they give:
+
+      valu_busy         SQ_ACTIVE_INST_VALU * 4 / (1024 SIMDs * cycles)
+      lds_util          SQ_LDS_IDX_ACTIVE / (256 CUs * cycles)
+      lds_bank_conflict SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE
+      wait_any, wait_lds, issue_any
+                        SQ_WAIT_ANY, SQ_WAIT_INST_LDS, SQ_ACTIVE_INST_ANY per
+                        SQ_WAVE_CYCLES (shares of the waves' resident cycles)
+
+    with cycles = GRBM_GUI_ACTIVE / 8 XCDs."""
     per = {}
     names = []
     for r in _rows(d, "counter_collection.csv"):
@@ -173,6 +212,7 @@ def summarize_counters(d):
         name, v = per.get(key, (r["Kernel_Name"], 0.0))
         per[key] = (name, v + float(r["Counter_Value"]))
     lines = [f"{'kernel':16s} {'launches':>8s} " + " ".join(f"{c:>20s}" for c in names)]
+    derived = []
     for k, pat in KERNEL_PATTERNS.items():
         cols, n = [], 0
         for c in names:
@@ -181,6 +221,11 @@ def summarize_counters(d):
             cols.append(sum(vals) / len(vals) if vals else float("nan"))
         if n:
             lines.append(f"{k:16s} {n:8d} " + " ".join(f"{v:20.4g}" for v in cols))
+            dv = _derived(dict(zip(names, cols)))
+            if dv:
+                derived.append(f"{k:16s} " + "  ".join(f"{lab} {v:.4g}" for lab, v in dv))
+    if derived:
+        lines += [""] + derived
     return "\n".join(lines)
 
 

