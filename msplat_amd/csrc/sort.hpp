@@ -864,6 +864,26 @@ __global__ __launch_bounds__(kEmitThreads) void emit_direct_kernel(
 // running or done); start[c] is the exclusive scan of the totals the
 // counting pass added up.  The first workgroup writes the cell ranges
 // (clipped to the capacity; entries at or past it are not written).
+// Cells [lo, hi) of a 256-cell set as 4 words of 64 bits, or-ed into bits
+// (words at or past nwords are left alone).
+MSPLAT_DEV void cell_range_bits(int lo, int hi, int nwords, uint64_t (&bits)[kFusedCells / 64]) {
+#pragma unroll
+  for (int k = 0; k < kFusedCells / 64; ++k) {
+    if (k >= nwords) break;  // uniform
+    const int a = imax(lo, 64 * k), b = imin(hi, 64 * k + 64);
+    if (a < b) {
+      const int len = b - a;
+      bits[k] |= (len == 64 ? ~0ull : (1ull << len) - 1ull) << (a - 64 * k);
+    }
+  }
+}
+
+MSPLAT_DEV uint64_t wave_or_u64(uint64_t v) {
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) v |= __shfl_xor(v, off, kWave);
+  return v;
+}
+
 __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
     const uint32_t *__restrict__ order, const int32_t *__restrict__ area_s, const uint2 *__restrict__ rect_s,
     int64_t n, const int64_t *__restrict__ nrows, int64_t cap, Views v, int S, int ncls,
@@ -871,7 +891,14 @@ __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
     int32_t *__restrict__ cell_ids, int32_t *__restrict__ cell_range, uint32_t *__restrict__ error,
     const uint32_t *__restrict__ upstream) {
   constexpr int kWaves = kEmitThreads / kWave;
-  constexpr int kSmall = 16;  // larger boxes are handled by the whole wave
+  constexpr int kWords = kFusedCells / 64;
+  // Rows of more than kSmall cells (the near, large Gaussians at the front of
+  // the depth order) are marked by cell, not by row: the wave walks the union
+  // of their cells and sets each cell's lanes with one ballot and one LDS
+  // atomic, where a per-row walk would issue an atomic per row and cell, most
+  // of them on the same few words.  (The 1080p frame's first workgroup took
+  // 13 us to mark its rows by row, the others about 3 us.)
+  constexpr int kSmall = 16;
   __shared__ uint64_t s_mask[kWaves][kFusedCells];
   __shared__ uint32_t s_wbase[kWaves][kFusedCells];
   __shared__ uint32_t s_gbase[kFusedCells];
@@ -893,7 +920,7 @@ __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
   const int64_t i = blk * kEmitThreads + tid;
   const int a = i < rows ? area_s[i] : 0;
   uint32_t g = 0u;
-  int w = 1, ncx = 1, base = 0;  // entry e of the row is cell base + (e / w) ncx + e % w
+  int w = 1, h = 0, ncx = 1, base = 0;  // the row's cells: base + y ncx + x, x < w, y < h
   if (a > 0) {
     g = order[i];
     const int b = view_of_row(v, g);
@@ -901,28 +928,42 @@ __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
     const int x0 = int(rp.x & 0xffffu), y0 = int(rp.x >> 16), x1 = int(rp.y & 0xffffu);
     const int cx0 = x0 / S;
     w = imax(1, (x1 + S - 1) / S - cx0);
+    h = a / w;  // a = w h
     ncx = v.ncx[b];
     base = v.cell0[b] + (y0 / S) * ncx + cx0;
   }
-  auto cell_of = [](int e, int bs, int ws, int ns) {
-    const int q = e / ws;
-    return bs + q * ns + (e - q * ws);
-  };
-  // which lanes of the wave hold each cell
-  if (a > 0 && a <= kSmall)
-    for (int e = 0; e < a; ++e) {
-      const int c = cell_of(e, base, w, ncx);
-      if (c < ncls) atomicOr(&s_mask[wave][c], 1ull << lane);
-    }
-  const uint64_t big = __ballot(a > kSmall);
-  for (uint64_t rem = big; rem;) {
-    const int src = __ffsll((unsigned long long)rem) - 1;
-    rem &= rem - 1;
-    const int sa = __shfl(a, src, kWave), sw = __shfl(w, src, kWave), sn = __shfl(ncx, src, kWave);
-    const int sb = __shfl(base, src, kWave);
-    for (int e = lane; e < sa; e += kWave) {
-      const int c = cell_of(e, sb, sw, sn);
-      if (c < ncls) atomicOr(&s_mask[wave][c], 1ull << src);
+  const bool big = a > kSmall;
+  const uint64_t below = (1ull << lane) - 1ull;
+  // which lanes of the wave hold each cell: small rows one atomic per cell
+  if (a > 0 && !big)
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < w; ++x) {
+        const int c = base + y * ncx + x;
+        if (c < ncls) atomicOr(&s_mask[wave][c], 1ull << lane);
+      }
+  // large rows by cell: their cells as bit sets, and the wave's union
+  uint64_t bits[kWords], uni[kWords];
+#pragma unroll
+  for (int k = 0; k < kWords; ++k) bits[k] = 0ull;
+  const bool any_big = __ballot(big) != 0ull;
+  const int nwords = (ncls + 63) / 64;
+  if (any_big) {  // uniform
+    if (big)
+      for (int y = 0; y < h; ++y) cell_range_bits(base + y * ncx, base + y * ncx + w, nwords, bits);
+#pragma unroll
+    for (int k = 0; k < kWords; ++k) {
+      uni[k] = 0ull;
+      if (k >= nwords) continue;  // uniform
+      const int vc = imin(imax(ncls - 64 * k, 0), 64);  // valid cells of word k
+      const uint64_t valid = vc == 64 ? ~0ull : (1ull << vc) - 1ull;
+      const uint64_t u = wave_or_u64(bits[k]) & valid;
+      uni[k] = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(u >> 32))) << 32) |
+               __builtin_amdgcn_readfirstlane(uint32_t(u));
+      for (uint64_t r = uni[k]; r; r &= r - 1) {
+        const int bit = __ffsll((unsigned long long)r) - 1;
+        const uint64_t m = __ballot((bits[k] >> bit) & 1ull);
+        if (lane == 0) atomicOr(&s_mask[wave][64 * k + bit], m);
+      }
     }
   }
   __syncthreads();
@@ -952,27 +993,27 @@ __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
   __syncthreads();
   // an expired look-back (this workgroup's, or an earlier one's): the lists are undefined
   if (tid == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicOr(&wait_expired, 1u);
-  const uint64_t below = (1ull << lane) - 1ull;
-  if (a > 0 && a <= kSmall)
-    for (int e = 0; e < a; ++e) {
-      const int c = cell_of(e, base, w, ncx);
-      if (c >= ncls) continue;
-      const int64_t pos = int64_t(s_gbase[c]) + s_wbase[wave][c] + __popcll(s_mask[wave][c] & below);
-      if (pos < cap) cell_ids[pos] = int32_t(g);
-    }
-  for (uint64_t rem = big; rem;) {
-    const int src = __ffsll((unsigned long long)rem) - 1;
-    rem &= rem - 1;
-    const int sa = __shfl(a, src, kWave), sw = __shfl(w, src, kWave), sn = __shfl(ncx, src, kWave);
-    const int sb = __shfl(base, src, kWave);
-    const uint32_t sg = uint32_t(__shfl(int(g), src, kWave));
-    const uint64_t sbelow = (1ull << src) - 1ull;
-    for (int e = lane; e < sa; e += kWave) {
-      const int c = cell_of(e, sb, sw, sn);
-      if (c >= ncls) continue;
-      const int64_t pos = int64_t(s_gbase[c]) + s_wbase[wave][c] + __popcll(s_mask[wave][c] & sbelow);
-      if (pos < cap) cell_ids[pos] = int32_t(sg);
-    }
+  auto slot = [&](int c) {
+    return int64_t(s_gbase[c]) + s_wbase[wave][c] + __popcll(s_mask[wave][c] & below);
+  };
+  if (a > 0 && !big)
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < w; ++x) {
+        const int c = base + y * ncx + x;
+        if (c >= ncls) continue;
+        const int64_t pos = slot(c);
+        if (pos < cap) cell_ids[pos] = int32_t(g);
+      }
+  if (any_big) {  // uniform: the large rows, cell by cell of the union
+#pragma unroll
+    for (int k = 0; k < kWords; ++k)
+      for (uint64_t r = uni[k]; r; r &= r - 1) {
+        const int bit = __ffsll((unsigned long long)r) - 1;
+        if ((bits[k] >> bit) & 1ull) {
+          const int64_t pos = slot(64 * k + bit);
+          if (pos < cap) cell_ids[pos] = int32_t(g);
+        }
+      }
   }
 }
 

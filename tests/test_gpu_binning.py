@@ -49,6 +49,29 @@ def test_bins_bitexact(seed, n, W, H, cell):
     _check_bins(uv, depth, radius, W, H, cell)
 
 
+@pytest.mark.parametrize("W,H,cell", [(128, 96, 1), (256, 192, 1), (240, 240, 1)])
+def test_bins_bitexact_large_boxes(W, H, cell):
+    """The emission marks rows of more than 16 cells by cell (the union of the
+    wave's large boxes, a 64-bit word per 64 cells): a scene whose first
+    Gaussians are large enough to cover dozens of cells, on grids of 48, 192
+    and 225 cells (one, three and four words), bit-exact with the oracle."""
+    g, cam = small_scene(11, n=3000, W=W, H=H, half_extent=1.5)
+    g["scale"][:400] = torch.exp(torch.empty(400, 3).uniform_(-2.0, -0.8, generator=torch.Generator().manual_seed(3)))
+    d = {k: v.to(DEV) for k, v in g.items()}
+    uv, depth, conic, radius, tiles = preprocess_forward(d["xyz"], d["scale"], d["rotate"], cam.intr.to(DEV),
+                                                         cam.extr.to(DEV), W, H)
+    rect, _, _ = osort.cell_lists(np_(uv), np_(depth), np_(radius), W, H, cell)
+    area = (rect[:, 2] - rect[:, 0]) * (rect[:, 3] - rect[:, 1])
+    gx, gy = (W + 15) // 16, (H + 15) // 16
+    big = (area > 16) & (np_(radius).reshape(-1) > 0)
+    last_cell = (rect[:, 3] - 1) * gx + rect[:, 2] - 1  # cell = tile with cells of one tile
+    beyond = int((big & (last_cell >= 64)).sum())
+    print(f"{W}x{H} cell {cell}: {int(big.sum())} rows of more than 16 cells, {beyond} reaching past the "
+          f"first 64 cells", flush=True)
+    assert big.sum() >= 200 and (gx * gy <= 64 or beyond >= 200)
+    _check_bins(uv, depth, radius, W, H, cell)
+
+
 def test_bins_bitexact_many_tiles():
     """2.2 M Gaussians: the depth sort's capacity spans more than 512 tiles of
     4096 keys, so its passes take the 4-wave scatter form (smaller inputs take
