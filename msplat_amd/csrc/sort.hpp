@@ -618,6 +618,58 @@ MSPLAT_DEV bool lookback_pair(const uint64_t *status, int64_t b, int64_t &hi, in
   }
 }
 
+// lookback_pair by a whole workgroup of NT threads (every thread calls it:
+// it has barriers): in a round, wave k reads the words of workgroups
+// r - 64 k - lane (r: the round's most recent), each wave sums down to its
+// nearest inclusive word, and the waves' sums are added in order through LDS
+// until one of them met an inclusive word; a round covers NT earlier
+// workgroups instead of 64.  Every thread returns the same sums.
+template <int NT>
+MSPLAT_DEV bool lookback_pair_wg(const uint64_t *status, int64_t b, int64_t &hi, int64_t &lo) {
+  constexpr int kW = NT / kWave;
+  __shared__ int64_t s_h[kW], s_l[kW];
+  __shared__ uint32_t s_f[kW];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  hi = 0;
+  lo = 0;
+  if (expiry_forced()) return false;  // tests: give up without looking
+  for (int64_t r = b - 1;; r -= NT) {
+    const int64_t q = r - int64_t(wave) * kWave - lane;  // before workgroup 0: an inclusive zero
+    uint64_t w = q >= 0 ? __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kLbIncl;
+    bool expired = false;
+    int64_t spins = 0;
+    while (__ballot((w & kLbFlags) == 0ull) != 0ull) {
+      if (++spins >= kLookbackSpins) {
+        expired = true;
+        break;
+      }
+      if ((w & kLbFlags) == 0ull) w = __hip_atomic_load(status + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint64_t incl = __ballot((w & kLbFlags) == kLbIncl);
+    const int stop = incl ? __ffsll((unsigned long long)incl) - 1 : kWave - 1;  // nearest inclusive word
+    const bool take = lane <= stop;
+    const int64_t wh = wave_sum_i64(take ? int64_t((w >> 31) & uint64_t(kLbField)) : 0);
+    const int64_t wl = wave_sum_i64(take ? int64_t(w & uint64_t(kLbField)) : 0);
+    if (lane == 0) {
+      s_h[wave] = wh;
+      s_l[wave] = wl;
+      s_f[wave] = (incl != 0ull ? 1u : 0u) | (expired ? 2u : 0u);
+    }
+    __syncthreads();
+    bool fin = false, ok = true;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) {
+      if (fin) break;
+      hi += s_h[k];
+      lo += s_l[k];
+      if (s_f[k] & 2u) ok = false;
+      if (s_f[k]) fin = true;
+    }
+    __syncthreads();  // the slots are reused by the next round
+    if (fin) return ok;
+  }
+}
+
 // Publishes workgroup b's sums (nh, nl) and returns its exclusive prefixes
 // (wave 0 calls it; lane 0 stores).  false if a wait expired.
 MSPLAT_DEV bool lookback_publish(uint64_t *status, int64_t b, int64_t nh, int64_t nl, int64_t &ph, int64_t &pl) {
@@ -685,44 +737,75 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
   uint32_t cr[kPrepRows];  // fused emission: the row's cells (first cell, width - 1, height - 1, ncx - 1)
   uint64_t m[kPrepRows];
   int64_t my_area = 0;
+  // Every row's inputs first, their loads in flight together; then the
+  // rectangles.  A workgroup's rows nearly always lie in one view, whose
+  // fields are then read once as uniform values rather than per lane from
+  // the view table (a dependent load per row).
+  int rr[kPrepRows];
+  float dd[kPrepRows], ux[kPrepRows], uy[kPrepRows];
 #pragma unroll
   for (int it = 0; it < kPrepRows; ++it) {
     const int64_t i = base + int64_t(it) * kThreads;
-    a[it] = 0;
-    db[it] = 0u;
-    cr[it] = 0u;
-    if (i < n) {
-      const int b = view_of_row(v, i);
-      const int r = radius[i];
-      db[it] = fbits(depth[i]);
-      TileRect t{0, 0, 0, 0};
-      if (r > 0 && !(db[it] >> 31)) {
-        t = tile_rect(uv[2 * i], uv[2 * i + 1], r, v.gx[b], v.gy[b]);
-        if (t.area() > 0) {
-          a[it] = rect_cells(pack_rect(t), S);
-          if (ctotal) {  // the rectangle of cells, packed; counted after the loop
-            const int cx0 = t.x0 / S, cx1 = (t.x1 + S - 1) / S, cy0 = t.y0 / S, cy1 = (t.y1 + S - 1) / S;
-            cr[it] = uint32_t(v.cell0[b] + cy0 * v.ncx[b] + cx0) | (uint32_t(cx1 - cx0 - 1) << 8) |
-                     (uint32_t(cy1 - cy0 - 1) << 16) | (uint32_t(v.ncx[b] - 1) << 24);
-          }
-        }
-      }
-      rect[i] = pack_rect(t);
-    }
-    m[it] = __ballot(a[it] > 0);
-    my_area += a[it];
-    if (lane == 0) s_cnt[it][wave] = uint32_t(__popcll(m[it]));
+    const bool in = i < n;
+    rr[it] = in ? radius[i] : 0;
+    dd[it] = in ? depth[i] : 0.f;
+    ux[it] = in ? uv[2 * i] : 0.f;
+    uy[it] = in ? uv[2 * i + 1] : 0.f;
   }
-  if (ctotal) {  // uniform: the rows' cells, after the loop so that its loads stay in flight together
+  const int64_t first = int64_t(blockIdx.x) * kThreads * kPrepRows;
+  const int64_t last_row = (first + kThreads * kPrepRows < n ? first + kThreads * kPrepRows : n) - 1;
+  const int vb0 = __builtin_amdgcn_readfirstlane(view_of_row(v, first));
+  const int vb1 = __builtin_amdgcn_readfirstlane(view_of_row(v, last_row));
+  // x / S for the tile coordinates (0 <= x <= 4096 + S): a float quotient
+  // (off by at most one) and one exact correction, in place of an integer
+  // division sequence per use
+  const float inv_s = 1.0f / float(S);
+  auto div_s = [&](int x) {
+    int q = int(float(x) * inv_s);
+    const int rem = x - q * S;
+    q += int(rem >= S) - int(rem < 0);
+    return q;
+  };
+  struct ViewGrid {
+    int gx, gy, cell0, ncx;
+  };
+  auto grid_of = [&](int b) { return ViewGrid{int(v.gx[b]), int(v.gy[b]), v.cell0[b], int(v.ncx[b])}; };
+  const ViewGrid g0 = grid_of(vb0);
+  auto rows = [&](auto one_view) {
 #pragma unroll
     for (int it = 0; it < kPrepRows; ++it) {
-      if (a[it] <= 0) continue;
-      const int c0 = int(cr[it] & 0xffu), w = int((cr[it] >> 8) & 0xffu) + 1, h = int((cr[it] >> 16) & 0xffu) + 1;
-      const int ncx = int(cr[it] >> 24) + 1;
-      for (int y = 0; y < h; ++y)
-        for (int x = 0; x < w; ++x) atomicAdd(&s_ct[c0 + y * ncx + x], 1u);
+      const int64_t i = base + int64_t(it) * kThreads;
+      a[it] = 0;
+      db[it] = 0u;
+      cr[it] = 0u;
+      if (i < n) {
+        const ViewGrid vg = decltype(one_view)::value ? g0 : grid_of(view_of_row(v, i));
+        const int r = rr[it];
+        db[it] = fbits(dd[it]);
+        TileRect t{0, 0, 0, 0};
+        if (r > 0 && !(db[it] >> 31)) {
+          t = tile_rect(ux[it], uy[it], r, vg.gx, vg.gy);
+          if (t.area() > 0) {
+            // the cells touched, as rect_cells(pack_rect(t), S) counts them
+            const int cx0 = div_s(t.x0), cx1 = div_s(t.x1 + S - 1), cy0 = div_s(t.y0), cy1 = div_s(t.y1 + S - 1);
+            a[it] = (cx1 - cx0) * (cy1 - cy0);
+            if (ctotal) {  // the rectangle of cells, packed; counted after the loop
+              cr[it] = uint32_t(vg.cell0 + cy0 * vg.ncx + cx0) | (uint32_t(cx1 - cx0 - 1) << 8) |
+                       (uint32_t(cy1 - cy0 - 1) << 16) | (uint32_t(vg.ncx - 1) << 24);
+            }
+          }
+        }
+        rect[i] = pack_rect(t);
+      }
+      m[it] = __ballot(a[it] > 0);
+      my_area += a[it];
+      if (lane == 0) s_cnt[it][wave] = uint32_t(__popcll(m[it]));
     }
-  }
+  };
+  if (vb0 == vb1)  // uniform
+    rows(std::true_type{});
+  else
+    rows(std::false_type{});
   my_area = wave_sum_i64(my_area);
   if (lane == 0) s_area[wave] = my_area;
   __syncthreads();
@@ -739,9 +822,30 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
   }
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) ne += s_area[w];
-  if (threadIdx.x < kWave) {
-    int64_t pl, pe;
-    const bool ok = lookback_publish(status, blockIdx.x, nl, ne, pl, pe);
+  // this workgroup's counts are published first, so that later workgroups'
+  // look-backs find them while this one adds up its rows' cells
+  const int64_t b = blockIdx.x;
+  if (b > 0 && threadIdx.x == 0)
+    __hip_atomic_store(status + b, lb_word(kLbAgg, nl, ne), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // the rows' cells
+  if (ctotal) {  // uniform
+#pragma unroll
+    for (int it = 0; it < kPrepRows; ++it) {
+      if (a[it] <= 0) continue;
+      const int c0 = int(cr[it] & 0xffu), w = int((cr[it] >> 8) & 0xffu) + 1, h = int((cr[it] >> 16) & 0xffu) + 1;
+      const int ncx = int(cr[it] >> 24) + 1;
+      for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) atomicAdd(&s_ct[c0 + y * ncx + x], 1u);
+    }
+  }
+  {
+    // the whole workgroup looks back (489 workgroups at 1 M rows: two rounds)
+    int64_t pl = 0, pe = 0;
+    bool ok = true;
+    if (b > 0) ok = lookback_pair_wg<kThreads>(status, b, pl, pe);
+    // an expired wait leaves the aggregate: later workgroups look past it
+    if (threadIdx.x == 0 && ok)
+      __hip_atomic_store(status + b, lb_word(kLbIncl, pl + nl, pe + ne), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x == 0) {
       s_base = pl;
       if (!ok) atomicOr(error, 1u);
