@@ -129,6 +129,21 @@ MSPLAT_DEV int view_of_row(const Views &v, int64_t i) {
 
 MSPLAT_DEV int lane_id() { return __lane_id(); }
 
+// A 64-bit value of the first active lane (uniform_u64) or of lane l
+// (lane_u64): readfirstlane and readlane return 32-bit ints, so each half is
+// taken as unsigned before the halves are joined (an int low half would be
+// sign-extended over the high one).
+MSPLAT_DEV uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(v)));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(v >> 32)));
+  return (uint64_t(hi) << 32) | uint64_t(lo);
+}
+MSPLAT_DEV uint64_t lane_u64(uint64_t v, int l) {
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(uint32_t(v), l));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(uint32_t(v >> 32), l));
+  return (uint64_t(hi) << 32) | uint64_t(lo);
+}
+
 // Workgroup barrier for LDS traffic only: each wave waits for its own LDS
 // operations, then for the other waves; its global loads stay in flight
 // across it (__syncthreads' fence would wait for them too).

@@ -996,13 +996,16 @@ __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
     const uint32_t *__restrict__ upstream) {
   constexpr int kWaves = kEmitThreads / kWave;
   constexpr int kWords = kFusedCells / 64;
-  // Rows of more than kSmall cells (the near, large Gaussians at the front of
-  // the depth order) are marked by cell, not by row: the wave walks the union
-  // of their cells and sets each cell's lanes with one ballot and one LDS
-  // atomic, where a per-row walk would issue an atomic per row and cell, most
-  // of them on the same few words.  (The 1080p frame's first workgroup took
-  // 13 us to mark its rows by row, the others about 3 us.)
+  // Rows of more than kSmall cells, and every row of a wave whose rows hold
+  // more than kHeavy cells in all (the near, large Gaussians at the front of
+  // the depth order), are marked by cell, not by row: the wave walks the union
+  // of their cells, gathers each cell's ballot into lane (cell mod 64) and
+  // adds them with one LDS atomic per 64 cells, where a per-row walk would
+  // issue an atomic per row and cell, most of them on the same few words.
+  // (The 1080p frame's first workgroups took 13 us to mark their rows by
+  // row, the others about 3 us.)
   constexpr int kSmall = 16;
+  constexpr int64_t kHeavy = 3 * kWave;
   __shared__ uint64_t s_mask[kWaves][kFusedCells];
   __shared__ uint32_t s_wbase[kWaves][kFusedCells];
   __shared__ uint32_t s_gbase[kFusedCells];
@@ -1036,7 +1039,8 @@ __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
     ncx = v.ncx[b];
     base = v.cell0[b] + (y0 / S) * ncx + cx0;
   }
-  const bool big = a > kSmall;
+  const bool heavy = __builtin_amdgcn_readfirstlane(uint32_t(wave_sum_i64(a) > kHeavy)) != 0u;
+  const bool big = heavy ? a > 0 : a > kSmall;
   const uint64_t below = (1ull << lane) - 1ull;
   // which lanes of the wave hold each cell: small rows one atomic per cell
   if (a > 0 && !big)
@@ -1061,13 +1065,15 @@ __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
       const int vc = imin(imax(ncls - 64 * k, 0), 64);  // valid cells of word k
       const uint64_t valid = vc == 64 ? ~0ull : (1ull << vc) - 1ull;
       const uint64_t u = wave_or_u64(bits[k]) & valid;
-      uni[k] = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(u >> 32))) << 32) |
-               __builtin_amdgcn_readfirstlane(uint32_t(u));
+      uni[k] = uniform_u64(u);
+      uint32_t mlo = 0u, mhi = 0u;  // lane j: the ballot of cell 64 k + j
       for (uint64_t r = uni[k]; r; r &= r - 1) {
         const int bit = __ffsll((unsigned long long)r) - 1;
         const uint64_t m = __ballot((bits[k] >> bit) & 1ull);
-        if (lane == 0) atomicOr(&s_mask[wave][64 * k + bit], m);
+        mlo = lane == bit ? uint32_t(m) : mlo;
+        mhi = lane == bit ? uint32_t(m >> 32) : mhi;
       }
+      if ((uni[k] >> lane) & 1ull) atomicOr(&s_mask[wave][64 * k + lane], (uint64_t(mhi) << 32) | mlo);
     }
   }
   __syncthreads();
@@ -1110,14 +1116,23 @@ __global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
       }
   if (any_big) {  // uniform: the large rows, cell by cell of the union
 #pragma unroll
-    for (int k = 0; k < kWords; ++k)
+    for (int k = 0; k < kWords; ++k) {
+      if (uni[k] == 0ull) continue;  // uniform
+      // lane j holds cell 64 k + j's first slot for this wave and its mask
+      const int cj = 64 * k + lane;
+      const bool on = (uni[k] >> lane) & 1ull;
+      const uint32_t sb = on ? s_gbase[cj] + s_wbase[wave][cj] : 0u;
+      const uint64_t sm = on ? s_mask[wave][cj] : 0ull;
       for (uint64_t r = uni[k]; r; r &= r - 1) {
         const int bit = __ffsll((unsigned long long)r) - 1;
+        const uint32_t base_c = uint32_t(__builtin_amdgcn_readlane(sb, bit));
+        const uint64_t m = lane_u64(sm, bit);
         if ((bits[k] >> bit) & 1ull) {
-          const int64_t pos = slot(64 * k + bit);
+          const int64_t pos = int64_t(base_c) + __popcll(m & below);
           if (pos < cap) cell_ids[pos] = int32_t(g);
         }
       }
+    }
   }
 }
 
