@@ -988,7 +988,11 @@ MSPLAT_DEV uint64_t wave_or_u64(uint64_t v) {
   return v;
 }
 
-__global__ __launch_bounds__(kEmitThreads) void emit_cells_kernel(
+// At most 72 SGPRs (78 with VCC and the other reserved registers): two of
+// these 16-wave workgroups then fit on a CU (measured: 78 in all fit two,
+// 86 only one), so the 282 workgroups of the 1080p frame all start at once
+// instead of 26 of them waiting for others to finish.
+__global__ __launch_bounds__(kEmitThreads) __attribute__((amdgpu_num_sgpr(72))) void emit_cells_kernel(
     const uint32_t *__restrict__ order, const int32_t *__restrict__ area_s, const uint2 *__restrict__ rect_s,
     int64_t n, const int64_t *__restrict__ nrows, int64_t cap, Views v, int S, int ncls,
     const uint32_t *__restrict__ ctotal, uint32_t *__restrict__ ticket, uint32_t *__restrict__ status,
