@@ -163,7 +163,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
   L.nlisted = c.take(8);
   L.area_s = c.take(un * 4);  // cell counts and tile rectangles in depth order (cell binning)
   L.rect_s = c.take(un * 8);
-  L.bstatus = c.take(size_t(sortk::prep_blocks(n)) * 8);  // look-back words of the counting pass
+  L.bstatus = c.take(size_t(sortk::prep_blocks(n) + 1) * 8);  // look-back words of the counting pass, its ticket
   L.ctotal = c.take(size_t(sortk::kCtSlices) * sortk::kFusedCells * 4);  // entries per cell (fused emission)
   (void)W;
   (void)H;
@@ -192,7 +192,7 @@ EmitLayout emit_layout(int64_t k, int W, int H) {
   L.sums = c.take(size_t(ns > 0 ? ns : 1) * 8);
   L.ghist = c.take(size_t(sortk::ghist_words()) * 4);
   L.status = c.take(size_t(sortk::status_words(k)) * 4);
-  L.estatus = c.take(size_t(emit_blocks(k)) * 8);  // look-back words of the cell-list emission
+  L.estatus = c.take(size_t(emit_blocks(k) + 1) * 8);  // look-back words of the cell-list emission, its ticket
   // the fused emission: per-cell look-back words of each workgroup, then its ticket
   L.cstatus = c.take((size_t(emit_blocks(k)) * sortk::kFusedCells + 1) * 4);
   L.total = c.off;
@@ -1197,7 +1197,7 @@ int bin_views_prepare_stages(int stages, const float *uv, const float *depth, co
     uint64_t *bstatus = at<uint64_t>(workspace, L.bstatus);
     uint32_t *ctotal = fused_emit(vw) ? at<uint32_t>(workspace, L.ctotal) : nullptr;
     launch_fill_zero_words({{rb.ghist, sortk::ghist_words()},
-                            {bstatus, 2 * sortk::prep_blocks(m)},
+                            {bstatus, 2 * (sortk::prep_blocks(m) + 1)},
                             {nlisted, 2},
                             {d_count, 4},
                             {ctotal, int64_t(sortk::kCtSlices) * sortk::kFusedCells}},
@@ -1290,7 +1290,7 @@ int msplat_bin_views_emit(const float *uv, const float *depth, const int32_t *ra
   // emission's look-back words
   launch_fill_zero_words({{cell_range, 2 * int64_t(vw.cell0[B])},
                           {count > 0 ? at<uint32_t>(workspace, L.ghist) : nullptr, sortk::ghist_words()},
-                          {count > 0 ? at<uint64_t>(workspace, L.estatus) : nullptr, 2 * emit_blocks(count)}},
+                          {count > 0 ? at<uint64_t>(workspace, L.estatus) : nullptr, 2 * (emit_blocks(count) + 1)}},
                          s);
   if (count == 0) {
     // no entries: the range kernel only folds the error words

@@ -726,12 +726,16 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
   __shared__ uint32_t s_cnt[kPrepRows][kWaves];
   __shared__ int64_t s_area[kWaves];
   __shared__ uint32_t s_ct[kFusedCells];
-  if (ctotal) {  // uniform
-    s_ct[threadIdx.x] = 0u;
-    __syncthreads();
-  }
+  __shared__ uint32_t s_ticket;
+  // Workgroups take their rows in ticket order (status[nblocks] is the
+  // ticket): a workgroup's look-back waits only on workgroups that have
+  // started, whichever XCD runs them and whatever else shares the GPU.
+  if (threadIdx.x == 0) s_ticket = atomicAdd(reinterpret_cast<uint32_t *>(status + gridDim.x), 1u);
+  if (ctotal) s_ct[threadIdx.x] = 0u;  // uniform
+  __syncthreads();
+  const int64_t blk = s_ticket;
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-  const int64_t base = int64_t(blockIdx.x) * kThreads * kPrepRows + threadIdx.x;
+  const int64_t base = blk * kThreads * kPrepRows + threadIdx.x;
   int a[kPrepRows];
   uint32_t db[kPrepRows];
   uint32_t cr[kPrepRows];  // fused emission: the row's cells (first cell, width - 1, height - 1, ncx - 1)
@@ -752,7 +756,7 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
     ux[it] = in ? uv[2 * i] : 0.f;
     uy[it] = in ? uv[2 * i + 1] : 0.f;
   }
-  const int64_t first = int64_t(blockIdx.x) * kThreads * kPrepRows;
+  const int64_t first = blk * kThreads * kPrepRows;
   const int64_t last_row = (first + kThreads * kPrepRows < n ? first + kThreads * kPrepRows : n) - 1;
   const int vb0 = __builtin_amdgcn_readfirstlane(view_of_row(v, first));
   const int vb1 = __builtin_amdgcn_readfirstlane(view_of_row(v, last_row));
@@ -824,7 +828,7 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
   for (int w = 0; w < kWaves; ++w) ne += s_area[w];
   // this workgroup's counts are published first, so that later workgroups'
   // look-backs find them while this one adds up its rows' cells
-  const int64_t b = blockIdx.x;
+  const int64_t b = blk;
   if (b > 0 && threadIdx.x == 0)
     __hip_atomic_store(status + b, lb_word(kLbAgg, nl, ne), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // the rows' cells
@@ -849,7 +853,7 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
     if (threadIdx.x == 0) {
       s_base = pl;
       if (!ok) atomicOr(error, 1u);
-      if (blockIdx.x == gridDim.x - 1) {
+      if (blk == int64_t(gridDim.x) - 1) {
         *nlisted = pl + nl;
         const int64_t e = pe + ne;
         count[0] = ok ? (e >= kLbField ? kLbField + 1 : e) : -1;
@@ -890,20 +894,27 @@ __global__ __launch_bounds__(kEmitThreads) void emit_direct_kernel(
   // sort's last pass, so every read here is coalesced).  Listed rows have
   // positive depth, so every entry has a cell class.
   __shared__ int64_t s_base;
+  __shared__ uint32_t s_ticket;
   // An expired wait of the counting pass or the depth sort (upstream) left the
   // depth order undefined: nothing here may be read.  The lists stay
   // unwritten; the range kernel flags them.
   if (*upstream) return;  // uniform: the word is final before this launch
-  const int64_t i = int64_t(blockIdx.x) * kEmitThreads + threadIdx.x;
   const int64_t rows = dev_len(nrows, n);  // rows past the listed ones have nothing to emit
+  // Workgroups past the rows leave before taking a ticket (status[nstatus]):
+  // the others take their rows in ticket order, so a look-back waits only on
+  // workgroups that have started.
   if (int64_t(blockIdx.x) * kEmitThreads >= rows || int64_t(blockIdx.x) >= nstatus) return;  // uniform
+  if (threadIdx.x == 0) s_ticket = atomicAdd(reinterpret_cast<uint32_t *>(status + nstatus), 1u);
+  __syncthreads();
+  const int64_t blk = s_ticket;
+  const int64_t i = blk * kEmitThreads + threadIdx.x;
   const int lane = threadIdx.x % kWave;
   const int a = i < rows ? area_s[i] : 0;
   int64_t tot;
   const int64_t ex = block_excl_scan<kEmitThreads>(a, &tot);
   if (threadIdx.x < kWave) {
     int64_t unused, pe;
-    const bool ok = lookback_publish(status, blockIdx.x, 0, tot, unused, pe);
+    const bool ok = lookback_publish(status, blk, 0, tot, unused, pe);
     if (threadIdx.x == 0) {
       s_base = pe;
       if (!ok) atomicOr(error, 1u);
