@@ -29,11 +29,17 @@ constexpr int kItems = 16;
 constexpr int kRadixTile = kThreads * kItems;  // keys per radix workgroup
 // A scatter pass over few tiles (a few hundred thousand keys: less than
 // the chip holds at once) is a chain of latencies, and 16 short waves of 4
-// keys per lane rank a tile sooner than 4 waves of 16.  Over many tiles the
-// pass is bound by throughput, and the 4-wave form does less LDS work per
-// key (per-wave digit counts for 4 waves instead of 16).
-constexpr int kScatterWide = 1024, kScatterNarrow = kThreads;
-constexpr int64_t kScatterWideTiles = 512;  // at most this many tiles: the 16-wave form
+// keys per lane rank a tile sooner than 4 waves of 16.
+constexpr int kScatterWide = 1024;
+// Onesweep (one launch per pass, tiles chained by a decoupled look-back) up
+// to this many tiles of capacity, multi-launch passes (histogram, scan,
+// scatter) beyond.  When every tile starts at once, a tile walks back over
+// the aggregates of all tiles started with it before it finds an inclusive
+// prefix, so look-back reads grow with the square of the tiles in flight:
+// from about 1.3 M keys on, the passes without look-back are faster.  A
+// device length is usually well below its capacity, so a sort with one
+// takes onesweep up to twice this capacity.
+constexpr int64_t kOnesweepTiles = 320;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kThreads * kScanItems;  // elements per scan workgroup
 
@@ -228,7 +234,7 @@ MSPLAT_HD int rect_cells(uint2 r, int S) {
 constexpr int kMaxPasses = 4;
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kValueMask = (1u << 30) - 1u;
 constexpr int64_t kLookbackSpins = int64_t(1) << 16;
-constexpr int kLookbackWidth = 8;
+constexpr int kLookbackWidth = 2;
 
 // ---- expired waits: see wait_expired in common.hpp ------------------------------
 // One thread: out[0] = wait_expired, then clear it when asked.
@@ -492,14 +498,15 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
   radix_hist_kernel<<<dim3(unsigned(nt)), dim3(kThreads), 0, s>>>(kin, cap, dlen, shift, bits, rb.hist, nt);
   const int64_t hlen = int64_t(1 << bits) * nt;
   exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
-  radix_scatter_tile_kernel<false, kScatterNarrow><<<dim3(unsigned(nt)), dim3(kScatterNarrow), 0, s>>>(
+  radix_scatter_tile_kernel<false, kScatterWide><<<dim3(unsigned(nt)), dim3(kScatterWide), 0, s>>>(
       kin, vin, cap, dlen, shift, bits, rb.hist, nt, kout, vout, nullptr, nullptr, nullptr, gather);
 }
 
 // Stable LSD sort of `cap` (or *dlen) pairs over `bits` key bits in npass
 // passes of db bits (db <= 8, npass <= kMaxPasses), ping-ponging a -> b -> a
-// ...; returns which buffer holds the result (0: a, 1: b).  Onesweep needs
-// cap < 2^30 (the look-back words carry 30-bit counts); larger inputs use the
+// ...; returns which buffer holds the result (0: a, 1: b).  Onesweep for at
+// most kOnesweepTiles tiles of capacity (twice that with dlen) below 2^30 keys
+// (the look-back words carry 30-bit counts); other inputs take the
 // multi-launch passes.  vlast, when given, receives the values of the last
 // pass instead of the other buffer.  ghist_zeroed: the caller has zeroed
 // rb.ghist (with other buffers, in one launch).
@@ -508,7 +515,8 @@ inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, in
                       uint32_t *vlast = nullptr, RadixGather gather = {}, bool ghist_zeroed = false) {
   const int db = (bits + npass - 1) / npass;
   const int64_t nt = radix_tiles(cap);
-  const bool onesweep = cap < (int64_t(1) << 30) && npass <= kMaxPasses;
+  const bool onesweep =
+      cap < (int64_t(1) << 30) && npass <= kMaxPasses && nt <= (dlen ? 2 : 1) * kOnesweepTiles;
   if (nt > 0 && onesweep) {
     if (!ghist_zeroed) launch_fill_zero(rb.ghist, ghist_words(), s);
     const int64_t nst = int64_t(npass) * 256 * nt;
@@ -525,12 +533,8 @@ inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, in
     uint32_t *dv = (p == npass - 1 && vlast) ? vlast : (in_b ? va : vb);
     const RadixGather g = p == npass - 1 ? gather : RadixGather{};
     if (nt > 0) {
-      if (onesweep && nt <= kScatterWideTiles)
+      if (onesweep)
         radix_scatter_tile_kernel<true, kScatterWide><<<dim3(unsigned(nt)), dim3(kScatterWide), 0, s>>>(
-            sk, sv, cap, dlen, shift, pb, rb.ghist + p * 256, nt, dk, dv, rb.ghist + kMaxPasses * 256 + p,
-            rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses, g);
-      else if (onesweep)
-        radix_scatter_tile_kernel<true, kScatterNarrow><<<dim3(unsigned(nt)), dim3(kScatterNarrow), 0, s>>>(
             sk, sv, cap, dlen, shift, pb, rb.ghist + p * 256, nt, dk, dv, rb.ghist + kMaxPasses * 256 + p,
             rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses, g);
       else

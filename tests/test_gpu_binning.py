@@ -73,11 +73,22 @@ def test_bins_bitexact_large_boxes(W, H, cell):
 
 
 def test_bins_bitexact_many_tiles():
-    """2.2 M Gaussians: the depth sort's capacity spans more than 512 tiles of
-    4096 keys, so its passes take the 4-wave scatter form (smaller inputs take
-    the 16-wave form, covered above)."""
+    """2.2 M Gaussians: the depth sort's capacity spans 538 tiles of 4096 keys;
+    its length is on the device, so it still takes the one-launch passes with
+    look-back (up to 640 tiles)."""
     _, uv, depth, _, radius, _ = _projected(7, 2_200_000, 640, 480)
     _check_bins(uv, depth, radius, 640, 480, 4)
+
+
+def test_bins_bitexact_multi_launch_sorts():
+    """2.8 M Gaussians on 1200 cells: the depth sort's capacity (684 tiles) and
+    the class sort's millions of entries both exceed what the look-back passes
+    take, so both sorts run as histogram, scan and scatter launches per pass."""
+    _, uv, depth, _, radius, _ = _projected(8, 2_800_000, 640, 480)
+    b = _check_bins(uv, depth, radius, 640, 480, 1)
+    k = b.cell_ids.numel()
+    print(f"2.8 M Gaussians, cell 1: {k} entries", flush=True)
+    assert k > 320 * 4096
 
 
 def test_bins_ties_and_negative_depth():
