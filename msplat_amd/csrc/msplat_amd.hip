@@ -138,6 +138,11 @@ struct PrepLayout {
 // Workgroups (of sortk::kEmitThreads threads) of the cell-list emission over
 // k entries, and so its look-back words; at least one word is always carved.
 // The status buffer is sized for that kernel's workgroup only.
+// Workgroups of a camera-gradient adjoint over n rows: at most kCamGrid.
+unsigned cam_grid(int64_t n) {
+  const unsigned g = grid_1d(n, kGeomThreads);
+  return g < unsigned(kCamGrid) ? g : unsigned(kCamGrid);
+}
 int64_t emit_blocks(int64_t k) { return k > 0 ? (k + sortk::kEmitThreads - 1) / sortk::kEmitThreads : 1; }
 
 // (W, H are part of the size query so that the layout may depend on the
@@ -954,7 +959,7 @@ int msplat_preprocess_backward_ld(const float *xyz, const float *scale, const fl
           "preprocess_backward: null pointer");
   Timed timed_("preprocess_bwd", s);
   if (cam)
-    preprocess_bwd_kernel<true><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+    preprocess_bwd_kernel<true><<<cam_grid(n), kGeomThreads, 0, s>>>(
         xyz, scale, rotate, n, intr, extr, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
         dL_drotate, dL_dintr, dL_dextr, ld_ddepth);
   else
@@ -1630,7 +1635,7 @@ int msplat_preprocess_views_backward(const float *xyz, const float *scale, const
           "preprocess_backward: null pointer");
   Timed timed_("preprocess_bwd", s);
   if (cam)
-    preprocess_bwd_views_kernel<true><<<grid_1d(n, kGeomThreads), kGeomThreads, 0, s>>>(
+    preprocess_bwd_views_kernel<true><<<cam_grid(n), kGeomThreads, 0, s>>>(
         xyz, scale, rotate, n, intr, extr, B, depth, radius, dL_duv, dL_ddepth, dL_dconic, dL_dxyz, dL_dscale,
         dL_drotate, dL_dintr, dL_dextr);
   else

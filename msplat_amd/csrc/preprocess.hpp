@@ -312,6 +312,10 @@ MSPLAT_DEV bool ewa_adjoint(const Camera &c, float px, float py, float pz, const
 
 // ---- camera-gradient reduction -----------------------------------------------
 // 16 partials per thread -> one atomic per value per workgroup.
+// Workgroups of the camera-gradient adjoints: the grid is capped here and the
+// rows strided over it.
+constexpr int64_t kCamGrid = 2048;
+
 MSPLAT_DEV void reduce_camera(float (&cam)[16], float *dintr, float *dextr) {
   __shared__ float red[(kGeomThreads / kWave) * 16];
   const int lane = threadIdx.x % kWave;
@@ -324,7 +328,7 @@ MSPLAT_DEV void reduce_camera(float (&cam)[16], float *dintr, float *dextr) {
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj)
     if (j == jj) red[wave * 16 + 4 * jj + row_value(row)] = tot[jj];
-  __syncthreads();
+  lds_barrier();  // orders LDS only: the rows' stores need not complete
   if (threadIdx.x < 16) {
     float s = 0.f;
 #pragma unroll
@@ -335,7 +339,7 @@ MSPLAT_DEV void reduce_camera(float (&cam)[16], float *dintr, float *dextr) {
       unsafeAtomicAdd(dextr + (threadIdx.x - 4), s);
     }
   }
-  __syncthreads();  // red is reused by the next call (the many-view adjoint reduces once per view)
+  lds_barrier();  // red may be reused by a later call
 }
 
 MSPLAT_DEV float3 ld3(const float *p, int64_t i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
@@ -523,10 +527,9 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_bwd_kernel(
     const int32_t *__restrict__ radius, const float *__restrict__ duv, const float *__restrict__ ddepth,
     const float *__restrict__ dconic, float *__restrict__ dxyz, float *__restrict__ dscale,
     float *__restrict__ drot, float *__restrict__ dintr, float *__restrict__ dextr, int ld_ddepth) {
-  const int64_t i = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
   const Camera c = load_camera(intr, extr);
   float cam[16] = {0.f};
-  if (i < n) {
+  auto row = [&](int64_t i) {
     float dp[3] = {0.f, 0.f, 0.f};
     float de[3] = {0.f, 0.f, 0.f};
     float3 ds = make_float3(0.f, 0.f, 0.f);
@@ -552,8 +555,17 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_bwd_kernel(
     st3(dxyz, i, dp[0] + de[0], dp[1] + de[1], dp[2] + de[2]);
     st3(dscale, i, ds.x, ds.y, ds.z);
     st4(drot, i, dq);
+  };
+  const int64_t first = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
+  if constexpr (kCam) {
+    // a capped grid strides over the rows: each workgroup's one reduction
+    // covers several rows, and fewer workgroups add into the 16 camera
+    // values (atomics on one address are serialised)
+    for (int64_t i = first; i < n; i += int64_t(gridDim.x) * kGeomThreads) row(i);
+    reduce_camera(cam, dintr, dextr);
+  } else if (first < n) {
+    row(first);
   }
-  if constexpr (kCam) reduce_camera(cam, dintr, dextr);
 }
 
 }  // namespace msplat
@@ -604,8 +616,20 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_fwd_views_kernel(
 // Adjoint over views that share one scene (per-view rows b * n + g): thread g
 // sums Gaussian g's gradients over the views (view 0 first) in registers and
 // stores them once; the covariance gradient of every view is summed before
-// the one covariance adjoint; camera partials are reduced per view (dintr +
-// 4 b, dextr + 12 b).
+// the one covariance adjoint.  The view loop is a latency chain (a row's depth
+// decides whether its gradients are read), so it is software-pipelined: while
+// view b is computed, the gradients of view b + 1 and the depth of view b + 2
+// are in flight.  Camera partials (dintr + 4 b, dextr + 12 b): each wave adds
+// its per-view totals into its own LDS slice (no workgroup barrier inside the
+// loop, which would also wait for the loads in flight), and the workgroup adds
+// the waves' slices, wave 0 first, and issues one atomic per view and value
+// at the end -- the same per-workgroup totals as one reduction per view.
+struct ViewGrad {
+  float u0, u1, dd, c0, c1, c2;
+  int32_t r;
+  bool drawn;  // depth != 0: the row was projected
+};
+
 template <bool kCam>
 __global__ __launch_bounds__(kGeomThreads) void preprocess_bwd_views_kernel(
     const float *__restrict__ xyz, const float *__restrict__ scale, const float *__restrict__ rot, int64_t n,
@@ -613,54 +637,110 @@ __global__ __launch_bounds__(kGeomThreads) void preprocess_bwd_views_kernel(
     const int32_t *__restrict__ radius, const float *__restrict__ duv, const float *__restrict__ ddepth,
     const float *__restrict__ dconic, float *__restrict__ dxyz, float *__restrict__ dscale,
     float *__restrict__ drot, float *__restrict__ dintr, float *__restrict__ dextr) {
-  const int64_t i = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
-  const bool live = i < n;
-  float3 p = make_float3(0.f, 0.f, 0.f), s = p;
-  float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (live) {
-    p = ld3(xyz, i);
-    s = ld3(scale, i);
-    q = ld4(rot, i);
+  constexpr int kWaves = kGeomThreads / kWave;
+  __shared__ float camacc[kCam ? kWaves * kMaxViews * 16 : 1];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  if constexpr (kCam) {  // this wave's slice, touched by this wave only until the end
+    for (int k = lane; k < nviews * 16; k += kWave) camacc[wave * kMaxViews * 16 + k] = 0.f;
   }
-  float cv[6];
-  bool have_cov = false;
-  float gp[3] = {0.f, 0.f, 0.f};
-  float gc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  bool seen = false;  // visible in some view
-  for (int b = 0; b < nviews; ++b) {
-    const Camera c = load_camera(intr + 4 * b, extr + 12 * b);
-    float cam[16] = {0.f};
-    const int64_t o = int64_t(b) * n + i;
-    if (live && depth[o] != 0.f) {
-      seen = true;
-      float dp[3] = {0.f, 0.f, 0.f};
-      float de[3] = {0.f, 0.f, 0.f};
-      project_adjoint<kCam>(c, p.x, p.y, p.z, duv[2 * o], duv[2 * o + 1], ddepth ? ddepth[o] : 0.f, dp, cam);
-      if (radius[o] > 0) {
-        if (!have_cov) {
-          cov3d_core(s, q, cv);
-          have_cov = true;
-        }
-        float dc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (ewa_adjoint<kCam>(c, p.x, p.y, p.z, cv, ld3(dconic, o), dc, de, cam)) {
-#pragma unroll
-          for (int k = 0; k < 6; ++k) gc[k] += dc[k];
-        } else {
-          de[0] = de[1] = de[2] = 0.f;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) gp[k] += dp[k] + de[k];
+  // Gaussians strided over a capped grid (each workgroup's camera totals then
+  // cover several rows; every workgroup adds them with one atomic per value)
+  for (int64_t i = int64_t(blockIdx.x) * kGeomThreads + threadIdx.x;
+       i - threadIdx.x < n; i += int64_t(gridDim.x) * kGeomThreads) {
+    // uniform over the workgroup: rows past n take part in the wave reductions
+    const bool live = i < n;
+    float3 p = make_float3(0.f, 0.f, 0.f), s = p;
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live) {
+      p = ld3(xyz, i);
+      s = ld3(scale, i);
+      q = ld4(rot, i);
     }
-    if constexpr (kCam) reduce_camera(cam, dintr ? dintr + 4 * b : nullptr, dextr ? dextr + 12 * b : nullptr);
+    // the gradients of view b's row, read only where it was projected
+    auto load_view = [&](int b, float d) {
+      ViewGrad g{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0, false};
+      if (live && d != 0.f) {
+        const int64_t o = int64_t(b) * n + i;
+        g.drawn = true;
+        g.u0 = duv[2 * o];
+        g.u1 = duv[2 * o + 1];
+        g.dd = ddepth ? ddepth[o] : 0.f;
+        g.r = radius[o];
+        g.c0 = dconic[3 * o];
+        g.c1 = dconic[3 * o + 1];
+        g.c2 = dconic[3 * o + 2];
+      }
+      return g;
+    };
+    auto load_depth = [&](int b) { return live && b < nviews ? depth[int64_t(b) * n + i] : 0.f; };
+    ViewGrad cur = load_view(0, load_depth(0));
+    float dnext = load_depth(1);
+    float cv[6];
+    bool have_cov = false;
+    float gp[3] = {0.f, 0.f, 0.f};
+    float gc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool seen = false;  // visible in some view
+    for (int b = 0; b < nviews; ++b) {
+      // in flight during view b: view b + 1's gradients, view b + 2's depth
+      const ViewGrad nxt = load_view(b + 1, b + 1 < nviews ? dnext : 0.f);
+      dnext = load_depth(b + 2);
+      const Camera c = load_camera(intr + 4 * b, extr + 12 * b);
+      float cam[16] = {0.f};
+      if (cur.drawn) {
+        seen = true;
+        float dp[3] = {0.f, 0.f, 0.f};
+        float de[3] = {0.f, 0.f, 0.f};
+        project_adjoint<kCam>(c, p.x, p.y, p.z, cur.u0, cur.u1, cur.dd, dp, cam);
+        if (cur.r > 0) {
+          if (!have_cov) {
+            cov3d_core(s, q, cv);
+            have_cov = true;
+          }
+          float dc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+          if (ewa_adjoint<kCam>(c, p.x, p.y, p.z, cv, make_float3(cur.c0, cur.c1, cur.c2), dc, de, cam)) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gc[k] += dc[k];
+          } else {
+            de[0] = de[1] = de[2] = 0.f;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gp[k] += dp[k] + de[k];
+      }
+      if constexpr (kCam) {
+        // lane 16 r + j (j < 4) holds the wave total of value 4 j + kRowValue[r]
+        float tot[4];
+        wave_sum_transposed<16>(cam, tot);
+        const int row = lane >> 4, j = lane & 15;
+        float *acc = camacc + wave * kMaxViews * 16 + b * 16;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+          if (j == jj) acc[4 * jj + row_value(row)] += tot[jj];
+      }
+      cur = nxt;
+    }
+    if (live) {
+      float3 ds = make_float3(0.f, 0.f, 0.f);
+      float4 dq = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (seen) cov3d_adjoint(s, q, gc, ds, dq);
+      st3(dxyz, i, gp[0], gp[1], gp[2]);
+      st3(dscale, i, ds.x, ds.y, ds.z);
+      st4(drot, i, dq);
+    }
   }
-  if (live) {
-    float3 ds = make_float3(0.f, 0.f, 0.f);
-    float4 dq = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (seen) cov3d_adjoint(s, q, gc, ds, dq);
-    st3(dxyz, i, gp[0], gp[1], gp[2]);
-    st3(dscale, i, ds.x, ds.y, ds.z);
-    st4(drot, i, dq);
+  if constexpr (kCam) {
+    lds_barrier();  // every wave's slice is complete (orders LDS only)
+    for (int t = threadIdx.x; t < nviews * 16; t += kGeomThreads) {
+      float sum = 0.f;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) sum += camacc[w * kMaxViews * 16 + t];
+      const int b = t / 16, v = t % 16;
+      if (v < 4) {
+        if (dintr) unsafeAtomicAdd(dintr + 4 * b + v, sum);
+      } else if (dextr) {
+        unsafeAtomicAdd(dextr + 12 * b + (v - 4), sum);
+      }
+    }
   }
 }
 
