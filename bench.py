@@ -287,6 +287,26 @@ def _time_steps(fn, steps: int, warmup: int) -> float:
     return 1e3 * (time.perf_counter() - t0) / steps
 
 
+def _tune_views(args, intrs, extrs, sizes) -> dict:
+    """Launch parameters for both sides of a many-view comparison, each measured
+    on this GPU: every view size's single-view configuration (msplat_amd.autotune,
+    used by the loop) and the batch's schedule (views per launch, cell, pixels per
+    lane: autotune_views, used by the batched step).  Returns what was chosen."""
+    from msplat_amd import autotune
+
+    xyz, scale, rotate, opacity, feature = (a.detach() for a in args)
+    chosen = {}
+    for b, (w, h) in enumerate(sizes):
+        if (w, h) not in chosen:
+            cfg, _ = autotune.autotune(xyz, scale, rotate, opacity, feature, intrs[b].detach(), extrs[b].detach(),
+                                       w, h)
+            chosen[(w, h)] = cfg
+    v, vcfg, _ = autotune.autotune_views(xyz, scale, rotate, opacity, feature, intrs.detach(), extrs.detach(), sizes)
+    cells = sorted({c.cell for c in chosen.values()})
+    return dict(single_view_cells=cells, batched=dict(cell=vcfg.cell, ppl_fwd=vcfg.ppl_fwd, ppl_bwd=vcfg.ppl_bwd,
+                                                      views_per_launch=v))
+
+
 def bench_multiview(dev, steps: int = 5, warmup: int = 2) -> dict:
     """N = 2M Gaussians, 16 cameras of seeded sizes in 400x300..1920x1080, C = 3, fwd + bwd (L2) with
     gradients for the Gaussians and all 16 extrinsics: a loop of single-view calls, and one many-view
@@ -319,13 +339,16 @@ def bench_multiview(dev, steps: int = 5, warmup: int = 2) -> dict:
 
     pix = sum(w * h for w, h in sizes)
     log(f"multiview: N={len(P['xyz'])}, {len(sizes)} views, {pix / 1e6:.2f} Mpix per step")
+    tuned = _tune_views(args, intrs, extrs, sizes)
+    log(f"multiview: tuned {tuned}")
     t_loop = _time_steps(loop_step, steps, warmup)
     t_batch = _time_steps(batched_step, steps, warmup)
     log(f"multiview: loop {t_loop:.2f} ms, batched {t_batch:.2f} ms per step")
     return dict(workload="N=2M, 16 views 400x300..1920x1080, C=3, fwd+bwd L2, grads to Gaussians and extrinsics",
                 pixels_per_step=pix, per_view_loop_ms=round(t_loop, 3), batched_ms=round(t_batch, 3),
                 per_view_loop_mpix_s=round(pix / t_loop / 1e3, 2), batched_mpix_s=round(pix / t_batch / 1e3, 2),
-                speedup_batched=round(t_loop / t_batch, 3), graph_capture="disabled (see msplat_amd/graphs.py)")
+                speedup_batched=round(t_loop / t_batch, 3), tuned=tuned,
+                graph_capture="disabled (see msplat_amd/graphs.py)")
 
 
 def bench_batched_1080p(dev, views: int = 8, steps: int = 3, warmup: int = 1) -> dict:
@@ -379,6 +402,8 @@ def bench_batched_1080p(dev, views: int = 8, steps: int = 3, warmup: int = 1) ->
         diff = max(float((x - y).abs().max()) for x, y in zip(a, b))
         del a, b
     pix = 1920 * 1080 * views
+    tuned = _tune_views(args, intrs, extrs, sizes)
+    log(f"1080p x{views}: tuned {tuned}")
     t_plain = _time_steps(plain_step, steps, warmup)
     t_loop = _time_steps(loop_step, steps, warmup)
     t_batch = _time_steps(batched_step, steps, warmup)
@@ -389,7 +414,7 @@ def bench_batched_1080p(dev, views: int = 8, steps: int = 3, warmup: int = 1) ->
                 batched_ms=round(t_batch, 3), batched_mpix_s=round(pix / t_batch / 1e3, 2),
                 plain_per_view_mpix_s=round(pix / t_plain / 1e3, 2),
                 speedup_vs_plain=round(t_plain / t_batch, 3), speedup_vs_loop=round(t_loop / t_batch, 3),
-                max_abs_diff_vs_plain=diff)
+                max_abs_diff_vs_plain=diff, tuned=tuned)
 
 
 def bench_serving(dev, rates, duration: float) -> dict:

@@ -78,7 +78,7 @@ def _views_step(args, intrs, extrs, sizes, bg, v):
 
 
 def autotune_views(xyz, scale, rotate, opacity, feature, intrs, extrs, sizes, bg: float = 0.0,
-                   groups=(1, 2, 4, 8, 16, 32, 64), cells=(8, 16), ppls=((2, 4), (4, 4)), reps: int = 3,
+                   groups=(1, 2, 4, 8, 16, 32, 64), cells=(8, 16, 32), ppls=((2, 4), (4, 4)), reps: int = 3,
                    record: bool = True):
     """The batch schedule of a many-view training step: how many views share
     a launch, and the cell size and pixels per lane of those launches.
