@@ -132,17 +132,18 @@ struct Carver {
 
 struct PrepLayout {
   size_t area, ka, va, kb, vb, offs, hist, sums, ghist, status, flag, nlisted, area_s, rect_s, bstatus, ctotal,
-      total;
+      dtotal, total;
 };
 
-// Workgroups (of sortk::kEmitThreads threads) of the cell-list emission over
-// k entries, and so its look-back words; at least one word is always carved.
-// The status buffer is sized for that kernel's workgroup only.
 // Workgroups of a camera-gradient adjoint over n rows: at most kCamGrid.
 unsigned cam_grid(int64_t n) {
   const unsigned g = grid_1d(n, kGeomThreads);
   return g < unsigned(kCamGrid) ? g : unsigned(kCamGrid);
 }
+
+// Workgroups (of sortk::kEmitThreads threads) of the cell-list emission over
+// k entries, and so its look-back words; at least one word is always carved.
+// The status buffer is sized for that kernel's workgroup only.
 int64_t emit_blocks(int64_t k) { return k > 0 ? (k + sortk::kEmitThreads - 1) / sortk::kEmitThreads : 1; }
 
 // (W, H are part of the size query so that the layout may depend on the
@@ -170,6 +171,7 @@ PrepLayout prep_layout(int64_t n, int W, int H) {
   L.rect_s = c.take(un * 8);
   L.bstatus = c.take(size_t(sortk::prep_blocks(n) + 1) * 8);  // look-back words of the counting pass, its ticket
   L.ctotal = c.take(size_t(sortk::kCtSlices) * sortk::kFusedCells * 4);  // entries per cell (fused emission)
+  L.dtotal = c.take(size_t(sortk::kCtSlices) * sortk::kMaxPasses * 256 * 4);  // depth digit totals, sliced
   (void)W;
   (void)H;
   L.total = c.off;
@@ -1201,17 +1203,23 @@ int bin_views_prepare_stages(int stages, const float *uv, const float *depth, co
     // word, the look-back words, and the counts (which stay 0 when m = 0)
     uint64_t *bstatus = at<uint64_t>(workspace, L.bstatus);
     uint32_t *ctotal = fused_emit(vw) ? at<uint32_t>(workspace, L.ctotal) : nullptr;
+    // a one-launch-per-pass depth sort gets its digit totals (in kCtSlices
+    // slices) and zeroed look-back words from the counting pass: no histogram
+    // launch
+    uint32_t *dtotal = sortk::takes_onesweep(m, 4, true) ? at<uint32_t>(workspace, L.dtotal) : nullptr;
     launch_fill_zero_words({{rb.ghist, sortk::ghist_words()},
                             {bstatus, 2 * (sortk::prep_blocks(m) + 1)},
                             {nlisted, 2},
                             {d_count, 4},
-                            {ctotal, int64_t(sortk::kCtSlices) * sortk::kFusedCells}},
+                            {ctotal, int64_t(sortk::kCtSlices) * sortk::kFusedCells},
+                            {dtotal, int64_t(sortk::kCtSlices) * sortk::kMaxPasses * 256}},
                            s);
     Timed timed_("bin_prepare", s);
     if (m > 0)
       sortk::bin_prepare_kernel<<<unsigned(sortk::prep_blocks(m)), sortk::kThreads, 0, s>>>(
           uv, depth, radius, m, vw, cell, reinterpret_cast<uint2 *>(rect), ka, va, bstatus, nlisted, d_count,
-          sortk::radix_error_word(rb), ctotal);
+          sortk::radix_error_word(rb), ctotal, dtotal, dtotal ? rb.status : nullptr,
+          dtotal ? int64_t(sortk::kMaxPasses) * 256 * sortk::radix_tiles(m) : 0);
   }
   if ((stages & 2) && m > 0) {
     // the listed rows (first *nlisted pairs) by depth; the last pass carries
@@ -1220,7 +1228,10 @@ int bin_views_prepare_stages(int stages, const float *uv, const float *depth, co
                                     at<uint2>(workspace, L.rect_s)};
     uint32_t *kb = at<uint32_t>(workspace, L.kb), *vb = at<uint32_t>(workspace, L.vb);
     Timed timed_("depth_sort", s);
-    sortk::radix_sort(ka, va, kb, vb, m, 32, 4, rb, s, nlisted, nullptr, gather, true);  // a -> b -> a -> b -> a
+    // a -> b -> a -> b -> a; digit totals from the counting pass
+    sortk::radix_sort(ka, va, kb, vb, m, 32, 4, rb, s, nlisted, nullptr, gather, true,
+                      sortk::takes_onesweep(m, 4, true) ? at<uint32_t>(workspace, L.dtotal) : nullptr,
+                      sortk::kCtSlices);
   }
   return launched("bin_prepare");
 }

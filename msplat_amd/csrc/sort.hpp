@@ -232,6 +232,7 @@ MSPLAT_HD int rect_cells(uint2 r, int S) {
 }
 
 constexpr int kMaxPasses = 4;
+constexpr int kMaxTotalSlices = 8;  // slices of a onesweep sort's digit totals
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kValueMask = (1u << 30) - 1u;
 constexpr int64_t kLookbackSpins = int64_t(1) << 16;
 constexpr int kLookbackWidth = 2;
@@ -301,8 +302,8 @@ MSPLAT_DEV uint32_t digit_lookback(uint32_t *status, int64_t tile, int d, uint32
 template <bool kOnesweep, int kScatterThreads>
 __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, int64_t cap,
-    const int64_t *__restrict__ dlen, int shift, int bits, const uint32_t *__restrict__ offsets, int64_t ntiles,
-    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t *__restrict__ ticket,
+    const int64_t *__restrict__ dlen, int shift, int bits, const uint32_t *__restrict__ offsets, int nsl,
+    int64_t ntiles, uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t *__restrict__ ticket,
     uint32_t *__restrict__ status, uint32_t *__restrict__ error, RadixGather gather) {
   const int64_t len = dev_len(dlen, cap);
   int64_t tile = blockIdx.x;
@@ -323,9 +324,13 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
   const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const uint32_t mask = (1u << bits) - 1u;
   const int ndig = 1 << bits;
-  // onesweep: the pass's digit totals do not depend on the tile; their load
-  // stays in flight until the look-back (the barriers before it order LDS only)
-  const uint32_t gtot = (kOnesweep && tid < ndig) ? offsets[tid] : 0u;
+  // onesweep: the pass's digit totals (the sum of nsl slices kMaxPasses x 256
+  // words apart) do not depend on the tile; their loads stay in flight until
+  // the look-back (the barriers before it order LDS only)
+  uint32_t part[kMaxTotalSlices];  // added up where the look-back needs them
+#pragma unroll
+  for (int k = 0; k < kMaxTotalSlices; ++k)
+    part[k] = (kOnesweep && tid < ndig && k < nsl) ? offsets[k * (kMaxPasses * 256) + tid] : 0u;
   if (kOnesweep && tid == 0) s_ticket = atomicAdd(ticket, 1u);
   for (int j = tid; j < kWaves * 256; j += kScatterThreads) s_run[j / 256][j % 256] = 0u;
   if (!kOnesweep && tid < ndig) s_gstart[tid] = offsets[int64_t(tid) * ntiles + tile];
@@ -392,6 +397,9 @@ __global__ __launch_bounds__(kScatterThreads) void radix_scatter_tile_kernel(
   if constexpr (kOnesweep) {
     // this tile's first slot per digit = digits below it in the whole pass
     // (offsets holds the pass's digit totals) + this digit in earlier tiles
+    uint32_t gtot = 0u;
+#pragma unroll
+    for (int k = 0; k < kMaxTotalSlices; ++k) gtot += part[k];
     const uint32_t doff = uint32_t(block_excl_scan<kScatterThreads>(int64_t(gtot), &dummy));
     if (tid < ndig) s_gstart[tid] = doff + digit_lookback(status, tile, tid, tot, error);
   }
@@ -499,7 +507,7 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
   const int64_t hlen = int64_t(1 << bits) * nt;
   exclusive_scan<uint32_t, uint32_t>(rb.hist, nullptr, hlen, rb.sums, rb.hist, nullptr, s);
   radix_scatter_tile_kernel<false, kScatterWide><<<dim3(unsigned(nt)), dim3(kScatterWide), 0, s>>>(
-      kin, vin, cap, dlen, shift, bits, rb.hist, nt, kout, vout, nullptr, nullptr, nullptr, gather);
+      kin, vin, cap, dlen, shift, bits, rb.hist, 1, nt, kout, vout, nullptr, nullptr, nullptr, gather);
 }
 
 // Stable LSD sort of `cap` (or *dlen) pairs over `bits` key bits in npass
@@ -510,14 +518,22 @@ inline void radix_pass(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
 // multi-launch passes.  vlast, when given, receives the values of the last
 // pass instead of the other buffer.  ghist_zeroed: the caller has zeroed
 // rb.ghist (with other buffers, in one launch).
+inline bool takes_onesweep(int64_t cap, int npass, bool has_dlen) {
+  return cap < (int64_t(1) << 30) && npass <= kMaxPasses && radix_tiles(cap) <= (has_dlen ? 2 : 1) * kOnesweepTiles;
+}
+
+// totals: the caller has stored the onesweep digit totals there, in nslices
+// slices of kMaxPasses x 256 words that add up to them, and zeroed the
+// look-back words: no histogram launch (ignored when the sort takes the
+// multi-launch passes).
 inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, int64_t cap, int bits, int npass,
                       const RadixBuffers &rb, hipStream_t s, const int64_t *dlen = nullptr,
-                      uint32_t *vlast = nullptr, RadixGather gather = {}, bool ghist_zeroed = false) {
+                      uint32_t *vlast = nullptr, RadixGather gather = {}, bool ghist_zeroed = false,
+                      const uint32_t *totals = nullptr, int nslices = 1) {
   const int db = (bits + npass - 1) / npass;
   const int64_t nt = radix_tiles(cap);
-  const bool onesweep =
-      cap < (int64_t(1) << 30) && npass <= kMaxPasses && nt <= (dlen ? 2 : 1) * kOnesweepTiles;
-  if (nt > 0 && onesweep) {
+  const bool onesweep = takes_onesweep(cap, npass, dlen != nullptr);
+  if (nt > 0 && onesweep && !totals) {
     if (!ghist_zeroed) launch_fill_zero(rb.ghist, ghist_words(), s);
     const int64_t nst = int64_t(npass) * 256 * nt;
     const unsigned hb = unsigned(imin(int(nt), 1024));
@@ -535,8 +551,9 @@ inline int radix_sort(uint32_t *ka, uint32_t *va, uint32_t *kb, uint32_t *vb, in
     if (nt > 0) {
       if (onesweep)
         radix_scatter_tile_kernel<true, kScatterWide><<<dim3(unsigned(nt)), dim3(kScatterWide), 0, s>>>(
-            sk, sv, cap, dlen, shift, pb, rb.ghist + p * 256, nt, dk, dv, rb.ghist + kMaxPasses * 256 + p,
-            rb.status + int64_t(p) * 256 * nt, rb.ghist + kMaxPasses * 256 + kMaxPasses, g);
+            sk, sv, cap, dlen, shift, pb, (totals ? totals : rb.ghist) + p * 256, totals ? nslices : 1, nt, dk, dv,
+            rb.ghist + kMaxPasses * 256 + p, rb.status + int64_t(p) * 256 * nt,
+            rb.ghist + kMaxPasses * 256 + kMaxPasses, g);
       else
         radix_pass(sk, sv, dk, dv, cap, shift, pb, rb, s, dlen, g);
     }
@@ -711,6 +728,11 @@ MSPLAT_HD int64_t prep_blocks(int64_t n) {
 // loads of all its rounds are in flight together, and a few hundred
 // workgroups (all resident at once) keep the look-back short.  Ranks follow
 // row order: round, then wave, then lane.
+// With dhist, it also adds the 8-bit digit counts of the listed depth keys
+// (4 passes x 256) to dhist -- the onesweep depth sort's digit totals, in
+// kCtSlices slices (workgroup b adds to slice b mod kCtSlices: the top digits'
+// few bins would otherwise take every workgroup's atomic on one address) --
+// and zeroes that sort's ndstatus look-back words in dstatus.
 // With ctotal (at most kFusedCells cells), it also adds every cell's entry
 // count to ctotal[k * kFusedCells + cell] (zero before the launch), for the
 // fused emission: kCtSlices slices, workgroup b adding to slice b mod
@@ -718,24 +740,35 @@ MSPLAT_HD int64_t prep_blocks(int64_t n) {
 // addresses (the emission adds the slices up).
 constexpr int kFusedCells = 256;
 constexpr int kCtSlices = 8;
+static_assert(kCtSlices <= kMaxTotalSlices, "the depth sort reads at most kMaxTotalSlices slices of digit totals");
 
 __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
     const float *__restrict__ uv, const float *__restrict__ depth, const int32_t *__restrict__ radius, int64_t n,
     Views v, int S, uint2 *__restrict__ rect, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
     uint64_t *__restrict__ status, int64_t *__restrict__ nlisted, int64_t *__restrict__ count,
-    uint32_t *__restrict__ error, uint32_t *__restrict__ ctotal) {
+    uint32_t *__restrict__ error, uint32_t *__restrict__ ctotal, uint32_t *__restrict__ dhist,
+    uint32_t *__restrict__ dstatus, int64_t ndstatus) {
   static_assert(kThreads == kFusedCells, "one thread per cell adds the cell counts");
+  static_assert(kThreads == 256, "one thread per digit adds the digit totals");
   constexpr int kWaves = kThreads / kWave;
   __shared__ int64_t s_base;
   __shared__ uint32_t s_cnt[kPrepRows][kWaves];
   __shared__ int64_t s_area[kWaves];
   __shared__ uint32_t s_ct[kFusedCells];
+  __shared__ uint32_t s_dh[kMaxPasses][256];
   __shared__ uint32_t s_ticket;
   // Workgroups take their rows in ticket order (status[nblocks] is the
   // ticket): a workgroup's look-back waits only on workgroups that have
   // started, whichever XCD runs them and whatever else shares the GPU.
   if (threadIdx.x == 0) s_ticket = atomicAdd(reinterpret_cast<uint32_t *>(status + gridDim.x), 1u);
   if (ctotal) s_ct[threadIdx.x] = 0u;  // uniform
+  if (dhist) {  // uniform
+#pragma unroll
+    for (int p = 0; p < kMaxPasses; ++p) s_dh[p][threadIdx.x] = 0u;
+    // the depth sort's look-back words, zeroed for its passes
+    for (int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x; j < ndstatus; j += int64_t(gridDim.x) * kThreads)
+      dstatus[j] = 0u;
+  }
   __syncthreads();
   const int64_t blk = s_ticket;
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
@@ -846,6 +879,14 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
         for (int x = 0; x < w; ++x) atomicAdd(&s_ct[c0 + y * ncx + x], 1u);
     }
   }
+  if (dhist) {  // uniform: the listed depth keys' 8-bit digits, for the depth sort's passes
+#pragma unroll
+    for (int it = 0; it < kPrepRows; ++it) {
+      if (a[it] <= 0) continue;
+#pragma unroll
+      for (int p = 0; p < kMaxPasses; ++p) atomicAdd(&s_dh[p][(db[it] >> (8 * p)) & 0xffu], 1u);
+    }
+  }
   {
     // the whole workgroup looks back (489 workgroups at 1 M rows: two rounds)
     int64_t pl = 0, pe = 0;
@@ -876,6 +917,12 @@ __global__ __launch_bounds__(kThreads) void bin_prepare_kernel(
   }
   if (ctotal && s_ct[threadIdx.x])  // s_ct is final: the barrier above
     atomicAdd(&ctotal[(blockIdx.x % kCtSlices) * kFusedCells + threadIdx.x], s_ct[threadIdx.x]);
+  if (dhist) {  // uniform; s_dh is final: the barrier above
+#pragma unroll
+    for (int p = 0; p < kMaxPasses; ++p)
+      if (s_dh[p][threadIdx.x])
+        atomicAdd(&dhist[(blockIdx.x % kCtSlices) * (kMaxPasses * 256) + p * 256 + threadIdx.x], s_dh[p][threadIdx.x]);
+  }
 }
 
 // Cell binning, emission (one launch): row i of the depth order writes its
