@@ -114,8 +114,9 @@ def algorithmic_work(name, w):
         # every row: mask byte in, dL/dshs out; projected rows: centre and dL/dvalue in, the geometry's xyz
         # gradient in and out (the SH part added); rows with a non-zero gradient: coefficients in
         "sh_bwd": ("hbm", n * (1 + 4 * Csh * D) + npj * (12 + 4 * Csh + 24) + ng * 4 * Csh * D),
-        # the binned Gaussians only: one key read for all digit histograms, 4 x (pair read + pair write)
-        "depth_sort": ("hbm", w["nvis"] * (4 + 4 * 16)),
+        # the binned Gaussians only (their digit counts come with the counting pass): 4 x (pair read +
+        # pair write), and the last pass gathers each one's rectangle (8 B) into depth order with its cell count
+        "depth_sort": ("hbm", w["nvis"] * (4 * 16 + 8 + 8 + 4)),
         # uv, depth, radius in; packed rectangle out; the listed rows' depth keys and ids out
         "bin_prepare": ("hbm", w["n"] * (8 + 4 + 4 + 8) + w["nvis"] * 8),
         "emit": ("hbm", w["count"] * 8 + w["nvis"] * 28),
