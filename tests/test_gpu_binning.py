@@ -83,12 +83,16 @@ def test_bins_bitexact_many_tiles():
 def test_bins_bitexact_multi_launch_sorts():
     """2.8 M Gaussians on 1200 cells: the depth sort's capacity (684 tiles) and
     the class sort's millions of entries both exceed what the look-back passes
-    take, so both sorts run as histogram, scan and scatter launches per pass."""
+    take, so both sorts run as histogram, scan and scatter launches per pass.
+    A capacity below the count writes only slots below it."""
     _, uv, depth, _, radius, _ = _projected(8, 2_800_000, 640, 480)
     b = _check_bins(uv, depth, radius, 640, 480, 1)
     k = b.cell_ids.numel()
     print(f"2.8 M Gaussians, cell 1: {k} entries", flush=True)
     assert k > 320 * 4096
+    short = binning.bin_gaussians(uv, depth, 640, 480, radius, 1, capacity=k // 3)
+    assert short.overflowed() and int(short.count[0].item()) == k
+    assert int(short.cell_range.max()) <= k // 3
 
 
 def test_bins_ties_and_negative_depth():
