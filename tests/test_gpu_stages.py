@@ -209,8 +209,10 @@ def test_sort_kat():
 
 
 @pytest.mark.parametrize("seed,W,H,n", [(0, 128, 96, 3000), (1, 640, 480, 20000), (2, 4000, 50, 5000),
-                                        (3, 33, 17, 200)])
+                                        (3, 33, 17, 200), (4, 640, 480, 60000)])
 def test_sort_bitexact(seed, W, H, n):
+    """The last case has millions of pairs: the class sort takes the multi-launch passes (more than 320
+    tiles of 4096 keys)."""
     rng = np.random.default_rng(seed)
     uv = rng.uniform(-40, max(W, H) + 40, size=(n, 2)).astype(np.float32)
     depth = rng.choice(np.float32([-2.5, -1.0, 0.5, 1.0, 1.0, 2.0, 3.25]), n) * rng.choice([1, 1.0001], n)
@@ -224,6 +226,8 @@ def test_sort_bitexact(seed, W, H, n):
     assert_bitexact(np_(idx), oidx, "idx_sorted")
     assert_bitexact(np_(trange), orng, "tile_range")
     assert (okeys < 0).any()
+    if n == 60000:
+        assert len(okeys) > 320 * 4096, len(okeys)
     # the step API: reference-order keys, sorted by torch, ranges from keys
     area = np.asarray(osort.gaussian_keys(uv, depth, radius, W, H)[2], np.int32)
     cums = torch.from_numpy(np.cumsum(area).astype(np.int32)).to(DEV)
